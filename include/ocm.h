@@ -1,0 +1,230 @@
+/*
+ * ocm.h -- C ABI of the MI355X MD step for OpenCafeMol force fields.
+ *
+ * This ABI replaces the OpenMM objects the reference builds and steps:
+ *   - OpenMM::System / OpenMM::Force, produced by SystemGenerator::generate
+ *     (src/SystemGenerator.cpp:34-111) and ForceFieldGeneratorBase::generate
+ *     (src/forcefield/ForceFieldGeneratorBase.hpp:9-16), become the plain
+ *     descriptors ocm_system_desc / ocm_force_desc below;
+ *   - OpenMM::LangevinIntegrator, produced by LangevinIntegratorGenerator
+ *     (src/IntegratorGenerator.cpp:5-14), becomes ocm_integrator_desc;
+ *   - OpenMM::Context (src/Simulator.cpp:17), its setPositions (:41),
+ *     setVelocities[ToTemperature] (src/Simulator.hpp:49-65), Integrator::step
+ *     (src/Simulator.cpp:60) and getState (src/Observer.cpp:34,107,113,339-344)
+ *     become ocm_context_* below.
+ *
+ * Units are OpenMM's: nm, ps, amu, kJ/mol, elementary charge, K.
+ * Every function returns an int status (OCM_OK on success); no C++ exception
+ * crosses this boundary.  ocm_last_error() returns a thread-local message for
+ * the last failing call on the calling thread.
+ *
+ * Ownership: descriptor arrays are copied at create time; the caller may free
+ * them afterwards.  Output arrays are caller-allocated.  One context may be
+ * driven by one host thread at a time.
+ */
+#ifndef OCM_H
+#define OCM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define OCM_OK               0
+#define OCM_ERR_INVALID      1  /* malformed descriptor or argument            */
+#define OCM_ERR_OOM          2  /* device allocation failed / HBM budget hit   */
+#define OCM_ERR_HIP          3  /* HIP runtime error                           */
+#define OCM_ERR_NUMERIC      4  /* NaN or fixed-point force overflow detected  */
+#define OCM_ERR_NO_DEVICE    5  /* no usable gfx950 device                     */
+#define OCM_ERR_STATE        6  /* call not valid in the current state         */
+
+/* ----------------------------------------------------------- force kinds */
+/* Bonded kinds: `indices` holds n_terms * arity bead indices and `params`
+ * holds n_terms * nparams doubles, in the order given.                     */
+enum ocm_force_kind {
+    /* E = 1/2 k (r - r0)^2 ; params {r0, k}
+     * HarmonicBondForceFieldGenerator.cpp:3-14 (OpenMM HarmonicBondForce). */
+    OCM_HARMONIC_BOND = 1,
+    /* E = k exp(-(r - v0)^2 / (2 sigma^2)) ; params {k, v0, sigma}
+     * GaussianBondForceFieldGenerator.cpp:5-20.                             */
+    OCM_GAUSSIAN_BOND = 2,
+    /* E = k (5 (r0/r)^12 - 6 (r0/r)^10) ; params {k, r0}
+     * GoContactForceFieldGenerator.cpp:5-17.                                */
+    OCM_GO_CONTACT = 3,
+    /* E = 1/2 k (theta - theta0)^2, theta at the middle bead ;
+     * params {theta0, k}.  HarmonicAngleForceFieldGenerator.cpp:3-15.      */
+    OCM_HARMONIC_ANGLE = 4,
+    /* E = k (spline(theta) - Emin + step(tmin-theta)(30(tmin-theta)+y0)
+     *        + step(theta-tmax)(30(theta-tmax)+y9)) ; params {k} ;
+     * globals {y0..y9, tmin, tmax, Emin}; spline is the natural cubic
+     * spline through (tmin + i (tmax-tmin)/9, y_i), zero outside [tmin,tmax].
+     * FlexibleLocalAngleForceFieldGenerator.cpp:3-45.                       */
+    OCM_FLEXIBLE_LOCAL_ANGLE = 5,
+    /* E = k exp(-d^2/(2 sigma^2)), d = (phi - phi0) wrapped to [-pi, pi) ;
+     * params {k, phi0, sigma}.  GaussianDihedralForceFieldGenerator.cpp:3-27 */
+    OCM_GAUSSIAN_DIHEDRAL = 6,
+    /* E = k (c - Emin + sum_n a_n cos(n phi) + b_n sin(n phi)), n = 1..3 ;
+     * params {k} ; globals {c, a1, b1, a2, b2, a3, b3, Emin}.
+     * FlexibleLocalDihedralForceFieldGenerator.cpp:5-63.                    */
+    OCM_FLEXIBLE_LOCAL_DIHEDRAL = 7,
+
+    /* Nonbonded kinds: `particle_params` holds n_particles * npp doubles.
+     * Energy is zero at r >= cutoff (OpenMM CutoffNonPeriodic/Periodic).   */
+    /* E = eps (s_i + s_j)^12 (r^-12 - rc^-12) ; npp 1 {radius} ;
+     * globals {eps}.  ExcludedVolumeForceFieldGenerator.cpp:23-92.          */
+    OCM_EXCLUDED_VOLUME = 16,
+    /* E = q_i q_j K (exp(-r/lambda)/r - corr) ; npp 1 {charge} ;
+     * globals {K = 1/(4 pi eps0 epsk), lambda, corr}.
+     * DebyeHuckelForceFieldGenerator.cpp:10-108.                            */
+    OCM_DEBYE_HUCKEL = 17,
+    /* E = step(r*-r) e (4 (s^12 - s^6) + 1), s = sig/r, r* = 2^(1/6) sig,
+     * sig = (sig_i+sig_j)/2, e = (e_i+e_j)/2 ; npp 2 {epsilon, sigma}.
+     * LennardJonesRepulsiveForceFieldGenerator.cpp:34-112.                  */
+    OCM_LJ_REPULSIVE = 18,
+    /* E = e (step(r-r*) 4 (s^12 - s^6) - step(r*-r)) * S(r), S the OpenMM
+     * switching function from switch_distance to cutoff ; npp 2 {epsilon,
+     * sigma}.  LennardJonesAttractiveForceFieldGenerator.cpp:38-121.        */
+    OCM_LJ_ATTRACTIVE = 19
+};
+
+/* One force (one reference ForceFieldGenerator::generate() result). */
+typedef struct ocm_force_desc {
+    int32_t        kind;            /* enum ocm_force_kind                    */
+    int32_t        group;           /* force group id, 0..31                  */
+    int32_t        use_periodic;    /* minimum-image displacements            */
+    int32_t        n_terms;         /* bonded: number of terms                */
+    const int32_t* indices;         /* bonded: n_terms * arity                */
+    const double*  params;          /* bonded: n_terms * nparams              */
+    int32_t        n_globals;       /* count of `globals`                     */
+    int32_t        n_categories;    /* nonbonded: categories in use (>=1) or 0 */
+    const double*  globals;         /* kind-specific force-level constants    */
+    /* ---- nonbonded only ---- */
+    const double*  particle_params; /* n_particles * npp                      */
+    const int32_t* category;        /* n_particles; -1 = not a participant;
+                                       NULL = every bead is category 0        */
+    const uint8_t* category_allow;  /* n_categories^2 symmetric 0/1 matrix;
+                                       NULL = every category pair interacts   */
+    double         cutoff;          /* nm                                     */
+    double         switch_distance; /* nm; <= 0 disables switching            */
+    int64_t        n_exclusions;
+    const int32_t* exclusions;      /* n_exclusions * 2 bead indices          */
+} ocm_force_desc;
+
+typedef struct ocm_system_desc {
+    int32_t               n_particles;
+    int32_t               n_forces;
+    const double*         masses;       /* amu; 0 = immobile bead            */
+    int32_t               use_periodic_box;
+    int32_t               reserved;
+    double                box[3];       /* nm, cuboid with a corner at origin */
+    const ocm_force_desc* forces;
+} ocm_system_desc;
+
+/* Langevin thermostat (LangevinIntegratorGenerator, IntegratorGenerator.hpp:19-42).
+ * Integrated with the BAOAB splitting; noise from counter-based
+ * Philox4x32-10 keyed by (seed, bead index) and counted by step.          */
+typedef struct ocm_integrator_desc {
+    double   temperature;  /* K     */
+    double   friction;     /* 1/ps  */
+    double   dt;           /* ps    */
+    uint64_t seed;
+} ocm_integrator_desc;
+
+typedef struct ocm_options {
+    int32_t device;            /* HIP device ordinal                         */
+    int32_t rebuild_interval;  /* steps between neighbor-list rebuilds (>=1) */
+    double  skin;              /* nm, neighbor-list buffer beyond cutoff     */
+    int32_t team_size;         /* lanes per pair-list row: 4,8,16,32,64; 0 = auto */
+    int32_t use_graph;         /* capture step chunks as hipGraphs           */
+    int32_t chunk_steps;       /* steps between host checks of device flags  */
+    int32_t reserved;
+} ocm_options;
+
+/* getState selection bits */
+#define OCM_STATE_POSITIONS   1
+#define OCM_STATE_VELOCITIES  2
+#define OCM_STATE_FORCES      4
+#define OCM_STATE_ENERGY      8
+
+typedef struct ocm_state {
+    double* positions;      /* n*3 or NULL                                     */
+    double* velocities;     /* n*3 or NULL                                     */
+    double* forces;         /* n*3 or NULL (kJ/mol/nm)                         */
+    double  energy_by_group[32]; /* kJ/mol, filled when ENERGY requested       */
+    double  potential_energy;    /* sum over the requested groups              */
+    double  kinetic_energy;
+    int64_t step;           /* steps taken since creation                      */
+} ocm_state;
+
+typedef struct ocm_context ocm_context;
+
+/* Fills `opt` with defaults (skin 0.3 nm, rebuild every 20 steps, ...). */
+void ocm_options_default(ocm_options* opt);
+
+/* OpenMM::Context(system, integrator, platform), src/Simulator.cpp:17.
+ * Uploads the system to the device; positions start at the origin until
+ * ocm_set_positions is called.                                             */
+int ocm_context_create(const ocm_system_desc* system,
+                       const ocm_integrator_desc* integrator,
+                       const ocm_options* options, ocm_context** out);
+int ocm_context_destroy(ocm_context* ctx);
+
+/* Context::setPositions, src/Simulator.cpp:41. xyz is n*3 nm. */
+int ocm_set_positions(ocm_context* ctx, const double* xyz, int32_t n);
+/* Context::setVelocities, src/Simulator.hpp:62-65. xyz is n*3 nm/ps. */
+int ocm_set_velocities(ocm_context* ctx, const double* xyz, int32_t n);
+/* Context::setVelocitiesToTemperature, src/Simulator.hpp:49-60: Maxwell-
+ * Boltzmann velocities from Philox stream 1 keyed by (seed, bead).       */
+int ocm_set_velocities_to_temperature(ocm_context* ctx, double temperature,
+                                      uint64_t seed);
+/* Integrator::step(n), src/Simulator.cpp:60. */
+int ocm_step(ocm_context* ctx, int64_t nsteps);
+/* Context::getState(types, false, groups), src/Observer.cpp:336-344.
+ * Forces and energies are restricted to the groups in `groups`.          */
+int ocm_get_state(ocm_context* ctx, int32_t what, uint32_t groups,
+                  ocm_state* out);
+
+/* Neighbor pairs (bead index pairs i<j, r < cutoff) that the device list of
+ * nonbonded force `force_index` interacts through, at the current positions.
+ * Writes up to `cap` pairs to out_pairs (2 ints each); *count gets the total. */
+int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index,
+                           int32_t* out_pairs, int64_t cap, int64_t* count);
+
+/* Per-kernel timing gathered with HIP events on the context's stream.
+ * kernel: 0 = pair forces, 1 = bonded forces, 2 = integrator,
+ *         3 = list build, 4 = record pack.                                 */
+int ocm_set_timing(ocm_context* ctx, int32_t enable);
+int ocm_get_timing(ocm_context* ctx, int32_t kernel, double* total_ms,
+                   int64_t* launches);
+
+/* Work figures of the current neighbor lists: listed (directed) entries and
+ * member beads summed over all pair groups.                                */
+int ocm_get_list_stats(ocm_context* ctx, int64_t* list_entries,
+                       int64_t* members, int64_t* rebuilds);
+
+/* ------------------------------------------------------------------ batch */
+/* Many independent systems packed into one set of device arrays; every
+ * stage is one launch over all of them.  A context is a batch of one.     */
+int ocm_batch_create(const ocm_system_desc* systems,
+                     const ocm_integrator_desc* integrators, int32_t n_systems,
+                     const ocm_options* options, ocm_context** out);
+int ocm_batch_set_positions(ocm_context* ctx, int32_t system,
+                            const double* xyz, int32_t n);
+int ocm_batch_set_velocities(ocm_context* ctx, int32_t system,
+                             const double* xyz, int32_t n);
+int ocm_batch_get_state(ocm_context* ctx, int32_t system, int32_t what,
+                        uint32_t groups, ocm_state* out);
+/* Per-system error status after the last ocm_step (OCM_OK or OCM_ERR_NUMERIC). */
+int ocm_batch_system_status(ocm_context* ctx, int32_t system);
+int32_t ocm_batch_size(ocm_context* ctx);
+
+const char* ocm_last_error(void);
+const char* ocm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCM_H */

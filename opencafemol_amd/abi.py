@@ -1,0 +1,143 @@
+"""ctypes mirror of include/ocm.h and the loader of the HIP library.
+
+The loader never falls back to anything: if ``libocm.so`` has not been built
+it raises, so a caller on a GPU box cannot silently run something else.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libocm.so")
+
+OCM_OK = 0
+OCM_ERR_INVALID = 1
+OCM_ERR_OOM = 2
+OCM_ERR_HIP = 3
+OCM_ERR_NUMERIC = 4
+OCM_ERR_NO_DEVICE = 5
+OCM_ERR_STATE = 6
+
+HARMONIC_BOND = 1
+GAUSSIAN_BOND = 2
+GO_CONTACT = 3
+HARMONIC_ANGLE = 4
+FLEXIBLE_LOCAL_ANGLE = 5
+GAUSSIAN_DIHEDRAL = 6
+FLEXIBLE_LOCAL_DIHEDRAL = 7
+EXCLUDED_VOLUME = 16
+DEBYE_HUCKEL = 17
+LJ_REPULSIVE = 18
+LJ_ATTRACTIVE = 19
+
+BONDED_ARITY = {1: 2, 2: 2, 3: 2, 4: 3, 5: 3, 6: 4, 7: 4}
+BONDED_NPARAMS = {1: 2, 2: 3, 3: 2, 4: 2, 5: 1, 6: 3, 7: 1}
+PAIR_NPP = {16: 1, 17: 1, 18: 2, 19: 2}
+
+STATE_POSITIONS = 1
+STATE_VELOCITIES = 2
+STATE_FORCES = 4
+STATE_ENERGY = 8
+
+# Every symbol include/ocm.h declares.
+EXPORTED = [
+    "ocm_options_default", "ocm_context_create", "ocm_context_destroy", "ocm_set_positions",
+    "ocm_set_velocities", "ocm_set_velocities_to_temperature", "ocm_step", "ocm_get_state",
+    "ocm_get_neighbor_pairs", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
+    "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
+    "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
+    "ocm_version",
+]
+
+
+class ForceDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("group", C.c_int32), ("use_periodic", C.c_int32),
+        ("n_terms", C.c_int32), ("indices", C.POINTER(C.c_int32)),
+        ("params", C.POINTER(C.c_double)), ("n_globals", C.c_int32),
+        ("n_categories", C.c_int32), ("globals", C.POINTER(C.c_double)),
+        ("particle_params", C.POINTER(C.c_double)), ("category", C.POINTER(C.c_int32)),
+        ("category_allow", C.POINTER(C.c_uint8)), ("cutoff", C.c_double),
+        ("switch_distance", C.c_double), ("n_exclusions", C.c_int64),
+        ("exclusions", C.POINTER(C.c_int32)),
+    ]
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int32), ("n_forces", C.c_int32), ("masses", C.POINTER(C.c_double)),
+        ("use_periodic_box", C.c_int32), ("reserved", C.c_int32), ("box", C.c_double * 3),
+        ("forces", C.POINTER(ForceDesc)),
+    ]
+
+
+class IntegratorDesc(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("friction", C.c_double), ("dt", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class Options(C.Structure):
+    _fields_ = [("device", C.c_int32), ("rebuild_interval", C.c_int32), ("skin", C.c_double),
+                ("team_size", C.c_int32), ("use_graph", C.c_int32), ("chunk_steps", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class State(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_double)), ("velocities", C.POINTER(C.c_double)),
+                ("forces", C.POINTER(C.c_double)), ("energy_by_group", C.c_double * 32),
+                ("potential_energy", C.c_double), ("kinetic_energy", C.c_double),
+                ("step", C.c_int64)]
+
+
+class OcmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ocm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libocm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"HIP library {path} is missing: build it with `python -m opencafemol_amd.build`")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sig = {
+        "ocm_options_default": (None, [P(Options)]),
+        "ocm_context_create": (C.c_int, [P(SystemDesc), P(IntegratorDesc), P(Options), P(C.c_void_p)]),
+        "ocm_batch_create": (C.c_int, [P(SystemDesc), P(IntegratorDesc), C.c_int32, P(Options), P(C.c_void_p)]),
+        "ocm_context_destroy": (C.c_int, [C.c_void_p]),
+        "ocm_set_positions": (C.c_int, [C.c_void_p, P(C.c_double), C.c_int32]),
+        "ocm_set_velocities": (C.c_int, [C.c_void_p, P(C.c_double), C.c_int32]),
+        "ocm_batch_set_positions": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), C.c_int32]),
+        "ocm_batch_set_velocities": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), C.c_int32]),
+        "ocm_set_velocities_to_temperature": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64]),
+        "ocm_step": (C.c_int, [C.c_void_p, C.c_int64]),
+        "ocm_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(State)]),
+        "ocm_batch_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, P(State)]),
+        "ocm_get_neighbor_pairs": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), C.c_int64, P(C.c_int64)]),
+        "ocm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+        "ocm_get_timing": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_int64)]),
+        "ocm_get_list_stats": (C.c_int, [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64)]),
+        "ocm_batch_system_status": (C.c_int, [C.c_void_p, C.c_int32]),
+        "ocm_batch_size": (C.c_int32, [C.c_void_p]),
+        "ocm_last_error": (C.c_char_p, []),
+        "ocm_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != OCM_OK:
+        msg = _lib.ocm_last_error().decode() if _lib is not None else ""
+        raise OcmError(code, msg)

@@ -1,0 +1,922 @@
+// engine.cpp -- packing of systems into device arrays and the chunked step loop.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <set>
+#include <sstream>
+
+namespace ocm {
+
+// ------------------------------------------------------------ kind tables
+bool is_bonded(int k) { return k >= OCM_HARMONIC_BOND && k <= OCM_FLEXIBLE_LOCAL_DIHEDRAL; }
+bool is_pair(int k) { return k >= OCM_EXCLUDED_VOLUME && k <= OCM_LJ_ATTRACTIVE; }
+int bonded_arity(int k) {
+    switch (k) {
+    case OCM_HARMONIC_BOND: case OCM_GAUSSIAN_BOND: case OCM_GO_CONTACT: return 2;
+    case OCM_HARMONIC_ANGLE: case OCM_FLEXIBLE_LOCAL_ANGLE: return 3;
+    case OCM_GAUSSIAN_DIHEDRAL: case OCM_FLEXIBLE_LOCAL_DIHEDRAL: return 4;
+    default: return 0;
+    }
+}
+int bonded_nparams(int k) {
+    switch (k) {
+    case OCM_HARMONIC_BOND: return 2;
+    case OCM_GAUSSIAN_BOND: return 3;
+    case OCM_GO_CONTACT: return 2;
+    case OCM_HARMONIC_ANGLE: return 2;
+    case OCM_FLEXIBLE_LOCAL_ANGLE: return 1;
+    case OCM_GAUSSIAN_DIHEDRAL: return 3;
+    case OCM_FLEXIBLE_LOCAL_DIHEDRAL: return 1;
+    default: return 0;
+    }
+}
+static int bonded_nglobals(int k) {
+    if (k == OCM_FLEXIBLE_LOCAL_ANGLE) return 13;
+    if (k == OCM_FLEXIBLE_LOCAL_DIHEDRAL) return 8;
+    return 0;
+}
+int pair_npp(int k) {
+    switch (k) {
+    case OCM_EXCLUDED_VOLUME: return 1;
+    case OCM_DEBYE_HUCKEL: return 1;
+    case OCM_LJ_REPULSIVE: case OCM_LJ_ATTRACTIVE: return 2;
+    default: return 0;
+    }
+}
+static int pair_nglobals(int k) {
+    if (k == OCM_EXCLUDED_VOLUME) return 1;
+    if (k == OCM_DEBYE_HUCKEL) return 3;
+    return 0;
+}
+static int pair_bit(int k) { return 1 << (k - OCM_EXCLUDED_VOLUME); }
+
+// Natural cubic spline second derivatives on uniform knots (the fit OpenMM's
+// Continuous1DFunction gives the FLA y-table).
+void natural_spline_y2(const double* y, int n, double h, double* y2) {
+    for (int i = 0; i < n; ++i) y2[i] = 0.0;
+    if (n < 3) return;
+    std::vector<double> diag(n), rhs(n);
+    for (int i = 1; i < n - 1; ++i) {
+        diag[i] = 4.0 * h;
+        rhs[i] = 6.0 * (y[i + 1] - 2.0 * y[i] + y[i - 1]) / h;
+    }
+    for (int i = 2; i < n - 1; ++i) {
+        const double m = h / diag[i - 1];
+        diag[i] -= m * h;
+        rhs[i] -= m * rhs[i - 1];
+    }
+    y2[n - 2] = rhs[n - 2] / diag[n - 2];
+    for (int i = n - 3; i >= 1; --i) y2[i] = (rhs[i] - h * y2[i + 1]) / diag[i];
+}
+
+static void invalid(const std::string& m) { throw Error(OCM_ERR_INVALID, m); }
+
+HostSystem copy_system(const ocm_system_desc& d, const ocm_integrator_desc& it) {
+    HostSystem s;
+    if (d.n_particles <= 0) invalid("system has no particles");
+    if (!d.masses) invalid("masses pointer is NULL");
+    if (d.n_forces < 0 || (d.n_forces > 0 && !d.forces)) invalid("forces pointer is NULL");
+    s.n = d.n_particles;
+    s.masses.assign(d.masses, d.masses + s.n);
+    for (double m : s.masses)
+        if (!(m >= 0.0) || !std::isfinite(m)) invalid("mass must be finite and >= 0");
+    s.periodic = d.use_periodic_box != 0;
+    for (int c = 0; c < 3; ++c) {
+        s.box[c] = d.box[c];
+        if (s.periodic && !(d.box[c] > 0.0)) invalid("periodic box edges must be > 0");
+    }
+    if (!(it.dt > 0.0) || !(it.temperature >= 0.0) || !(it.friction >= 0.0))
+        invalid("integrator needs dt > 0, temperature >= 0, friction >= 0");
+    s.integ = it;
+    for (int fi = 0; fi < d.n_forces; ++fi) {
+        const ocm_force_desc& f = d.forces[fi];
+        HostForce h;
+        h.kind = f.kind;
+        h.group = f.group;
+        h.use_periodic = f.use_periodic;
+        if (f.group < 0 || f.group > 31) invalid("force group must be in 0..31");
+        if (h.use_periodic && !s.periodic) invalid("force uses periodic boundaries but the system has no box");
+        if (is_bonded(f.kind)) {
+            const int ar = bonded_arity(f.kind), np = bonded_nparams(f.kind), ng = bonded_nglobals(f.kind);
+            if (f.n_terms < 0 || (f.n_terms > 0 && (!f.indices || !f.params))) invalid("bonded force arrays missing");
+            if (f.n_globals != ng || (ng > 0 && !f.globals)) {
+                std::ostringstream o;
+                o << "force kind " << f.kind << " needs " << ng << " globals, got " << f.n_globals;
+                invalid(o.str());
+            }
+            h.n_terms = f.n_terms;
+            h.indices.assign(f.indices, f.indices + static_cast<size_t>(ar) * f.n_terms);
+            h.params.assign(f.params, f.params + static_cast<size_t>(np) * f.n_terms);
+            if (ng) h.globals.assign(f.globals, f.globals + ng);
+            for (int v : h.indices)
+                if (v < 0 || v >= s.n) invalid("bonded term index out of range");
+            for (double p : h.params)
+                if (!std::isfinite(p)) invalid("bonded parameter is not finite");
+            for (double g : h.globals)
+                if (!std::isfinite(g)) invalid("force global is not finite");
+        } else if (is_pair(f.kind)) {
+            const int npp = pair_npp(f.kind), ng = pair_nglobals(f.kind);
+            if (!f.particle_params) invalid("nonbonded force needs particle_params");
+            if (f.n_globals != ng || (ng > 0 && !f.globals)) {
+                std::ostringstream o;
+                o << "force kind " << f.kind << " needs " << ng << " globals, got " << f.n_globals;
+                invalid(o.str());
+            }
+            if (!(f.cutoff > 0.0)) invalid("nonbonded cutoff must be > 0");
+            h.cutoff = f.cutoff;
+            h.switch_distance = f.switch_distance > 0 ? f.switch_distance : 0.0;
+            if (h.switch_distance > 0 && f.kind != OCM_LJ_ATTRACTIVE)
+                invalid("switching is supported for LennardJonesAttractive only");
+            if (h.switch_distance >= h.cutoff) invalid("switch distance must be below the cutoff");
+            h.particle_params.assign(f.particle_params, f.particle_params + static_cast<size_t>(npp) * s.n);
+            if (ng) h.globals.assign(f.globals, f.globals + ng);
+            if (f.category) {
+                if (f.n_categories < 1 || f.n_categories > 32) invalid("n_categories must be in 1..32");
+                h.n_categories = f.n_categories;
+                h.category.assign(f.category, f.category + s.n);
+                for (int c : h.category)
+                    if (c < -1 || c >= f.n_categories) invalid("category out of range");
+                if (f.category_allow) {
+                    h.allow.assign(f.category_allow, f.category_allow + f.n_categories * f.n_categories);
+                    for (int a = 0; a < f.n_categories; ++a)
+                        for (int b = 0; b < f.n_categories; ++b)
+                            if ((h.allow[a * f.n_categories + b] != 0) != (h.allow[b * f.n_categories + a] != 0))
+                                invalid("category_allow must be symmetric");
+                }
+            } else {
+                h.n_categories = 1;
+            }
+            for (int i = 0; i < s.n; ++i) {
+                const bool part = h.category.empty() || h.category[i] >= 0;
+                if (!part) continue;
+                for (int k = 0; k < npp; ++k)
+                    if (!std::isfinite(h.particle_params[static_cast<size_t>(npp) * i + k]))
+                        invalid("participating bead has a non-finite parameter");
+            }
+            if (f.n_exclusions < 0 || (f.n_exclusions > 0 && !f.exclusions)) invalid("exclusions pointer is NULL");
+            h.exclusions.reserve(f.n_exclusions);
+            for (int64_t e = 0; e < f.n_exclusions; ++e) {
+                int a = f.exclusions[2 * e], b = f.exclusions[2 * e + 1];
+                if (a < 0 || b < 0 || a >= s.n || b >= s.n) invalid("exclusion index out of range");
+                if (a == b) continue;
+                if (a > b) std::swap(a, b);
+                h.exclusions.emplace_back(a, b);
+            }
+            std::sort(h.exclusions.begin(), h.exclusions.end());
+            h.exclusions.erase(std::unique(h.exclusions.begin(), h.exclusions.end()), h.exclusions.end());
+        } else {
+            std::ostringstream o;
+            o << "unknown force kind " << f.kind;
+            invalid(o.str());
+        }
+        s.forces.push_back(std::move(h));
+    }
+    return s;
+}
+
+// --------------------------------------------------------------- engine
+void Engine::check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) throw Error(OCM_ERR_OOM, std::string(what) + ": out of device memory");
+        throw Error(OCM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+template <typename T> T* Engine::dalloc(size_t n) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    check_hip(hipMalloc(&p, bytes), "hipMalloc");
+    allocs_.push_back(p);
+    return static_cast<T*>(p);
+}
+
+template <typename T> void Engine::upload(T* dst, const std::vector<T>& src) {
+    if (src.empty()) return;
+    check_hip(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+}
+
+Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(std::move(systems)), opt_(opt) {
+    if (sys_.empty()) invalid("no systems");
+    if (opt_.rebuild_interval < 1) opt_.rebuild_interval = 1;
+    if (opt_.chunk_steps < 1) opt_.chunk_steps = 200;
+    if (!(opt_.skin >= 0.0)) invalid("skin must be >= 0");
+    interval_ = opt_.rebuild_interval;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(OCM_ERR_NO_DEVICE, "no HIP device found");
+    if (opt_.device < 0 || opt_.device >= ndev) throw Error(OCM_ERR_NO_DEVICE, "device ordinal out of range");
+    check_hip(hipSetDevice(opt_.device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    check_hip(hipGetDeviceProperties(&prop, opt_.device), "hipGetDeviceProperties");
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+        throw Error(OCM_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    pack();
+}
+
+Engine::~Engine() {
+    drop_graphs();
+    for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocs_) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::drop_graphs() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+}
+
+// Merge key of a nonbonded force: forces with equal keys share one neighbor
+// list and are evaluated in one pass over it.
+struct PairKey {
+    int periodic;
+    double cutoff;
+    const HostForce* f;
+};
+static bool same_list(const HostForce& a, const HostForce& b) {
+    return a.use_periodic == b.use_periodic && a.cutoff == b.cutoff && a.category == b.category &&
+           a.n_categories == b.n_categories && a.allow == b.allow && a.exclusions == b.exclusions;
+}
+
+void Engine::pack() {
+    const int nsys = n_systems();
+    // ---- beads
+    int nb = 0;
+    bead_begin_.resize(nsys);
+    for (int s = 0; s < nsys; ++s) { bead_begin_[s] = nb; nb += sys_[s].n; }
+    d_.nb = nb;
+    d_.nsys = nsys;
+    std::vector<int> sys_of(nb), local(nb);
+    std::vector<double> inv_m(nb), mass(nb);
+    std::vector<DevSys> dsys(nsys);
+    sys_status_.assign(nsys, OCM_OK);
+    sys_step_origin_.assign(nsys, 0);
+    sys_group_mask_.assign(nsys, 0);
+    const double kB = 1.380649e-23 * 6.02214076e23 / 1000.0;  // kJ/mol/K
+    for (int s = 0; s < nsys; ++s) {
+        const HostSystem& H = sys_[s];
+        for (int i = 0; i < H.n; ++i) {
+            const int b = bead_begin_[s] + i;
+            sys_of[b] = s;
+            local[b] = i;
+            mass[b] = H.masses[i];
+            inv_m[b] = H.masses[i] > 0 ? 1.0 / H.masses[i] : 0.0;
+        }
+        DevSys& D = dsys[s];
+        std::memset(&D, 0, sizeof(D));
+        D.periodic = H.periodic;
+        for (int c = 0; c < 3; ++c) {
+            D.box[c] = H.box[c];
+            D.inv_box[c] = H.periodic ? 1.0 / H.box[c] : 0.0;
+        }
+        D.bead_begin = bead_begin_[s];
+        D.bead_count = H.n;
+        D.half_dt = 0.5 * H.integ.dt;
+        D.c1 = std::exp(-H.integ.friction * H.integ.dt);
+        D.noise_kT = kB * H.integ.temperature * (1.0 - D.c1 * D.c1);
+        D.seed = H.integ.seed;
+        D.step_origin = 0;
+        for (const HostForce& f : H.forces) sys_group_mask_[s] |= 1u << f.group;
+    }
+
+    // ---- bonded terms and tables
+    std::vector<BTerm> terms;
+    std::vector<FlaTable> fla;
+    std::vector<FldTable> fld;
+    for (int s = 0; s < nsys; ++s) {
+        const HostSystem& H = sys_[s];
+        for (const HostForce& f : H.forces) {
+            if (!is_bonded(f.kind)) continue;
+            int table = 0;
+            if (f.kind == OCM_FLEXIBLE_LOCAL_ANGLE) {
+                FlaTable T{};
+                for (int k = 0; k < 10; ++k) T.y[k] = f.globals[k];
+                T.tmin = f.globals[10];
+                T.tmax = f.globals[11];
+                T.emin = f.globals[12];
+                if (!(T.tmax > T.tmin)) invalid("FlexibleLocalAngle needs max_theta > min_theta");
+                T.h = (T.tmax - T.tmin) / 9.0;
+                natural_spline_y2(T.y, 10, T.h, T.y2);
+                table = static_cast<int>(fla.size());
+                fla.push_back(T);
+            } else if (f.kind == OCM_FLEXIBLE_LOCAL_DIHEDRAL) {
+                FldTable T{};
+                for (int k = 0; k < 8; ++k) T.c[k] = f.globals[k];
+                table = static_cast<int>(fld.size());
+                fld.push_back(T);
+            }
+            const int ar = bonded_arity(f.kind), np = bonded_nparams(f.kind);
+            for (int t = 0; t < f.n_terms; ++t) {
+                BTerm T{};
+                for (int k = 0; k < 4; ++k) T.idx[k] = k < ar ? bead_begin_[s] + f.indices[ar * t + k] : -1;
+                T.kind = f.kind;
+                T.meta = (f.group & 31) | ((f.use_periodic ? 1 : 0) << 5) | (table << 8);
+                for (int k = 0; k < 3; ++k) T.p[k] = k < np ? f.params[np * t + k] : 0.0;
+                terms.push_back(T);
+            }
+        }
+    }
+    std::stable_sort(terms.begin(), terms.end(), [](const BTerm& a, const BTerm& b) { return a.kind < b.kind; });
+    d_.nt = static_cast<int>(terms.size());
+
+    // ---- pair groups
+    std::vector<DevPairGroup> groups;
+    std::vector<int> member_group, member_bead, member_type, member_cat, excl_start, excl_count, excl;
+    std::vector<float> types_f;
+    std::vector<double> types_d;
+    int ncell_total = 0;
+    int max_types = 0;
+    force_group_of_.assign(nsys, {});
+    for (int s = 0; s < nsys; ++s) {
+        const HostSystem& H = sys_[s];
+        force_group_of_[s].assign(H.forces.size(), -1);
+        std::vector<std::vector<int>> merged;   // lists of force indices
+        for (int fi = 0; fi < static_cast<int>(H.forces.size()); ++fi) {
+            const HostForce& f = H.forces[fi];
+            if (!is_pair(f.kind)) continue;
+            bool placed = false;
+            for (auto& g : merged) {
+                const HostForce& h = H.forces[g[0]];
+                bool kind_taken = false;
+                for (int o : g) kind_taken |= H.forces[o].kind == f.kind;
+                if (!kind_taken && same_list(h, f)) { g.push_back(fi); placed = true; break; }
+            }
+            if (!placed) merged.push_back({fi});
+        }
+        for (const auto& g : merged) {
+            const HostForce& f0 = H.forces[g[0]];
+            const int gid = static_cast<int>(groups.size());
+            DevPairGroup G;
+            std::memset(&G, 0, sizeof(G));
+            G.sys = s;
+            G.periodic = f0.use_periodic ? 1 : 0;
+            G.rc = f0.cutoff;
+            G.rc2 = f0.cutoff * f0.cutoff;
+            double skin = opt_.skin;
+            if (G.periodic) {
+                const double half = 0.5 * std::min({H.box[0], H.box[1], H.box[2]});
+                if (f0.cutoff > half) invalid("nonbonded cutoff exceeds half the periodic box");
+                skin = std::min(skin, half - f0.cutoff);
+            }
+            const double rl = f0.cutoff + skin;
+            G.rl2 = rl * rl;
+            G.half_skin2 = 0.25 * skin * skin;
+            for (int c = 0; c < 3; ++c) {
+                G.box[c] = H.box[c];
+                G.inv_box[c] = G.periodic ? 1.0 / H.box[c] : 0.0;
+            }
+            G.egroup[0] = G.egroup[1] = G.egroup[2] = G.egroup[3] = 0;
+            const HostForce* byk[4] = {nullptr, nullptr, nullptr, nullptr};
+            for (int fi : g) {
+                const HostForce& f = H.forces[fi];
+                const int k = f.kind - OCM_EXCLUDED_VOLUME;
+                byk[k] = &f;
+                G.kinds |= pair_bit(f.kind);
+                G.egroup[k] = f.group;
+                force_group_of_[s][fi] = gid;
+            }
+            if (byk[0]) {
+                G.exv_eps = byk[0]->globals[0];
+                G.exv_corr = std::pow(1.0 / f0.cutoff, 12);
+            }
+            if (byk[1]) {
+                G.dh_K = byk[1]->globals[0];
+                if (!(byk[1]->globals[1] > 0)) invalid("Debye length must be > 0");
+                G.dh_inv_lambda = 1.0 / byk[1]->globals[1];
+                G.dh_corr = byk[1]->globals[2];
+            }
+            if (byk[3]) G.lja_rsw = byk[3]->switch_distance;
+            // categories
+            G.n_cat = f0.n_categories;
+            for (int a = 0; a < G.n_cat; ++a) {
+                uint32_t m = 0;
+                for (int b = 0; b < G.n_cat; ++b)
+                    if (f0.allow.empty() || f0.allow[a * G.n_cat + b]) m |= 1u << b;
+                G.allow[a] = m;
+            }
+            // members and types
+            G.member_begin = static_cast<int>(member_bead.size());
+            G.type_begin = static_cast<int>(types_d.size() / kTypeStride);
+            std::map<std::array<double, 6>, int> tmap;
+            std::vector<int> bead_to_member(H.n, -1);
+            for (int i = 0; i < H.n; ++i) {
+                const int cat = f0.category.empty() ? 0 : f0.category[i];
+                if (cat < 0) continue;
+                std::array<double, 6> tup{0, 0, 0, 0, 0, 0};
+                if (byk[0]) tup[0] = byk[0]->particle_params[i];
+                if (byk[1]) tup[1] = byk[1]->particle_params[i];
+                if (byk[2]) { tup[2] = byk[2]->particle_params[2 * i]; tup[3] = byk[2]->particle_params[2 * i + 1]; }
+                if (byk[3]) { tup[4] = byk[3]->particle_params[2 * i]; tup[5] = byk[3]->particle_params[2 * i + 1]; }
+                auto it = tmap.find(tup);
+                int ty;
+                if (it == tmap.end()) {
+                    ty = static_cast<int>(tmap.size());
+                    tmap.emplace(tup, ty);
+                    for (int k = 0; k < kTypeStride; ++k) {
+                        const double v = k < 6 ? tup[k] : 0.0;
+                        types_d.push_back(v);
+                        types_f.push_back(static_cast<float>(v));
+                    }
+                } else {
+                    ty = it->second;
+                }
+                bead_to_member[i] = static_cast<int>(member_bead.size());
+                member_group.push_back(gid);
+                member_bead.push_back(bead_begin_[s] + i);
+                member_type.push_back(ty);
+                member_cat.push_back(cat);
+            }
+            G.member_count = static_cast<int>(member_bead.size()) - G.member_begin;
+            G.n_types = static_cast<int>(tmap.size());
+            if (G.n_types * kTypeStride * sizeof(float) > 64 * 1024)
+                invalid("more than 2048 distinct per-particle parameter sets in one nonbonded force");
+            max_types = std::max(max_types, G.n_types);
+            // exclusions in member space, symmetric CSR sorted by partner
+            std::vector<std::vector<int>> adj(G.member_count);
+            for (const auto& e : f0.exclusions) {
+                const int a = bead_to_member[e.first], b = bead_to_member[e.second];
+                if (a < 0 || b < 0) continue;
+                adj[a - G.member_begin].push_back(b);
+                adj[b - G.member_begin].push_back(a);
+            }
+            for (int m = 0; m < G.member_count; ++m) {
+                auto& v = adj[m];
+                std::sort(v.begin(), v.end());
+                v.erase(std::unique(v.begin(), v.end()), v.end());
+                excl_start.push_back(static_cast<int>(excl.size()));
+                excl_count.push_back(static_cast<int>(v.size()));
+                excl.insert(excl.end(), v.begin(), v.end());
+            }
+            // cell grid
+            long long est_neighbors;
+            if (G.periodic) {
+                long long total = 1;
+                for (int c = 0; c < 3; ++c) {
+                    int n = static_cast<int>(std::floor(H.box[c] / rl));
+                    if (n < 3) { n = 1; G.reach[c] = 0; } else { G.reach[c] = 1; }
+                    G.ncell[c] = n;
+                    total *= n;
+                }
+                while (total > (1LL << 24)) {  // cap the grid; coarser cells keep the list exact
+                    total = 1;
+                    for (int c = 0; c < 3; ++c) {
+                        if (G.ncell[c] >= 6) G.ncell[c] /= 2;
+                        if (G.ncell[c] < 3) { G.ncell[c] = 1; G.reach[c] = 0; }
+                        total *= G.ncell[c];
+                    }
+                }
+                for (int c = 0; c < 3; ++c) G.cell_scale[c] = G.ncell[c] / H.box[c];
+                const double vol = H.box[0] * H.box[1] * H.box[2];
+                const double rho = G.member_count / vol;
+                est_neighbors = static_cast<long long>(G.member_count * (rho * 4.18879 * rl * rl * rl * 3.0 + 8.0));
+            } else {
+                for (int c = 0; c < 3; ++c) { G.ncell[c] = 1; G.reach[c] = 0; G.cell_scale[c] = 0; }
+                est_neighbors = static_cast<long long>(G.member_count) * std::min(G.member_count, 256);
+            }
+            G.cell_begin = ncell_total;
+            ncell_total += G.ncell[0] * G.ncell[1] * G.ncell[2];
+            group_list_estimate_.push_back(est_neighbors);
+            group_member_begin_.push_back(G.member_begin);
+            group_member_count_.push_back(G.member_count);
+            groups.push_back(G);
+        }
+    }
+    d_.nm = static_cast<int>(member_bead.size());
+    d_.ncell = std::max(ncell_total, 1);
+    d_.ngroups = static_cast<int>(groups.size());
+    d_.max_types = std::max(max_types, 1);
+
+    // ---- device allocation and upload
+    const int n3 = 3 * nb;
+    double* pos = dalloc<double>(n3);
+    double* vel = dalloc<double>(n3);
+    long long* frc = dalloc<long long>(n3);
+    long long* frc2 = dalloc<long long>(n3);
+    d_.px = pos; d_.py = pos + nb; d_.pz = pos + 2 * nb;
+    d_.vx = vel; d_.vy = vel + nb; d_.vz = vel + 2 * nb;
+    d_.f = {frc, frc + nb, frc + 2 * nb};
+    d_.f2 = {frc2, frc2 + nb, frc2 + 2 * nb};
+    check_hip(hipMemset(pos, 0, sizeof(double) * n3), "memset");
+    check_hip(hipMemset(vel, 0, sizeof(double) * n3), "memset");
+    check_hip(hipMemset(frc, 0, sizeof(long long) * n3), "memset");
+    bx_ = dalloc<double>(n3);
+    bv_ = dalloc<double>(n3);
+    bf_ = dalloc<long long>(n3);
+    bstep_ = dalloc<long long>(2);
+    d_.sys = dalloc<DevSys>(nsys);
+    upload(d_.sys, dsys);
+    d_.sys_of = dalloc<int>(nb); upload(d_.sys_of, sys_of);
+    d_.local_id = dalloc<int>(nb); upload(d_.local_id, local);
+    d_.inv_m = dalloc<double>(nb); upload(d_.inv_m, inv_m);
+    d_.mass = dalloc<double>(nb); upload(d_.mass, mass);
+    d_.gstep = dalloc<long long>(2);
+    check_hip(hipMemset(d_.gstep, 0, 2 * sizeof(long long)), "memset");
+    d_.err = dalloc<int>(nsys);
+    check_hip(hipMemset(d_.err, 0, nsys * sizeof(int)), "memset");
+    d_.flags = dalloc<int>(1);
+    check_hip(hipMemset(d_.flags, 0, sizeof(int)), "memset");
+    d_.terms = dalloc<BTerm>(terms.size()); upload(d_.terms, terms);
+    d_.fla = dalloc<FlaTable>(fla.size()); upload(d_.fla, fla);
+    d_.fld = dalloc<FldTable>(fld.size()); upload(d_.fld, fld);
+    d_.groups = dalloc<DevPairGroup>(groups.size()); upload(d_.groups, groups);
+    const int nm = d_.nm;
+    d_.member_group = dalloc<int>(nm); upload(d_.member_group, member_group);
+    d_.member_bead = dalloc<int>(nm); upload(d_.member_bead, member_bead);
+    d_.member_type = dalloc<int>(nm); upload(d_.member_type, member_type);
+    d_.member_cat = dalloc<int>(nm); upload(d_.member_cat, member_cat);
+    d_.member_slot = dalloc<int>(nm);
+    d_.member_cell = dalloc<int>(nm);
+    d_.member_rank = dalloc<int>(nm);
+    d_.excl_start = dalloc<int>(nm); upload(d_.excl_start, excl_start);
+    d_.excl_count = dalloc<int>(nm); upload(d_.excl_count, excl_count);
+    d_.excl = dalloc<int>(excl.size()); upload(d_.excl, excl);
+    d_.refx = dalloc<double>(nm); d_.refy = dalloc<double>(nm); d_.refz = dalloc<double>(nm);
+    d_.rec = dalloc<PairRec>(nm);
+    d_.slot_member = dalloc<int>(nm);
+    d_.cell_count = dalloc<int>(d_.ncell);
+    d_.cell_start = dalloc<int>(d_.ncell);
+    d_.row_count = dalloc<int>(nm);
+    d_.row_start = dalloc<int>(nm);
+    check_hip(hipMemset(d_.row_count, 0, sizeof(int) * std::max(nm, 1)), "memset");
+    check_hip(hipMemset(d_.row_start, 0, sizeof(int) * std::max(nm, 1)), "memset");
+    long long est = 0;
+    for (long long e : group_list_estimate_) est += e;
+    d_.list_capacity = std::max<long long>(1 << 16, est);
+    if (d_.list_capacity > std::numeric_limits<int>::max()) d_.list_capacity = std::numeric_limits<int>::max();
+    d_.col = dalloc<int>(d_.list_capacity);
+    col_bytes_ = d_.list_capacity * static_cast<long long>(sizeof(int));
+    const int scan_n = std::max(d_.ncell, nm);
+    d_.scan_partial = dalloc<int>(scan_n / 2048 + 2);
+    d_.scan_total = dalloc<int>(2);
+    check_hip(hipMemset(d_.scan_total, 0, 2 * sizeof(int)), "memset");
+    d_.types_f = dalloc<float>(types_f.size()); upload(d_.types_f, types_f);
+    d_.types_d = dalloc<double>(types_d.size()); upload(d_.types_d, types_d);
+    d_.energy = dalloc<long long>(static_cast<size_t>(nsys) * 32);
+    d_.ke = dalloc<long long>(nsys);
+    d_.dump_count = dalloc<int>(1);
+    int team = opt_.team_size;
+    if (team != 4 && team != 8 && team != 16 && team != 32 && team != 64) team = 16;
+    build_pair_blocks(team);
+    team_tuned_ = opt_.team_size != 0;
+    check_hip(hipDeviceSynchronize(), "upload sync");
+}
+
+void Engine::build_pair_blocks(int team) {
+    d_.team = team;
+    const int rpb = kPairBlock / team;
+    std::vector<int2> blocks;
+    for (int g = 0; g < d_.ngroups; ++g) {
+        for (int r = 0; r < group_member_count_[g]; r += rpb) blocks.push_back(make_int2(g, group_member_begin_[g] + r));
+    }
+    // the block table buffer is reallocated only when it grows
+    static_assert(sizeof(int2) == 8, "int2 layout");
+    if (static_cast<int>(blocks.size()) > d_.n_pair_blocks || !d_.pair_blocks) d_.pair_blocks = dalloc<int2>(blocks.size());
+    upload(d_.pair_blocks, blocks);
+    d_.n_pair_blocks = static_cast<int>(blocks.size());
+    drop_graphs();
+}
+
+// ----------------------------------------------------------- timing
+template <typename F> void Engine::timed(int slot, F&& f) {
+    if (!timing_) { f(); return; }
+    if (event_next_ + 2 > event_pool_.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            check_hip(hipEventCreate(&e), "hipEventCreate");
+            event_pool_.push_back(e);
+        }
+    }
+    hipEvent_t a = event_pool_[event_next_++], b = event_pool_[event_next_++];
+    check_hip(hipEventRecord(a, stream_), "hipEventRecord");
+    f();
+    check_hip(hipEventRecord(b, stream_), "hipEventRecord");
+    pending_events_.push_back({slot, {a, b}});
+}
+
+void Engine::collect_timing() {
+    for (auto& pe : pending_events_) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+            timing_slots_[pe.first].ms += ms;
+            timing_slots_[pe.first].launches += 1;
+        }
+    }
+    pending_events_.clear();
+    event_next_ = 0;
+}
+
+void Engine::sync() {
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if (timing_) collect_timing();
+}
+
+// ------------------------------------------------------------- stepping
+void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
+    if (d_.nm > 0) {
+        if (rebuild) { timed(3, [&] { launch_rebuild(d_, stream_); }); ++rebuilds_; }
+        else timed(4, [&] { launch_pack(d_, stream_); });
+    }
+    timed(1, [&] { launch_bonded(d_, stream_, false, f, groups); });
+    timed(0, [&] { launch_pair(d_, stream_, false, f, groups); });
+}
+
+void Engine::enqueue_chunk(int k, int interval) {
+    for (int s = 0; s < k; ++s) {
+        timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0); });
+        enqueue_forces(d_.f, s % interval == 0, 0xFFFFFFFFu);
+    }
+    timed(2, [&] { launch_half_kick(d_, stream_, k & 1); });
+}
+
+void Engine::snapshot() {
+    const size_t n3 = 3 * static_cast<size_t>(d_.nb);
+    check_hip(hipMemcpyAsync(bx_, d_.px, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "snapshot");
+    check_hip(hipMemcpyAsync(bv_, d_.vx, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "snapshot");
+    check_hip(hipMemcpyAsync(bf_, d_.f.fx, n3 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "snapshot");
+    check_hip(hipMemcpyAsync(bstep_, d_.gstep, 2 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "snapshot");
+}
+
+void Engine::restore() {
+    const size_t n3 = 3 * static_cast<size_t>(d_.nb);
+    check_hip(hipMemcpyAsync(d_.px, bx_, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "restore");
+    check_hip(hipMemcpyAsync(d_.vx, bv_, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "restore");
+    check_hip(hipMemcpyAsync(d_.f.fx, bf_, n3 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
+    check_hip(hipMemcpyAsync(d_.gstep, bstep_, 2 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
+}
+
+void Engine::grow_list(long long needed) {
+    long long cap = static_cast<long long>(needed * 1.25) + (1 << 16);
+    if (cap > std::numeric_limits<int>::max()) cap = std::numeric_limits<int>::max();
+    if (cap <= d_.list_capacity) throw Error(OCM_ERR_OOM, "neighbor list exceeds 2^31 entries");
+    sync();
+    auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(d_.col));
+    if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+    d_.col = dalloc<int>(cap);
+    d_.list_capacity = cap;
+    col_bytes_ = cap * static_cast<long long>(sizeof(int));
+    drop_graphs();
+}
+
+void Engine::ensure_forces() {
+    if (!forces_dirty_) return;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
+        launch_zero_forces(d_, d_.f, stream_);
+        enqueue_forces(d_.f, true, 0xFFFFFFFFu);
+        sync();
+        int flags = 0;
+        check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
+        if (flags & kFlagListOverflow) {
+            int total = 0;
+            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
+            grow_list(total);
+            continue;
+        }
+        forces_dirty_ = false;
+        if (!team_tuned_) autotune_team();
+        return;
+    }
+    throw Error(OCM_ERR_OOM, "neighbor list did not fit after repeated growth");
+}
+
+void Engine::autotune_team() {
+    team_tuned_ = true;
+    if (d_.nm == 0) return;
+    // Forces are fixed-point sums, so every team size gives identical bits;
+    // pick the fastest on the current list.
+    const int cands[4] = {4, 8, 16, 32};
+    hipEvent_t a, b;
+    check_hip(hipEventCreate(&a), "event");
+    check_hip(hipEventCreate(&b), "event");
+    float best = 1e30f;
+    int best_team = d_.team;
+    for (int t : cands) {
+        build_pair_blocks(t);
+        launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);  // warm-up into the scratch buffer
+        check_hip(hipEventRecord(a, stream_), "event");
+        for (int r = 0; r < 3; ++r) launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);
+        check_hip(hipEventRecord(b, stream_), "event");
+        check_hip(hipEventSynchronize(b), "event");
+        float ms = 0;
+        check_hip(hipEventElapsedTime(&ms, a, b), "event");
+        if (ms < best) { best = ms; best_team = t; }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    build_pair_blocks(best_team);
+    // the tuning launches may have flagged numeric trouble in the scratch pass only
+    check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
+    sync();
+}
+
+int Engine::run_chunk(int k) {
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        snapshot();
+        check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
+        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
+        const int interval = std::min(interval_, k);
+        const bool graph = opt_.use_graph && !timing_;
+        if (graph) {
+            const auto key = std::make_pair(k, interval);
+            auto it = graphs_.find(key);
+            if (it == graphs_.end()) {
+                hipGraph_t g;
+                check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+                enqueue_chunk(k, interval);
+                check_hip(hipStreamEndCapture(stream_, &g), "capture");
+                hipGraphExec_t ge;
+                check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
+                (void)hipGraphDestroy(g);
+                it = graphs_.emplace(key, ge).first;
+            }
+            check_hip(hipGraphLaunch(it->second, stream_), "hipGraphLaunch");
+            rebuilds_ += (k + interval - 1) / interval;
+        } else {
+            enqueue_chunk(k, interval);
+        }
+        sync();
+        int flags = 0;
+        std::vector<int> err(d_.nsys);
+        check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
+        check_hip(hipMemcpy(err.data(), d_.err, sizeof(int) * d_.nsys, hipMemcpyDeviceToHost), "err");
+        if (flags & kFlagListOverflow) {
+            int total = 0;
+            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
+            restore();
+            grow_list(total);
+            continue;
+        }
+        bool stale = false, numeric = false;
+        for (int s = 0; s < d_.nsys; ++s) {
+            if (err[s] & kErrNumeric) { sys_status_[s] = OCM_ERR_NUMERIC; numeric = true; }
+            else if ((err[s] & kErrStale) && sys_status_[s] == OCM_OK) stale = true;
+        }
+        if (stale && interval > 1) {
+            restore();
+            interval_ = std::max(1, interval / 2);
+            continue;
+        }
+        return numeric ? OCM_ERR_NUMERIC : OCM_OK;
+    }
+    throw Error(OCM_ERR_HIP, "step chunk could not complete");
+}
+
+int Engine::step(long long n) {
+    if (n < 0) invalid("negative step count");
+    if (n == 0) return OCM_OK;
+    ensure_forces();
+    int status = OCM_OK;
+    long long remaining = n;
+    while (remaining > 0) {
+        const int k = static_cast<int>(std::min<long long>(remaining, opt_.chunk_steps));
+        const int st = run_chunk(k);
+        if (st != OCM_OK) status = st;
+        remaining -= k;
+    }
+    return status;
+}
+
+// ------------------------------------------------------------ state I/O
+void Engine::set_positions(int s, const double* xyz) {
+    const HostSystem& H = sys_.at(s);
+    std::vector<double> x(H.n), y(H.n), z(H.n);
+    for (int i = 0; i < H.n; ++i) {
+        x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2];
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i])) invalid("position is not finite");
+    }
+    const size_t off = bead_begin_[s];
+    check_hip(hipMemcpy(d_.px + off, x.data(), H.n * sizeof(double), hipMemcpyHostToDevice), "positions");
+    check_hip(hipMemcpy(d_.py + off, y.data(), H.n * sizeof(double), hipMemcpyHostToDevice), "positions");
+    check_hip(hipMemcpy(d_.pz + off, z.data(), H.n * sizeof(double), hipMemcpyHostToDevice), "positions");
+    forces_dirty_ = true;
+    sys_status_[s] = OCM_OK;
+}
+
+void Engine::set_velocities(int s, const double* xyz) {
+    const HostSystem& H = sys_.at(s);
+    std::vector<double> x(H.n), y(H.n), z(H.n);
+    for (int i = 0; i < H.n; ++i) { x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2]; }
+    const size_t off = bead_begin_[s];
+    check_hip(hipMemcpy(d_.vx + off, x.data(), H.n * sizeof(double), hipMemcpyHostToDevice), "velocities");
+    check_hip(hipMemcpy(d_.vy + off, y.data(), H.n * sizeof(double), hipMemcpyHostToDevice), "velocities");
+    check_hip(hipMemcpy(d_.vz + off, z.data(), H.n * sizeof(double), hipMemcpyHostToDevice), "velocities");
+}
+
+void Engine::set_velocities_to_temperature(int s, double T, uint64_t seed) {
+    if (!(T >= 0)) invalid("temperature must be >= 0");
+    const double kB = 1.380649e-23 * 6.02214076e23 / 1000.0;
+    launch_velocity_noise(d_, stream_, s, kB * T, seed);
+    sync();
+}
+
+static void to_host3(const double* dx, const double* dy, const double* dz, int n, double* out) {
+    std::vector<double> x(n), y(n), z(n);
+    (void)hipMemcpy(x.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipMemcpy(y.data(), dy, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipMemcpy(z.data(), dz, n * sizeof(double), hipMemcpyDeviceToHost);
+    for (int i = 0; i < n; ++i) { out[3 * i] = x[i]; out[3 * i + 1] = y[i]; out[3 * i + 2] = z[i]; }
+}
+
+void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
+    const HostSystem& H = sys_.at(s);
+    const size_t off = bead_begin_[s];
+    sync();
+    if ((what & OCM_STATE_POSITIONS) && out->positions) to_host3(d_.px + off, d_.py + off, d_.pz + off, H.n, out->positions);
+    if ((what & OCM_STATE_VELOCITIES) && out->velocities) to_host3(d_.vx + off, d_.vy + off, d_.vz + off, H.n, out->velocities);
+    long long gs[2] = {0, 0};
+    check_hip(hipMemcpy(gs, d_.gstep, sizeof(gs), hipMemcpyDeviceToHost), "gstep");
+    out->step = gs[0] - sys_step_origin_[s];
+    if ((what & OCM_STATE_FORCES) && out->forces) {
+        const uint32_t present = sys_group_mask_[s];
+        ForceBuf src = d_.f;
+        if ((present & groups) != present) {
+            launch_zero_forces(d_, d_.f2, stream_);
+            enqueue_forces(d_.f2, true, groups);
+            src = d_.f2;
+        } else {
+            ensure_forces();
+        }
+        sync();
+        std::vector<long long> fx(H.n), fy(H.n), fz(H.n);
+        check_hip(hipMemcpy(fx.data(), src.fx + off, H.n * sizeof(long long), hipMemcpyDeviceToHost), "forces");
+        check_hip(hipMemcpy(fy.data(), src.fy + off, H.n * sizeof(long long), hipMemcpyDeviceToHost), "forces");
+        check_hip(hipMemcpy(fz.data(), src.fz + off, H.n * sizeof(long long), hipMemcpyDeviceToHost), "forces");
+        for (int i = 0; i < H.n; ++i) {
+            out->forces[3 * i] = fx[i] * kFixInv;
+            out->forces[3 * i + 1] = fy[i] * kFixInv;
+            out->forces[3 * i + 2] = fz[i] * kFixInv;
+        }
+    }
+    if (what & OCM_STATE_ENERGY) {
+        launch_zero_energy(d_, stream_);
+        if (d_.nm > 0) { launch_rebuild(d_, stream_); ++rebuilds_; }
+        launch_bonded(d_, stream_, true, d_.f2, groups);
+        launch_pair(d_, stream_, true, d_.f2, groups);
+        launch_kinetic(d_, stream_);
+        sync();
+        int flags = 0;
+        check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
+        if (flags & kFlagListOverflow) {
+            int total = 0;
+            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
+            grow_list(total);
+            check_hip(hipMemset(d_.flags, 0, sizeof(int)), "memset");
+            get_state(s, OCM_STATE_ENERGY, groups, out);
+            return;
+        }
+        std::vector<long long> e(32);
+        long long ke = 0;
+        check_hip(hipMemcpy(e.data(), d_.energy + static_cast<size_t>(s) * 32, 32 * sizeof(long long), hipMemcpyDeviceToHost), "energy");
+        check_hip(hipMemcpy(&ke, d_.ke + s, sizeof(long long), hipMemcpyDeviceToHost), "ke");
+        out->potential_energy = 0;
+        for (int g = 0; g < 32; ++g) {
+            out->energy_by_group[g] = e[g] * kFixInv;
+            if ((groups >> g) & 1u) out->potential_energy += out->energy_by_group[g];
+        }
+        out->kinetic_energy = ke * kFixInv;
+    }
+}
+
+long long Engine::neighbor_pairs(int s, int force_index, int32_t* out, long long cap) {
+    const HostSystem& H = sys_.at(s);
+    if (force_index < 0 || force_index >= static_cast<int>(H.forces.size())) invalid("force index out of range");
+    const int g = force_group_of_[s][force_index];
+    if (g < 0) invalid("force is not a nonbonded force");
+    ensure_forces();
+    check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
+    launch_rebuild(d_, stream_);
+    ++rebuilds_;
+    long long want = std::max<long long>(cap, 1);
+    int2* buf = dalloc<int2>(static_cast<size_t>(want));
+    d_.dump_pairs = buf;
+    launch_neighbor_dump(d_, stream_, g, want, group_member_count_[g]);
+    sync();
+    int cnt = 0;
+    check_hip(hipMemcpy(&cnt, d_.dump_count, sizeof(int), hipMemcpyDeviceToHost), "dump");
+    const long long ncopy = std::min<long long>(cnt, cap);
+    std::vector<int2> host(static_cast<size_t>(std::max<long long>(ncopy, 1)));
+    if (ncopy > 0) check_hip(hipMemcpy(host.data(), buf, ncopy * sizeof(int2), hipMemcpyDeviceToHost), "dump");
+    auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(buf));
+    if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+    d_.dump_pairs = nullptr;
+    std::vector<std::pair<int, int>> v;
+    for (long long k = 0; k < ncopy; ++k) v.emplace_back(host[k].x - bead_begin_[s], host[k].y - bead_begin_[s]);
+    std::sort(v.begin(), v.end());
+    for (long long k = 0; k < ncopy; ++k) { out[2 * k] = v[k].first; out[2 * k + 1] = v[k].second; }
+    return cnt;
+}
+
+void Engine::list_stats(long long* entries, long long* members, long long* rebuilds) {
+    sync();
+    int total = 0;
+    check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
+    if (entries) *entries = total;
+    if (members) *members = d_.nm;
+    if (rebuilds) *rebuilds = rebuilds_;
+}
+
+}  // namespace ocm

@@ -1,0 +1,135 @@
+// engine.hpp -- host side of the device step: packs systems into device arrays
+// and drives the kernels.  A single-system context is a batch of one.
+#pragma once
+
+#include "../../include/ocm.h"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace ocm {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Owned copy of one ocm_force_desc.
+struct HostForce {
+    int kind = 0, group = 0, use_periodic = 0, n_terms = 0, n_categories = 0;
+    std::vector<int> indices;
+    std::vector<double> params, globals, particle_params;
+    std::vector<int> category;       // empty: every bead is category 0
+    std::vector<uint8_t> allow;      // empty: all category pairs allowed
+    double cutoff = 0, switch_distance = 0;
+    std::vector<std::pair<int, int>> exclusions;
+};
+
+struct HostSystem {
+    int n = 0;
+    std::vector<double> masses;
+    bool periodic = false;
+    double box[3] = {0, 0, 0};
+    std::vector<HostForce> forces;
+    ocm_integrator_desc integ{};
+};
+
+HostSystem copy_system(const ocm_system_desc& d, const ocm_integrator_desc& it);
+int bonded_arity(int kind);
+int bonded_nparams(int kind);
+int pair_npp(int kind);
+bool is_bonded(int kind);
+bool is_pair(int kind);
+void natural_spline_y2(const double* y, int n, double h, double* y2);
+
+struct TimingSlot {
+    double ms = 0;
+    long long launches = 0;
+};
+
+class Engine {
+public:
+    Engine(std::vector<HostSystem> systems, const ocm_options& opt);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int n_systems() const { return static_cast<int>(sys_.size()); }
+    int n_particles(int s) const { return sys_.at(s).n; }
+
+    void set_positions(int s, const double* xyz);
+    void set_velocities(int s, const double* xyz);
+    void set_velocities_to_temperature(int s, double T, uint64_t seed);
+    // Advances every system by n steps.  Returns OCM_OK or OCM_ERR_NUMERIC
+    // (some system produced NaN/overflow; see system_status).
+    int step(long long n);
+    void get_state(int s, int what, uint32_t groups, ocm_state* out);
+    long long neighbor_pairs(int s, int force_index, int32_t* out, long long cap);
+    int system_status(int s) const { return sys_status_.at(s); }
+
+    void set_timing(bool on) { timing_ = on; }
+    TimingSlot timing(int k) const { return timing_slots_.at(k); }
+    void list_stats(long long* entries, long long* members, long long* rebuilds);
+
+private:
+    void pack();
+    void alloc_device();
+    void build_pair_blocks(int team);
+    void ensure_forces();
+    void enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups);
+    void enqueue_chunk(int k, int interval);
+    int run_chunk(int k);
+    void snapshot();
+    void restore();
+    void check_hip(hipError_t e, const char* what);
+    void sync();
+    void grow_list(long long needed);
+    void autotune_team();
+    void drop_graphs();
+    template <typename F> void timed(int slot, F&& f);
+    void collect_timing();
+
+    std::vector<HostSystem> sys_;
+    ocm_options opt_;
+    hipStream_t stream_ = nullptr;
+    DevPtrs d_{};
+    std::vector<void*> allocs_;
+    template <typename T> T* dalloc(size_t n);
+    template <typename T> void upload(T* dst, const std::vector<T>& src);
+
+    // host mirrors
+    std::vector<int> bead_begin_;
+    std::vector<std::vector<int>> force_group_of_;   // [sys][force] -> pair group or -1
+    std::vector<int> group_member_begin_, group_member_count_;
+    std::vector<long long> group_list_estimate_;
+    std::vector<uint32_t> sys_group_mask_;
+    std::vector<int> sys_status_;
+    std::vector<int64_t> sys_step_origin_;
+    bool forces_dirty_ = true;
+    int interval_ = 20;
+    long long rebuilds_ = 0;
+    bool timing_ = false;
+    std::array<TimingSlot, 5> timing_slots_{};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_events_;
+    std::vector<hipEvent_t> event_pool_;
+    size_t event_next_ = 0;
+    bool team_tuned_ = false;
+
+    // snapshot buffers
+    double *bx_ = nullptr, *bv_ = nullptr;
+    long long* bf_ = nullptr;
+    long long* bstep_ = nullptr;
+
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs_;
+    long long col_bytes_ = 0;
+};
+
+}  // namespace ocm

@@ -1,0 +1,921 @@
+// kernels.hip -- gfx950 kernels of the MD step.
+//
+// Step stages (one launch each, covering every system of a batch):
+//   k_integrate      BAOAB: kick (B), drift (A), Ornstein-Uhlenbeck (O), drift (A)
+//   k_pack           refresh the cell-sorted pair records, check list validity
+//   list build       k_cell_assign -> scan -> k_place -> k_nl_count -> scan -> k_nl_fill
+//   k_bonded         all bonded kinds, one thread per term
+//   k_pair           neighbor-list pair forces, a team of lanes per member row
+//   k_half_kick      closing half kick at the end of a chunk
+// Forces are computed in FP32 from FP64 displacements and accumulated as
+// 64-bit fixed point (2^-32 kJ/mol/nm); positions/velocities are FP64.
+// The energy pass (template ENERGY=true) evaluates terms in FP64.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ocm {
+
+// ------------------------------------------------------------ small helpers
+// Out-of-range or non-finite contributions are flagged and replaced by 0, so
+// the float-to-integer conversion is only ever applied to representable values.
+__device__ inline long long to_fix(float v, int& bad) {
+    const bool ok = fabsf(v) < kFixLimitF;
+    bad |= !ok;
+    return ok ? static_cast<long long>(v * 4294967296.0f) : 0LL;
+}
+__device__ inline long long to_fix_d(double v, int& bad) {
+    const bool ok = fabs(v) < 1073741824.0;
+    bad |= !ok;
+    return ok ? static_cast<long long>(v * kFixScale) : 0LL;
+}
+
+// Minimum-image component: d - L * rint(d / L), written with one fused
+// multiply-add so the CPU oracle reproduces the same bits.
+__device__ inline double min_image(double d, double L, double invL) {
+    return __fma_rn(-L, rint(d * invL), d);
+}
+__device__ inline double dist2(double dx, double dy, double dz) {
+    return __fma_rn(dz, dz, __fma_rn(dy, dy, dx * dx));
+}
+
+template <typename T>
+__device__ inline T warp_sum(T v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, width);
+    return v;
+}
+
+// ------------------------------------------------------------------ fills
+__global__ void k_fill_i32(int* p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_i64(long long* p, long long v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------- integrator
+// pending != 0: the closing half kick of the previous step is still owed, so
+// the opening kick uses a full dt.  gstep[pin] is this step's global step;
+// thread 0 writes gstep[pout] = gstep[pin] + 1 for the next step.
+__global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* __restrict__ sys_of,
+                            const int* __restrict__ local_id, const double* __restrict__ inv_m,
+                            double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+                            double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                            long long* __restrict__ fx, long long* __restrict__ fy,
+                            long long* __restrict__ fz, long long* gstep, int pin, int pout,
+                            int pending, int* __restrict__ err) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long step = gstep[pin];
+    if (b == 0) gstep[pout] = step + 1;
+    if (b >= nb) return;
+    const int s = sys_of[b];
+    const DevSys S = sys[s];
+    const double im = inv_m[b];
+    const double f0 = fx[b] * kFixInv, f1 = fy[b] * kFixInv, f2 = fz[b] * kFixInv;
+    fx[b] = 0; fy[b] = 0; fz[b] = 0;
+    if (im == 0.0) return;
+    const double kick = pending ? 2.0 * S.half_dt : S.half_dt;
+    double v0 = vx[b] + kick * f0 * im;
+    double v1 = vy[b] + kick * f1 * im;
+    double v2 = vz[b] + kick * f2 * im;
+    double x0 = px[b] + S.half_dt * v0;
+    double x1 = py[b] + S.half_dt * v1;
+    double x2 = pz[b] + S.half_dt * v2;
+    double g[3];
+    gauss3(S.seed, static_cast<uint64_t>(step - S.step_origin), static_cast<uint32_t>(local_id[b]), 0u, g);
+    const double sig = sqrt(S.noise_kT * im);
+    v0 = S.c1 * v0 + sig * g[0];
+    v1 = S.c1 * v1 + sig * g[1];
+    v2 = S.c1 * v2 + sig * g[2];
+    x0 += S.half_dt * v0;
+    x1 += S.half_dt * v1;
+    x2 += S.half_dt * v2;
+    if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) atomicOr(&err[s], kErrNumeric);
+    px[b] = x0; py[b] = x1; pz[b] = x2;
+    vx[b] = v0; vy[b] = v1; vz[b] = v2;
+}
+
+// Closing half kick; also resets both global-step slots to the value reached.
+__global__ void k_half_kick(int nb, const DevSys* __restrict__ sys, const int* __restrict__ sys_of,
+                            const double* __restrict__ inv_m, double* __restrict__ vx,
+                            double* __restrict__ vy, double* __restrict__ vz,
+                            const long long* __restrict__ fx, const long long* __restrict__ fy,
+                            const long long* __restrict__ fz, long long* gstep, int pcur) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) {
+        const long long s = gstep[pcur];
+        gstep[0] = s;
+        gstep[1] = s;
+    }
+    if (b >= nb) return;
+    const double im = inv_m[b];
+    if (im == 0.0) return;
+    const double h = sys[sys_of[b]].half_dt;
+    vx[b] += h * (fx[b] * kFixInv) * im;
+    vy[b] += h * (fy[b] * kFixInv) * im;
+    vz[b] += h * (fz[b] * kFixInv) * im;
+}
+
+// ------------------------------------------------------------ record pack
+__global__ void k_pack(int nm, const DevPairGroup* __restrict__ groups,
+                       const int* __restrict__ member_group, const int* __restrict__ member_bead,
+                       const int* __restrict__ member_type, const int* __restrict__ member_slot,
+                       const double* __restrict__ px, const double* __restrict__ py,
+                       const double* __restrict__ pz, const double* __restrict__ refx,
+                       const double* __restrict__ refy, const double* __restrict__ refz,
+                       PairRec* __restrict__ rec, int* __restrict__ err) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const int b = member_bead[m];
+    PairRec r;
+    r.x = px[b]; r.y = py[b]; r.z = pz[b];
+    r.type = member_type[m];
+    r.bead = b;
+    rec[member_slot[m]] = r;
+    const DevPairGroup& G = groups[member_group[m]];
+    const double dx = r.x - refx[m], dy = r.y - refy[m], dz = r.z - refz[m];
+    if (!(dist2(dx, dy, dz) <= G.half_skin2)) atomicOr(&err[G.sys], kErrStale);
+}
+
+// ------------------------------------------------------------ list build
+__global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
+                              const int* __restrict__ member_group, const int* __restrict__ member_bead,
+                              const double* __restrict__ px, const double* __restrict__ py,
+                              const double* __restrict__ pz, int* __restrict__ member_cell,
+                              int* __restrict__ member_rank, int* __restrict__ cell_count) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const DevPairGroup& G = groups[member_group[m]];
+    const int b = member_bead[m];
+    int c[3] = {0, 0, 0};
+    if (G.periodic) {
+        const double p[3] = {px[b], py[b], pz[b]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double u = p[d] * G.inv_box[d];
+            u -= floor(u);
+            int ci = isfinite(u) ? static_cast<int>(u * G.ncell[d]) : 0;
+            c[d] = ci < 0 ? 0 : (ci >= G.ncell[d] ? G.ncell[d] - 1 : ci);
+        }
+    }
+    const int cell = G.cell_begin + (c[2] * G.ncell[1] + c[1]) * G.ncell[0] + c[0];
+    member_cell[m] = cell;
+    member_rank[m] = atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_place(int nm, const int* __restrict__ member_bead, const int* __restrict__ member_type,
+                        const int* __restrict__ member_cell, const int* __restrict__ member_rank,
+                        const int* __restrict__ cell_start, const double* __restrict__ px,
+                        const double* __restrict__ py, const double* __restrict__ pz,
+                        int* __restrict__ member_slot, int* __restrict__ slot_member,
+                        PairRec* __restrict__ rec, double* __restrict__ refx,
+                        double* __restrict__ refy, double* __restrict__ refz) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const int slot = cell_start[member_cell[m]] + member_rank[m];
+    member_slot[m] = slot;
+    slot_member[slot] = m;
+    const int b = member_bead[m];
+    PairRec r;
+    r.x = px[b]; r.y = py[b]; r.z = pz[b];
+    r.type = member_type[m];
+    r.bead = b;
+    rec[slot] = r;
+    refx[m] = r.x; refy[m] = r.y; refz[m] = r.z;
+}
+
+__device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, int mj) {
+    // exclusion rows are sorted by partner member index
+    int lo = beg, hi = beg + cnt;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = excl[mid];
+        if (v == mj) return true;
+        if (v < mj) lo = mid + 1; else hi = mid;
+    }
+    return false;
+}
+
+// Walks the cell stencil of slot i and calls visit(slot_j) for every listed
+// partner (r^2 < rl^2 at the build positions, allowed, not excluded).
+template <typename Visit>
+__device__ inline void nl_walk(int i, const DevPairGroup& G, const int* __restrict__ slot_member,
+                               const int* __restrict__ member_cell, const int* __restrict__ member_cat,
+                               const int* __restrict__ excl_start, const int* __restrict__ excl_count,
+                               const int* __restrict__ excl, const int* __restrict__ cell_start,
+                               const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
+                               Visit visit) {
+    const int m = slot_member[i];
+    const int cell = member_cell[m] - G.cell_begin;
+    const int cx = cell % G.ncell[0];
+    const int cy = (cell / G.ncell[0]) % G.ncell[1];
+    const int cz = cell / (G.ncell[0] * G.ncell[1]);
+    const uint32_t amask = G.allow[member_cat[m]];
+    const int eb = excl_start[m], ec = excl_count[m];
+    const PairRec ri = rec[i];
+    for (int dz = -G.reach[2]; dz <= G.reach[2]; ++dz) {
+        int z = cz + dz;
+        if (z < 0) z += G.ncell[2]; else if (z >= G.ncell[2]) z -= G.ncell[2];
+        for (int dy = -G.reach[1]; dy <= G.reach[1]; ++dy) {
+            int y = cy + dy;
+            if (y < 0) y += G.ncell[1]; else if (y >= G.ncell[1]) y -= G.ncell[1];
+            for (int dx = -G.reach[0]; dx <= G.reach[0]; ++dx) {
+                int x = cx + dx;
+                if (x < 0) x += G.ncell[0]; else if (x >= G.ncell[0]) x -= G.ncell[0];
+                const int c = G.cell_begin + (z * G.ncell[1] + y) * G.ncell[0] + x;
+                const int s0 = cell_start[c], s1 = s0 + cell_count[c];
+                for (int s = s0; s < s1; ++s) {
+                    if (s == i) continue;
+                    const int mj = slot_member[s];
+                    if (!((amask >> member_cat[mj]) & 1u)) continue;
+                    const PairRec rj = rec[s];
+                    double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
+                    if (G.periodic) {
+                        ddx = min_image(ddx, G.box[0], G.inv_box[0]);
+                        ddy = min_image(ddy, G.box[1], G.inv_box[1]);
+                        ddz = min_image(ddz, G.box[2], G.inv_box[2]);
+                    }
+                    if (!(dist2(ddx, ddy, ddz) < G.rl2)) continue;
+                    if (ec && excluded(excl, eb, ec, mj)) continue;
+                    visit(s);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_nl_count(int ns, const DevPairGroup* __restrict__ groups,
+                           const int* __restrict__ member_group, const int* __restrict__ slot_member,
+                           const int* __restrict__ member_cell, const int* __restrict__ member_cat,
+                           const int* __restrict__ excl_start, const int* __restrict__ excl_count,
+                           const int* __restrict__ excl, const int* __restrict__ cell_start,
+                           const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
+                           int* __restrict__ row_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const DevPairGroup& G = groups[member_group[slot_member[i]]];
+    int cnt = 0;
+    nl_walk(i, G, slot_member, member_cell, member_cat, excl_start, excl_count, excl, cell_start,
+            cell_count, rec, [&](int) { ++cnt; });
+    row_count[i] = cnt;
+}
+
+__global__ void k_nl_fill(int ns, const DevPairGroup* __restrict__ groups,
+                          const int* __restrict__ member_group, const int* __restrict__ slot_member,
+                          const int* __restrict__ member_cell, const int* __restrict__ member_cat,
+                          const int* __restrict__ excl_start, const int* __restrict__ excl_count,
+                          const int* __restrict__ excl, const int* __restrict__ cell_start,
+                          const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
+                          const int* __restrict__ row_start, int* __restrict__ row_count,
+                          const int* __restrict__ total, long long capacity, int* __restrict__ col,
+                          int* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    if (static_cast<long long>(*total) > capacity) {
+        row_count[i] = 0;  // the pair kernel then reads nothing
+        if (i == 0) atomicOr(flags, kFlagListOverflow);
+        return;
+    }
+    const DevPairGroup& G = groups[member_group[slot_member[i]]];
+    const int base = row_start[i];
+    int k = 0;
+    nl_walk(i, G, slot_member, member_cell, member_cat, excl_start, excl_count, excl, cell_start,
+            cell_count, rec, [&](int s) { col[base + k] = s; ++k; });
+}
+
+// -------------------------------------------------------------------- scan
+// Exclusive scan of n ints in three launches: per-block scan (kScanItems
+// items per block), scan of the block sums (one block), add-back.
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 8;
+constexpr int kScanItems = kScanThreads * kScanPerThread;
+
+__device__ inline int block_exclusive_scan(int v, int* sh, int* block_total) {
+    // v is this thread's sum; returns exclusive prefix among threads
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) sh[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) { const int t = sh[w]; sh[w] = run; run += t; }
+        sh[kScanThreads / 64] = run;
+    }
+    __syncthreads();
+    const int res = incl - v + sh[wid];
+    *block_total = sh[kScanThreads / 64];
+    __syncthreads();
+    return res;
+}
+
+__global__ void k_scan_blocks(const int* __restrict__ in, int* __restrict__ out, int n,
+                              int* __restrict__ partial) {
+    __shared__ int sh[kScanThreads / 64 + 1];
+    const int base = blockIdx.x * kScanItems + threadIdx.x * kScanPerThread;
+    int vals[kScanPerThread];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        vals[k] = (base + k < n) ? in[base + k] : 0;
+        sum += vals[k];
+    }
+    int tot;
+    int run = block_exclusive_scan(sum, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += vals[k];
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Single block; scans nb partial sums in place and writes the grand total.
+__global__ void k_scan_partials(int* __restrict__ partial, int nb, int* __restrict__ total) {
+    __shared__ int sh[kScanThreads / 64 + 1];
+    int carry = 0;
+    for (int base = 0; base < nb; base += kScanItems) {
+        const int b0 = base + threadIdx.x * kScanPerThread;
+        int vals[kScanPerThread];
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k) {
+            vals[k] = (b0 + k < nb) ? partial[b0 + k] : 0;
+            sum += vals[k];
+        }
+        int tot;
+        int run = block_exclusive_scan(sum, sh, &tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k) {
+            if (b0 + k < nb) partial[b0 + k] = run;
+            run += vals[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_add(int* __restrict__ out, int n, const int* __restrict__ partial) {
+    const int add = partial[blockIdx.x];
+    const int base = blockIdx.x * kScanItems + threadIdx.x * kScanPerThread;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+// ----------------------------------------------------------------- bonded
+template <typename R> struct V3T { R x, y, z; };
+
+template <typename R>
+__device__ inline V3T<R> bdisp(const double* __restrict__ px, const double* __restrict__ py,
+                               const double* __restrict__ pz, int i, int j, const DevSys& S, bool pbc) {
+    double dx = px[j] - px[i], dy = py[j] - py[i], dz = pz[j] - pz[i];
+    if (pbc && S.periodic) {
+        dx = min_image(dx, S.box[0], S.inv_box[0]);
+        dy = min_image(dy, S.box[1], S.inv_box[1]);
+        dz = min_image(dz, S.box[2], S.inv_box[2]);
+    }
+    return {static_cast<R>(dx), static_cast<R>(dy), static_cast<R>(dz)};
+}
+template <typename R> __device__ inline V3T<R> cross3(V3T<R> a, V3T<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename R> __device__ inline R dot3(V3T<R> a, V3T<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+template <typename R> __device__ inline R r_sqrt(R v);
+template <> __device__ inline float r_sqrt<float>(float v) { return sqrtf(v); }
+template <> __device__ inline double r_sqrt<double>(double v) { return sqrt(v); }
+template <typename R> __device__ inline R r_exp(R v);
+template <> __device__ inline float r_exp<float>(float v) { return expf(v); }
+template <> __device__ inline double r_exp<double>(double v) { return exp(v); }
+template <typename R> __device__ inline R r_atan2(R a, R b);
+template <> __device__ inline float r_atan2<float>(float a, float b) { return atan2f(a, b); }
+template <> __device__ inline double r_atan2<double>(double a, double b) { return atan2(a, b); }
+template <typename R> __device__ inline R r_floor(R v);
+template <> __device__ inline float r_floor<float>(float v) { return floorf(v); }
+template <> __device__ inline double r_floor<double>(double v) { return floor(v); }
+template <typename R> __device__ inline void r_sincos(R v, R* s, R* c);
+template <> __device__ inline void r_sincos<float>(float v, float* s, float* c) { sincosf(v, s, c); }
+template <> __device__ inline void r_sincos<double>(double v, double* s, double* c) { sincos(v, s, c); }
+
+struct ForceOut {
+    long long* fx;
+    long long* fy;
+    long long* fz;
+};
+
+template <typename R>
+__device__ inline void add_force(const ForceOut& o, int b, R gx, R gy, R gz, int& bad) {
+    // force on bead b = g (already the negative gradient)
+    if constexpr (sizeof(R) == 4) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fx[b]), static_cast<unsigned long long>(to_fix(gx, bad)));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fy[b]), static_cast<unsigned long long>(to_fix(gy, bad)));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fz[b]), static_cast<unsigned long long>(to_fix(gz, bad)));
+    } else {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fx[b]), static_cast<unsigned long long>(to_fix_d(gx, bad)));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fy[b]), static_cast<unsigned long long>(to_fix_d(gy, bad)));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fz[b]), static_cast<unsigned long long>(to_fix_d(gz, bad)));
+    }
+}
+
+// One bonded term.  R = float for the force pass, double for the energy pass.
+template <bool ENERGY, typename R>
+__device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double* __restrict__ px,
+                                   const double* __restrict__ py, const double* __restrict__ pz,
+                                   const FlaTable* __restrict__ fla, const FldTable* __restrict__ fld,
+                                   const ForceOut& out, long long* __restrict__ energy, int sys,
+                                   int& bad) {
+    const bool pbc = (t.meta >> 5) & 1;
+    const int group = t.meta & 31;
+    const int table = t.meta >> 8;
+    R e = 0;
+    switch (t.kind) {
+    case 1: case 2: case 3: {  // harmonic bond, Gaussian bond, Go contact
+        const V3T<R> d = bdisp<R>(px, py, pz, t.idx[0], t.idx[1], S, pbc);
+        const R r = r_sqrt<R>(dot3(d, d));
+        R dedr;
+        if (t.kind == 1) {
+            const R r0 = static_cast<R>(t.p[0]), k = static_cast<R>(t.p[1]);
+            const R dr = r - r0;
+            e = R(0.5) * k * dr * dr;
+            dedr = k * dr;
+        } else if (t.kind == 2) {
+            const R k = static_cast<R>(t.p[0]), v0 = static_cast<R>(t.p[1]), s = static_cast<R>(t.p[2]);
+            const R dr = r - v0;
+            const R is2 = R(1) / (s * s);
+            e = k * r_exp<R>(-R(0.5) * dr * dr * is2);
+            dedr = -e * dr * is2;
+        } else {
+            const R k = static_cast<R>(t.p[0]), r0 = static_cast<R>(t.p[1]);
+            const R q = r0 / r;
+            const R q2 = q * q, q4 = q2 * q2, q10 = q4 * q4 * q2, q12 = q10 * q2;
+            e = k * (R(5) * q12 - R(6) * q10);
+            dedr = R(60) * k * (q10 - q12) / r;
+        }
+        if (!ENERGY) {
+            const R s = dedr / r;
+            add_force<R>(out, t.idx[0], d.x * s, d.y * s, d.z * s, bad);
+            add_force<R>(out, t.idx[1], -d.x * s, -d.y * s, -d.z * s, bad);
+        }
+        break;
+    }
+    case 4: case 5: {  // harmonic angle, flexible local angle
+        const V3T<R> a = bdisp<R>(px, py, pz, t.idx[1], t.idx[0], S, pbc);  // x_i - x_j
+        const V3T<R> c = bdisp<R>(px, py, pz, t.idx[1], t.idx[2], S, pbc);  // x_k - x_j
+        const V3T<R> p = cross3(a, c);
+        const R pn = r_sqrt<R>(dot3(p, p));
+        const R th = r_atan2<R>(pn, dot3(a, c));
+        R dedt;
+        if (t.kind == 4) {
+            const R t0 = static_cast<R>(t.p[0]), k = static_cast<R>(t.p[1]);
+            e = R(0.5) * k * (th - t0) * (th - t0);
+            dedt = k * (th - t0);
+        } else {
+            const FlaTable& T = fla[table];
+            const R k = static_cast<R>(t.p[0]);
+            const R tmin = static_cast<R>(T.tmin), tmax = static_cast<R>(T.tmax), h = static_cast<R>(T.h);
+            R u = -static_cast<R>(T.emin), du = 0;
+            if (th >= tmin && th <= tmax) {
+                int lo = static_cast<int>(r_floor<R>((th - tmin) / h));
+                lo = lo < 0 ? 0 : (lo > 8 ? 8 : lo);
+                const R xl = tmin + lo * h, xu = tmin + (lo + 1) * h;
+                const R A = (xu - th) / h, B = (th - xl) / h;
+                const R ylo = static_cast<R>(T.y[lo]), yhi = static_cast<R>(T.y[lo + 1]);
+                const R dlo = static_cast<R>(T.y2[lo]), dhi = static_cast<R>(T.y2[lo + 1]);
+                u += A * ylo + B * yhi + ((A * A * A - A) * dlo + (B * B * B - B) * dhi) * h * h / R(6);
+                du += (yhi - ylo) / h + ((R(1) - R(3) * A * A) * dlo + (R(3) * B * B - R(1)) * dhi) * h / R(6);
+            }
+            if (tmin - th >= R(0)) { u += R(30) * (tmin - th) + static_cast<R>(T.y[0]); du -= R(30); }
+            if (th - tmax >= R(0)) { u += R(30) * (th - tmax) + static_cast<R>(T.y[9]); du += R(30); }
+            e = k * u;
+            dedt = k * du;
+        }
+        if (!ENERGY && pn > R(0)) {
+            const R ia = R(1) / (dot3(a, a) * pn), ic = R(1) / (dot3(c, c) * pn);
+            const V3T<R> ap = cross3(a, p), pc = cross3(p, c);
+            const V3T<R> gi = {ap.x * ia, ap.y * ia, ap.z * ia};
+            const V3T<R> gk = {pc.x * ic, pc.y * ic, pc.z * ic};
+            add_force<R>(out, t.idx[0], -dedt * gi.x, -dedt * gi.y, -dedt * gi.z, bad);
+            add_force<R>(out, t.idx[2], -dedt * gk.x, -dedt * gk.y, -dedt * gk.z, bad);
+            add_force<R>(out, t.idx[1], dedt * (gi.x + gk.x), dedt * (gi.y + gk.y), dedt * (gi.z + gk.z), bad);
+        }
+        break;
+    }
+    case 6: case 7: {  // Gaussian dihedral, flexible local dihedral
+        const V3T<R> b1 = bdisp<R>(px, py, pz, t.idx[0], t.idx[1], S, pbc);
+        const V3T<R> b2 = bdisp<R>(px, py, pz, t.idx[1], t.idx[2], S, pbc);
+        const V3T<R> b3 = bdisp<R>(px, py, pz, t.idx[2], t.idx[3], S, pbc);
+        const V3T<R> n1 = cross3(b1, b2), n2 = cross3(b2, b3);
+        const R b2n = r_sqrt<R>(dot3(b2, b2));
+        const R phi = r_atan2<R>(b2n * dot3(b1, n2), dot3(n1, n2));
+        R dedp;
+        if (t.kind == 6) {
+            const R k = static_cast<R>(t.p[0]), p0 = static_cast<R>(t.p[1]), s = static_cast<R>(t.p[2]);
+            const R pi = static_cast<R>(3.1415926535897932385);
+            const R dt = phi - p0;
+            const R dtp = dt - r_floor<R>((dt + pi) / (R(2) * pi)) * (R(2) * pi);
+            const R is2 = R(1) / (s * s);
+            e = k * r_exp<R>(-R(0.5) * dtp * dtp * is2);
+            dedp = -e * dtp * is2;
+        } else {
+            const FldTable& T = fld[table];
+            const R k = static_cast<R>(t.p[0]);
+            R s1, c1;
+            r_sincos<R>(phi, &s1, &c1);
+            const R s2 = R(2) * s1 * c1, c2 = c1 * c1 - s1 * s1;
+            const R s3 = s1 * c2 + c1 * s2, c3 = c1 * c2 - s1 * s2;
+            const R a1 = static_cast<R>(T.c[1]), b1c = static_cast<R>(T.c[2]);
+            const R a2 = static_cast<R>(T.c[3]), b2c = static_cast<R>(T.c[4]);
+            const R a3 = static_cast<R>(T.c[5]), b3c = static_cast<R>(T.c[6]);
+            const R u = static_cast<R>(T.c[0] - T.c[7]) + a1 * c1 + b1c * s1 + a2 * c2 + b2c * s2 + a3 * c3 + b3c * s3;
+            const R du = -a1 * s1 + b1c * c1 + R(2) * (-a2 * s2 + b2c * c2) + R(3) * (-a3 * s3 + b3c * c3);
+            e = k * u;
+            dedp = k * du;
+        }
+        if (!ENERGY) {
+            const R n1s = dot3(n1, n1), n2s = dot3(n2, n2);
+            if (n1s > R(0) && n2s > R(0) && b2n > R(0)) {
+                const R fi = -b2n / n1s, fl = b2n / n2s;
+                const V3T<R> gi = {n1.x * fi, n1.y * fi, n1.z * fi};
+                const V3T<R> gl = {n2.x * fl, n2.y * fl, n2.z * fl};
+                const R ib2 = R(1) / (b2n * b2n);
+                const R f1 = dot3(b1, b2) * ib2, f3 = dot3(b3, b2) * ib2;
+                // middle beads: -(1 + f1) g_i + f3 g_l and f1 g_i - (1 + f3) g_l
+                const R a1 = -(R(1) + f1), a3 = -(R(1) + f3);
+                const V3T<R> gj = {gi.x * a1 + gl.x * f3, gi.y * a1 + gl.y * f3, gi.z * a1 + gl.z * f3};
+                const V3T<R> gk = {gi.x * f1 + gl.x * a3, gi.y * f1 + gl.y * a3, gi.z * f1 + gl.z * a3};
+                add_force<R>(out, t.idx[0], -dedp * gi.x, -dedp * gi.y, -dedp * gi.z, bad);
+                add_force<R>(out, t.idx[1], -dedp * gj.x, -dedp * gj.y, -dedp * gj.z, bad);
+                add_force<R>(out, t.idx[2], -dedp * gk.x, -dedp * gk.y, -dedp * gk.z, bad);
+                add_force<R>(out, t.idx[3], -dedp * gl.x, -dedp * gl.y, -dedp * gl.z, bad);
+            }
+        }
+        break;
+    }
+    default:
+        break;
+    }
+    if (ENERGY) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&energy[sys * 32 + group]),
+                  static_cast<unsigned long long>(to_fix_d(static_cast<double>(e), bad)));
+    }
+}
+
+template <bool ENERGY>
+__global__ void k_bonded(int nt, const BTerm* __restrict__ terms, const DevSys* __restrict__ sys,
+                         const int* __restrict__ sys_of, const double* __restrict__ px,
+                         const double* __restrict__ py, const double* __restrict__ pz,
+                         const FlaTable* __restrict__ fla, const FldTable* __restrict__ fld,
+                         ForceOut out, long long* __restrict__ energy, uint32_t groups,
+                         int* __restrict__ err) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const BTerm T = terms[t];
+    if (!((groups >> (T.meta & 31)) & 1u)) return;
+    const int s = sys_of[T.idx[0]];
+    const DevSys S = sys[s];
+    int bad = 0;
+    if (ENERGY)
+        bonded_term<true, double>(T, S, px, py, pz, fla, fld, out, energy, s, bad);
+    else
+        bonded_term<false, float>(T, S, px, py, pz, fla, fld, out, energy, s, bad);
+    if (bad) atomicOr(&err[s], kErrNumeric);
+}
+
+// ------------------------------------------------------------------- pair
+// Energy terms and (dE/dr)/r of all kinds of a pair group for one pair.
+// R = float (force pass, FP32 type rows tf) or double (energy pass, rows td).
+template <typename R>
+__device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* __restrict__ tfi,
+                                 const float* __restrict__ tfj, const double* __restrict__ tdi,
+                                 const double* __restrict__ tdj, R r2, R* dedr_r, R e[4]) {
+    auto P = [&](bool i_side, int k) -> R {
+        if constexpr (sizeof(R) == 4) return i_side ? tfi[k] : tfj[k];
+        else return i_side ? tdi[k] : tdj[k];
+    };
+    const R ir2 = R(1) / r2;
+    R f = 0;
+    R r = 0;
+    if ((kinds & kDh) || ((kinds & kLja) && G.lja_rsw > 0)) r = r_sqrt<R>(r2);
+    if (kinds & kExv) {
+        const R s = P(true, 0) + P(false, 0);
+        const R s2 = s * s;
+        const R q2 = s2 * ir2;
+        const R q6 = q2 * q2 * q2, q12 = q6 * q6;
+        const R s6 = s2 * s2 * s2;
+        const R eps = static_cast<R>(G.exv_eps);
+        f += R(-12) * eps * q12 * ir2;
+        e[0] = eps * (q12 - s6 * s6 * static_cast<R>(G.exv_corr));
+    }
+    if (kinds & kDh) {
+        const R qq = P(true, 1) * P(false, 1);
+        const R il = static_cast<R>(G.dh_inv_lambda);
+        const R ex = r_exp<R>(-r * il);
+        const R K = static_cast<R>(G.dh_K);
+        const R ir = R(1) / r;
+        f += -qq * K * ex * (ir + il) * ir2;
+        e[1] = qq * K * (ex * ir - static_cast<R>(G.dh_corr));
+    }
+    const R c13 = static_cast<R>(1.2599210498948731648);  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
+    if (kinds & kLjr) {
+        const R eb = R(0.5) * (P(true, 2) + P(false, 2));
+        const R sb = R(0.5) * (P(true, 3) + P(false, 3));
+        const R s2 = sb * sb * ir2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        if (c13 * sb * sb - r2 >= R(0)) {
+            f += eb * (R(-48) * s12 + R(24) * s6) * ir2;
+            e[2] = eb * (R(4) * (s12 - s6) + R(1));
+        } else {
+            e[2] = 0;
+        }
+    }
+    if (kinds & kLja) {
+        const R eb = R(0.5) * (P(true, 4) + P(false, 4));
+        const R sb = R(0.5) * (P(true, 5) + P(false, 5));
+        const R s2 = sb * sb * ir2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        const R th2 = c13 * sb * sb;
+        R u = 0, du = 0;  // du = (dU/dr)/r
+        if (r2 - th2 >= R(0)) {
+            u += R(4) * eb * (s12 - s6);
+            du += eb * (R(-48) * s12 + R(24) * s6) * ir2;
+        }
+        if (th2 - r2 >= R(0)) u -= eb;
+        if (G.lja_rsw > 0) {
+            const R rsw = static_cast<R>(G.lja_rsw);
+            if (r > rsw) {
+                const R inv = R(1) / (static_cast<R>(G.rc) - rsw);
+                const R x = (r - rsw) * inv;
+                const R x2 = x * x, x3 = x2 * x;
+                const R S = R(1) - x3 * (R(10) - R(15) * x + R(6) * x2);
+                const R dS = R(-30) * x2 * (R(1) - x) * (R(1) - x) * inv;
+                du = du * S + u * dS / r;
+                u = u * S;
+            }
+        }
+        f += du;
+        e[3] = u;
+    }
+    *dedr_r = f;
+}
+
+template <bool ENERGY, int TEAM>
+__global__ __launch_bounds__(kPairBlock) void k_pair(
+    const DevPairGroup* __restrict__ groups, const int2* __restrict__ blocks,
+    const PairRec* __restrict__ rec, const int* __restrict__ row_start,
+    const int* __restrict__ row_count, const int* __restrict__ col,
+    const float* __restrict__ types_f, const double* __restrict__ types_d, ForceBuf out,
+    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err) {
+    extern __shared__ float lds_types[];
+    const int2 bt = blocks[blockIdx.x];
+    const DevPairGroup& G = groups[bt.x];
+    if (!ENERGY) {
+        const float* src = types_f + static_cast<size_t>(G.type_begin) * kTypeStride;
+        for (int k = threadIdx.x; k < G.n_types * kTypeStride; k += blockDim.x) lds_types[k] = src[k];
+        __syncthreads();
+    }
+    int kinds = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (((G.kinds >> k) & 1) && ((gmask >> G.egroup[k]) & 1u)) kinds |= 1 << k;
+    const int lane = threadIdx.x % TEAM;
+    const int row = bt.y + threadIdx.x / TEAM;
+    const int row_end = G.member_begin + G.member_count;
+    long long ax = 0, ay = 0, az = 0;
+    long long ae0 = 0, ae1 = 0, ae2 = 0, ae3 = 0;
+    int bad = 0;
+    int bead_i = -1;
+    if (row < row_end && kinds) {
+        const PairRec ri = rec[row];
+        bead_i = ri.bead;
+        const int beg = row_start[row], cnt = row_count[row];
+        const float* tfi = lds_types + ri.type * kTypeStride;
+        const double* tdi = types_d + static_cast<size_t>(G.type_begin + ri.type) * kTypeStride;
+        for (int k = lane; k < cnt; k += TEAM) {
+            const int s = col[beg + k];
+            const PairRec rj = rec[s];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+            if (G.periodic) {
+                dx = min_image(dx, G.box[0], G.inv_box[0]);
+                dy = min_image(dy, G.box[1], G.inv_box[1]);
+                dz = min_image(dz, G.box[2], G.inv_box[2]);
+            }
+            const double r2 = dist2(dx, dy, dz);
+            if (!(r2 < G.rc2)) continue;
+            if (ENERGY) {
+                if (row < s) {
+                    double f;
+                    double e[4] = {0, 0, 0, 0};
+                    const double* tdj = types_d + static_cast<size_t>(G.type_begin + rj.type) * kTypeStride;
+                    pair_eval<double>(kinds, G, nullptr, nullptr, tdi, tdj, r2, &f, e);
+                    ae0 += to_fix_d(e[0], bad);
+                    ae1 += to_fix_d(e[1], bad);
+                    ae2 += to_fix_d(e[2], bad);
+                    ae3 += to_fix_d(e[3], bad);
+                }
+            } else {
+                float f;
+                float e[4];
+                const float* tfj = lds_types + rj.type * kTypeStride;
+                pair_eval<float>(kinds, G, tfi, tfj, nullptr, nullptr, static_cast<float>(r2), &f, e);
+                ax += to_fix(f * static_cast<float>(dx), bad);
+                ay += to_fix(f * static_cast<float>(dy), bad);
+                az += to_fix(f * static_cast<float>(dz), bad);
+            }
+        }
+    }
+    if (ENERGY) {
+        ae0 = warp_sum(ae0, TEAM);
+        ae1 = warp_sum(ae1, TEAM);
+        ae2 = warp_sum(ae2, TEAM);
+        ae3 = warp_sum(ae3, TEAM);
+        if (lane == 0 && bead_i >= 0) {
+            const long long v[4] = {ae0, ae1, ae2, ae3};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if ((kinds >> k) & 1)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&energy[G.sys * 32 + G.egroup[k]]),
+                              static_cast<unsigned long long>(v[k]));
+        }
+    } else {
+        ax = warp_sum(ax, TEAM);
+        ay = warp_sum(ay, TEAM);
+        az = warp_sum(az, TEAM);
+        if (lane == 0 && bead_i >= 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&out.fx[bead_i]), static_cast<unsigned long long>(ax));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&out.fy[bead_i]), static_cast<unsigned long long>(ay));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&out.fz[bead_i]), static_cast<unsigned long long>(az));
+        }
+    }
+    if (bad) atomicOr(&err[G.sys], kErrNumeric);
+}
+
+// ------------------------------------------------------------- energies
+__global__ void k_kinetic(int nb, const int* __restrict__ sys_of, const double* __restrict__ mass,
+                          const double* __restrict__ vx, const double* __restrict__ vy,
+                          const double* __restrict__ vz, long long* __restrict__ ke,
+                          int* __restrict__ err) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const double e = 0.5 * mass[b] * (vx[b] * vx[b] + vy[b] * vy[b] + vz[b] * vz[b]);
+    int bad = 0;
+    const long long q = to_fix_d(e, bad);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&ke[sys_of[b]]), static_cast<unsigned long long>(q));
+    if (bad) atomicOr(&err[sys_of[b]], kErrNumeric);
+}
+
+// Lists the interacting pairs (r < cutoff) of one pair group as bead pairs
+// i<j, using the same distance test as k_pair.
+__global__ void k_neighbor_dump(int group, const DevPairGroup* __restrict__ groups,
+                                const PairRec* __restrict__ rec, const int* __restrict__ row_start,
+                                const int* __restrict__ row_count, const int* __restrict__ col,
+                                int* __restrict__ count, int2* __restrict__ out, long long cap) {
+    const DevPairGroup& G = groups[group];
+    const int row = G.member_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= G.member_begin + G.member_count) return;
+    const PairRec ri = rec[row];
+    const int beg = row_start[row], cnt = row_count[row];
+    for (int k = 0; k < cnt; ++k) {
+        const PairRec rj = rec[col[beg + k]];
+        if (!(ri.bead < rj.bead)) continue;
+        double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+        if (G.periodic) {
+            dx = min_image(dx, G.box[0], G.inv_box[0]);
+            dy = min_image(dy, G.box[1], G.inv_box[1]);
+            dz = min_image(dz, G.box[2], G.inv_box[2]);
+        }
+        if (!(dist2(dx, dy, dz) < G.rc2)) continue;
+        const int slot = atomicAdd(count, 1);
+        if (slot < cap) out[slot] = make_int2(ri.bead, rj.bead);
+    }
+}
+
+// Maxwell-Boltzmann velocities for the beads of one system (Philox stream 1).
+__global__ void k_velocity_noise(int nb, int sys, const int* __restrict__ sys_of,
+                                 const int* __restrict__ local_id, const double* __restrict__ inv_m,
+                                 double kT, uint64_t seed, double* __restrict__ vx,
+                                 double* __restrict__ vy, double* __restrict__ vz) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb || sys_of[b] != sys) return;
+    double g[3];
+    gauss3(seed, 0, static_cast<uint32_t>(local_id[b]), 1u, g);
+    const double s = sqrt(kT * inv_m[b]);
+    vx[b] = s * g[0];
+    vy[b] = s * g[1];
+    vz[b] = s * g[2];
+}
+
+// ------------------------------------------------------------ launchers
+static inline int nblk(long long n, int b) { return static_cast<int>((n + b - 1) / b); }
+
+void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st) {
+    if (d.nb == 0) return;
+    k_fill_i64<<<nblk(d.nb, 256), 256, 0, st>>>(f.fx, 0, d.nb);
+    k_fill_i64<<<nblk(d.nb, 256), 256, 0, st>>>(f.fy, 0, d.nb);
+    k_fill_i64<<<nblk(d.nb, 256), 256, 0, st>>>(f.fz, 0, d.nb);
+}
+
+void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending) {
+    k_integrate<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
+                                                 d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
+                                                 pin, pout, pending, d.err);
+}
+
+void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur) {
+    k_half_kick<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.inv_m, d.vx, d.vy, d.vz, d.f.fx,
+                                                 d.f.fy, d.f.fz, d.gstep, pcur);
+}
+
+void launch_pack(const DevPtrs& d, hipStream_t st) {
+    if (d.nm == 0) return;
+    k_pack<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.groups, d.member_group, d.member_bead, d.member_type,
+                                            d.member_slot, d.px, d.py, d.pz, d.refx, d.refy, d.refz, d.rec,
+                                            d.err);
+}
+
+static void launch_scan(const DevPtrs& d, hipStream_t st, const int* in, int* out, int n, int* total) {
+    const int nb = nblk(n, kScanItems);
+    k_scan_blocks<<<nb, kScanThreads, 0, st>>>(in, out, n, d.scan_partial);
+    k_scan_partials<<<1, kScanThreads, 0, st>>>(d.scan_partial, nb, total);
+    k_scan_add<<<nb, kScanThreads, 0, st>>>(out, n, d.scan_partial);
+}
+
+void launch_rebuild(const DevPtrs& d, hipStream_t st) {
+    if (d.nm == 0) return;
+    k_fill_i32<<<nblk(d.ncell, 256), 256, 0, st>>>(d.cell_count, 0, d.ncell);
+    k_cell_assign<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.groups, d.member_group, d.member_bead, d.px, d.py,
+                                                   d.pz, d.member_cell, d.member_rank, d.cell_count);
+    launch_scan(d, st, d.cell_count, d.cell_start, d.ncell, d.scan_total);
+    k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
+                                             d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
+                                             d.rec, d.refx, d.refy, d.refz);
+    k_nl_count<<<nblk(d.nm, 128), 128, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
+                                                 d.member_cat, d.excl_start, d.excl_count, d.excl,
+                                                 d.cell_start, d.cell_count, d.rec, d.row_count);
+    launch_scan(d, st, d.row_count, d.row_start, d.nm, d.scan_total + 1);
+    k_nl_fill<<<nblk(d.nm, 128), 128, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
+                                               d.member_cat, d.excl_start, d.excl_count, d.excl, d.cell_start,
+                                               d.cell_count, d.rec, d.row_start, d.row_count, d.scan_total + 1,
+                                               d.list_capacity, d.col, d.flags);
+}
+
+void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {
+    if (d.nt == 0) return;
+    ForceOut o{f.fx, f.fy, f.fz};
+    if (energy)
+        k_bonded<true><<<nblk(d.nt, 256), 256, 0, st>>>(d.nt, d.terms, d.sys, d.sys_of, d.px, d.py, d.pz,
+                                                        d.fla, d.fld, o, d.energy, groups, d.err);
+    else
+        k_bonded<false><<<nblk(d.nt, 256), 256, 0, st>>>(d.nt, d.terms, d.sys, d.sys_of, d.px, d.py, d.pz,
+                                                         d.fla, d.fld, o, d.energy, groups, d.err);
+}
+
+template <bool E>
+static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, uint32_t groups) {
+    const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
+    const dim3 grid(d.n_pair_blocks), block(kPairBlock);
+#define OCM_PAIR(T)                                                                                   \
+    k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, \
+                                           d.col, d.types_f, d.types_d, f, d.energy, groups, d.err)
+    switch (d.team) {
+    case 4: OCM_PAIR(4); break;
+    case 8: OCM_PAIR(8); break;
+    case 32: OCM_PAIR(32); break;
+    case 64: OCM_PAIR(64); break;
+    default: OCM_PAIR(16); break;
+    }
+#undef OCM_PAIR
+}
+
+void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {
+    if (d.n_pair_blocks == 0) return;
+    if (energy) pair_dispatch<true>(d, st, f, groups);
+    else pair_dispatch<false>(d, st, f, groups);
+}
+
+void launch_zero_energy(const DevPtrs& d, hipStream_t st) {
+    k_fill_i64<<<nblk(d.nsys * 32, 256), 256, 0, st>>>(d.energy, 0, d.nsys * 32);
+    k_fill_i64<<<nblk(d.nsys, 256), 256, 0, st>>>(d.ke, 0, d.nsys);
+}
+
+void launch_kinetic(const DevPtrs& d, hipStream_t st) {
+    k_kinetic<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys_of, d.mass, d.vx, d.vy, d.vz, d.ke, d.err);
+}
+
+void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members) {
+    k_fill_i32<<<1, 64, 0, st>>>(d.dump_count, 0, 1);
+    if (members == 0) return;
+    k_neighbor_dump<<<nblk(members, 128), 128, 0, st>>>(group, d.groups, d.rec, d.row_start, d.row_count,
+                                                        d.col, d.dump_count, d.dump_pairs, cap);
+}
+
+void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed) {
+    k_velocity_noise<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, sys, d.sys_of, d.local_id, d.inv_m, kT, seed,
+                                                      d.vx, d.vy, d.vz);
+}
+
+}  // namespace ocm

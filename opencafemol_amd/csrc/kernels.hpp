@@ -1,0 +1,73 @@
+// kernels.hpp -- device buffer table and launch entry points of the step stages.
+#pragma once
+
+#include "device_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ocm {
+
+struct ForceBuf {
+    long long* fx;
+    long long* fy;
+    long long* fz;
+};
+
+// Every device array of a packed batch.  Sizes: nb beads, nsys systems,
+// nt bonded terms, nm pair-group members (= slots), ncell cells.
+struct DevPtrs {
+    int nb = 0, nsys = 0, nt = 0, nm = 0, ncell = 0, ngroups = 0;
+    DevSys* sys = nullptr;
+    int* sys_of = nullptr;
+    int* local_id = nullptr;
+    double* inv_m = nullptr;
+    double* mass = nullptr;
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    ForceBuf f{};          // step forces
+    ForceBuf f2{};         // scratch forces for group-restricted getState
+    long long* gstep = nullptr;   // two slots, see k_integrate
+    int* err = nullptr;           // nsys error words
+    int* flags = nullptr;         // batch-wide flag word
+    BTerm* terms = nullptr;
+    FlaTable* fla = nullptr;
+    FldTable* fld = nullptr;
+    DevPairGroup* groups = nullptr;
+    int *member_group = nullptr, *member_bead = nullptr, *member_type = nullptr, *member_cat = nullptr;
+    int *member_slot = nullptr, *member_cell = nullptr, *member_rank = nullptr;
+    int *excl_start = nullptr, *excl_count = nullptr, *excl = nullptr;
+    double *refx = nullptr, *refy = nullptr, *refz = nullptr;
+    PairRec* rec = nullptr;
+    int* slot_member = nullptr;
+    int *cell_count = nullptr, *cell_start = nullptr;
+    int *row_count = nullptr, *row_start = nullptr, *col = nullptr;
+    long long list_capacity = 0;
+    int* scan_partial = nullptr;
+    int* scan_total = nullptr;     // [0] cells total, [1] list total
+    float* types_f = nullptr;
+    double* types_d = nullptr;
+    int max_types = 0;
+    int2* pair_blocks = nullptr;   // (group, first slot) per pair-kernel block
+    int n_pair_blocks = 0;
+    int team = 16;
+    long long* energy = nullptr;   // nsys * 32 fixed point
+    long long* ke = nullptr;       // nsys fixed point
+    int* dump_count = nullptr;
+    int2* dump_pairs = nullptr;
+};
+
+constexpr int kPairBlock = 256;
+
+void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st);
+void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending);
+void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur);
+void launch_pack(const DevPtrs& d, hipStream_t st);
+void launch_rebuild(const DevPtrs& d, hipStream_t st);
+void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups);
+void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups);
+void launch_zero_energy(const DevPtrs& d, hipStream_t st);
+void launch_kinetic(const DevPtrs& d, hipStream_t st);
+void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members);
+void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed);
+
+}  // namespace ocm

@@ -1,0 +1,120 @@
+"""ctypes face of oracle/liboracle.so (test infrastructure only).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to
+check or time the CPU restatement; the product never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from opencafemol_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        L.oracle_eval.argtypes = [P(abi.SystemDesc), P(C.c_double), C.c_uint32, P(C.c_double), P(C.c_double)]
+        L.oracle_eval.restype = C.c_int
+        L.oracle_neighbor_pairs.argtypes = [P(abi.SystemDesc), C.c_int32, P(C.c_double), C.c_double,
+                                            P(C.c_int32), C.c_int64]
+        L.oracle_neighbor_pairs.restype = C.c_int64
+        L.oracle_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        L.oracle_philox4x32_10.restype = None
+        L.oracle_gauss3.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_double)]
+        L.oracle_gauss3.restype = None
+        L.oracle_natural_spline.argtypes = [P(C.c_double), C.c_int32, C.c_double, P(C.c_double)]
+        L.oracle_natural_spline.restype = None
+        L.oracle_fla_min_energy.argtypes = [P(C.c_double), P(C.c_double), C.c_double, C.c_double]
+        L.oracle_fla_min_energy.restype = C.c_double
+        L.oracle_fld_min_energy.argtypes = [P(C.c_double)]
+        L.oracle_fld_min_energy.restype = C.c_double
+        L.oracle_baoab.argtypes = [P(abi.SystemDesc), P(abi.IntegratorDesc), P(C.c_double), P(C.c_double),
+                                   C.c_int64, C.c_int64]
+        L.oracle_baoab.restype = C.c_int
+        L.oracle_kinetic.argtypes = [P(abi.SystemDesc), P(C.c_double)]
+        L.oracle_kinetic.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def evaluate(system, x, groups=0xFFFFFFFF):
+    """(forces (n,3), energy_by_group (32,)) of a SystemSpec at positions x (nm)."""
+    sd, keep = system.descriptor()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    F = np.zeros((system.n, 3))
+    E = np.zeros(32)
+    lib().oracle_eval(C.byref(sd), _dp(x), groups, _dp(F), _dp(E))
+    del keep
+    return F, E
+
+
+def neighbor_pairs(system, force_index, x, radius=None):
+    sd, keep = system.descriptor()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    r = system.forces[force_index].cutoff if radius is None else radius
+    n = lib().oracle_neighbor_pairs(C.byref(sd), force_index, _dp(x), r, None, 0)
+    out = np.zeros((max(n, 1), 2), dtype=np.int32)
+    lib().oracle_neighbor_pairs(C.byref(sd), force_index, _dp(x), r, out.ctypes.data_as(C.POINTER(C.c_int32)), n)
+    del keep
+    return out[:n]
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox4x32_10(c, k, o)
+    return [o[i] for i in range(4)]
+
+
+def gauss3(seed, step, bead, stream=0):
+    g = np.zeros(3)
+    lib().oracle_gauss3(seed, step, bead, stream, _dp(g))
+    return g
+
+
+def natural_spline(y, h):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    y2 = np.zeros_like(y)
+    lib().oracle_natural_spline(_dp(y), len(y), h, _dp(y2))
+    return y2
+
+
+def fla_min_energy(y, y2tab, tmin=1.30900, tmax=2.87979):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    y2 = np.ascontiguousarray(y2tab, dtype=np.float64)
+    return lib().oracle_fla_min_energy(_dp(y), _dp(y2), tmin, tmax)
+
+
+def fld_min_energy(t7):
+    t = np.ascontiguousarray(t7, dtype=np.float64)
+    return lib().oracle_fld_min_energy(_dp(t))
+
+
+def baoab(system, integrator, x, v, step0, nsteps):
+    sd, keep = system.descriptor()
+    it = integrator.descriptor()
+    x = np.array(x, dtype=np.float64, order="C")
+    v = np.array(v, dtype=np.float64, order="C")
+    lib().oracle_baoab(C.byref(sd), C.byref(it), _dp(x), _dp(v), step0, nsteps)
+    del keep
+    return x, v
+
+
+def kinetic(system, v):
+    sd, keep = system.descriptor()
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().oracle_kinetic(C.byref(sd), _dp(v))

@@ -1,0 +1,173 @@
+"""Host logic that feeds the device step: unit conversions, generator
+constants, exclusion sets, interaction-group semantics and config builders."""
+import math
+
+import numpy as np
+import pytest
+
+from opencafemol_amd import abi, aicg2, configs
+from opencafemol_amd.aicg2_tables import FLA_Y, FLA_Y2, FLD
+from opencafemol_amd.model import ForceSpec, SystemSpec
+
+
+def test_cafetime():
+    # src/util/Constants.hpp:23: sqrt(1/4.184) * 0.1 ps
+    assert configs.CAFETIME == pytest.approx(0.0488882129, rel=1e-9)
+
+
+def test_debye_huckel_constants_known_answers():
+    # water at 300 K and 0.1 M: eps_k from the reference polynomial, and the
+    # Debye length against the textbook 0.304 nm / sqrt(I[M]) (298 K,
+    # eps_r 78.5) rescaled by sqrt(eps_k T / (78.5 * 298))
+    epsk = configs.dielectric_water(300.0, 0.1)
+    assert epsk == pytest.approx(75.85, abs=0.01)
+    K, lam, corr, rc = configs.debye_huckel_constants(0.1, 300.0, 5.5)
+    textbook = 0.304 / math.sqrt(0.1) * math.sqrt(epsk * 300.0 / (78.5 * 298.0))
+    assert lam == pytest.approx(0.9486, abs=2e-4)
+    assert lam == pytest.approx(textbook, rel=3e-3)
+    # Coulomb constant: 138.935 kJ nm / mol e^2 divided by eps_k
+    assert K * epsk == pytest.approx(138.935458, rel=1e-6)
+    assert rc == pytest.approx(5.5 * lam)
+    assert corr == pytest.approx(math.exp(-5.5) / rc)
+    with pytest.raises(ValueError):
+        configs.debye_huckel_constants(0.0, 300.0, 5.5)
+
+
+def test_ionic_strength_for_unit_debye_length():
+    I = configs.ionic_strength_for_length(1.0, 300.0)
+    assert I == pytest.approx(0.0902, abs=2e-4)
+    assert configs.debye_huckel_constants(I, 300.0, 1.0)[1] == pytest.approx(1.0, rel=1e-9)
+
+
+def test_min_energies_match_oracle(oracle):
+    for aa in FLA_Y:
+        assert aicg2.fla_min_energy(FLA_Y[aa], FLA_Y2[aa]) == oracle.fla_min_energy(FLA_Y[aa], FLA_Y2[aa])
+    for key in FLD:
+        assert aicg2.fld_min_energy(FLD[key]) == oracle.fld_min_energy(FLD[key])
+
+
+@pytest.mark.parametrize("first,second,expect", [
+    ("ALA", "GLY", ("R1", "GLY")), ("GLY", "GLY", ("R1", "GLY")), ("GLY", "PRO", ("GLY", "PRO")),
+    ("ALA", "PRO", ("R2", "PRO")), ("VAL", "ALA", ("VAL", "R3")), ("GLY", "ALA", ("GLY", "R3")),
+])
+def test_fld_class_selection(first, second, expect):
+    # ReadTOMLForceFieldGenerator.cpp:636-766
+    assert aicg2.fld_class(first, second) == expect
+
+
+def test_group_ids_follow_system_generator():
+    s, it, x = configs.aicg2_chain(30, seed=1)
+    fla = [f for f in s.forces if f.kind == abi.FLEXIBLE_LOCAL_ANGLE]
+    fld = [f for f in s.forces if f.kind == abi.FLEXIBLE_LOCAL_DIHEDRAL]
+    assert len({f.group for f in fla}) == 1 and len({f.group for f in fld}) == 1
+    assert len(fla) > 1 and len(fld) > 1
+    assert sorted(s.group_ids.values()) == list(range(len(s.group_ids)))
+
+
+def test_chain_exclusions_match_graph_search():
+    ex = configs.chain_exclusions([(0, 10), (10, 5)], 3)
+    got = {tuple(p) for p in ex}
+    want = {(i, j) for s, n in [(0, 10), (10, 5)] for i in range(s, s + n) for j in range(i + 1, min(i + 4, s + n))}
+    assert got == want
+
+
+def _interaction_groups(params_present, group_of, ignore_pairs):
+    """InteractionGroup.hpp:17-101: set pairs built from participation,
+    group names and ignored group pairs."""
+    n = len(params_present)
+    if not ignore_pairs:
+        if all(params_present):
+            return None
+        part = {i for i in range(n) if params_present[i]}
+        return [(part, part)]
+    names = sorted({a for p in ignore_pairs for a in p})
+    groups = {nm: set() for nm in names}
+    others = set()
+    for i in range(n):
+        if params_present[i]:
+            if group_of[i] in groups:
+                groups[group_of[i]].add(i)
+            else:
+                others.add(i)
+    out = [(others, others)]
+    for a_i, a in enumerate(names):
+        out.append((groups[a], others))
+        for b in names[a_i:]:
+            if (a, b) not in ignore_pairs:
+                out.append((groups[a], groups[b]))
+    return out
+
+
+def _pairs_from_sets(groups, n):
+    """OpenMM semantics: a pair interacts if it lies in set1 x set2 of some group."""
+    if groups is None:
+        return {(i, j) for i in range(n) for j in range(i + 1, n)}
+    out = set()
+    for s1, s2 in groups:
+        for a in s1:
+            for b in s2:
+                if a != b:
+                    out.add((min(a, b), max(a, b)))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_category_allow_matrix_equals_interaction_groups(seed):
+    # the device descriptor encodes interaction groups as per-bead categories
+    # plus a symmetric allow matrix; check it reproduces the set semantics
+    rng = np.random.default_rng(seed)
+    n = 24
+    present = rng.random(n) < 0.8
+    names = ["A", "B", "C"]
+    group_of = [names[k] if k < 3 else None for k in rng.integers(0, 5, n)]
+    ignore = {("A", "A"), ("A", "C")} if seed % 2 else set()
+    sets = _interaction_groups(list(present), group_of, ignore)
+    want = _pairs_from_sets(sets, n)
+    # category encoding: 0 = others, 1.. = named groups that appear in ignores
+    related = sorted({a for p in ignore for a in p})
+    cat = np.full(n, -1)
+    for i in range(n):
+        if present[i]:
+            cat[i] = related.index(group_of[i]) + 1 if group_of[i] in related else 0
+    nc = len(related) + 1
+    allow = np.ones((nc, nc), dtype=np.uint8)
+    for a, b in ignore:
+        ia, ib = related.index(a) + 1, related.index(b) + 1
+        allow[ia, ib] = allow[ib, ia] = 0
+    got = {(i, j) for i in range(n) for j in range(i + 1, n)
+           if cat[i] >= 0 and cat[j] >= 0 and allow[cat[i], cat[j]]}
+    assert got == want
+
+
+def test_go_chain_config_properties():
+    s, it, x = configs.go_chain(62, seed=0)
+    b = np.linalg.norm(np.diff(x, axis=0), axis=1)
+    assert np.allclose(b, 0.38)
+    d = np.linalg.norm(x[:, None] - x[None], axis=2)
+    iu = np.triu_indices(62, k=2)
+    assert d[iu].min() >= 0.4 - 1e-12
+    go = [f for f in s.forces if f.kind == abi.GO_CONTACT][0]
+    assert np.all(go.indices[:, 1] - go.indices[:, 0] >= 4)
+    assert np.all(go.params[:, 1] < 0.65)
+    exv = [f for f in s.forces if f.kind == abi.EXCLUDED_VOLUME][0]
+    assert exv.cutoff == pytest.approx(0.8)
+    dh = [f for f in s.forces if f.kind == abi.DEBYE_HUCKEL][0]
+    assert (dh.category >= 0).sum() == round(0.2 * 62)
+    assert dh.cutoff == pytest.approx(3.0, rel=1e-6)
+
+
+def test_hps_slab_config_properties():
+    s, it, x = configs.hps_slab(n_chains=10, chain_len=40, box=(8.0, 8.0, 40.0), slab_z=6.0, seed=2)
+    assert x.shape == (400, 3)
+    assert x[:, 2].min() > 15.0 and x[:, 2].max() < 25.0
+    q = [f for f in s.forces if f.kind == abi.DEBYE_HUCKEL][0].particle_params.ravel()
+    assert (q > 0).sum() == 60 and (q < 0).sum() == 60
+    lja = [f for f in s.forces if f.kind == abi.LJ_ATTRACTIVE][0]
+    assert lja.switch_distance == pytest.approx(0.9 * lja.cutoff)
+
+
+def test_temperature_ladder():
+    t = configs.temperature_ladder(256)
+    assert t[0] == pytest.approx(280.0) and t[-1] == pytest.approx(400.0)
+    r = np.array(t[1:]) / np.array(t[:-1])
+    assert np.allclose(r, r[0])
