@@ -186,6 +186,14 @@ int ocm_step(ocm_context* ctx, int64_t nsteps);
 int ocm_get_state(ocm_context* ctx, int32_t what, uint32_t groups,
                   ocm_state* out);
 
+/* LocalEnergyMinimizer::minimize(context, tolerance, max_iterations),
+ * src/Simulator.cpp:42-46.  Capped steepest descent on the device: each
+ * iteration moves every mobile bead along its force by min(alpha |F|, 0.01 nm);
+ * stops when the RMS force component of every system is <= tolerance
+ * (kJ/mol/nm) or after max_iterations (0 = 100000).  Not OpenMM's L-BFGS: the
+ * minimum reached is a local one, not the same point.                       */
+int ocm_minimize(ocm_context* ctx, double tolerance, int32_t max_iterations);
+
 /* Neighbor pairs (bead index pairs i<j, r < cutoff) that the device list of
  * nonbonded force `force_index` interacts through, at the current positions.
  * Writes up to `cap` pairs to out_pairs (2 ints each); *count gets the total. */

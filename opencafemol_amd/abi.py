@@ -45,7 +45,7 @@ EXPORTED = [
     "ocm_get_neighbor_pairs", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
     "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
     "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
-    "ocm_version",
+    "ocm_version", "ocm_minimize",
 ]
 
 
@@ -128,6 +128,7 @@ def load_library(path=LIB_PATH):
         "ocm_batch_size": (C.c_int32, [C.c_void_p]),
         "ocm_last_error": (C.c_char_p, []),
         "ocm_version": (C.c_char_p, []),
+        "ocm_minimize": (C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

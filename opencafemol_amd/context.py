@@ -89,6 +89,9 @@ class Batch:
         out["step"] = st.step
         return out
 
+    def minimize(self, tolerance=10.0, max_iterations=0):
+        check(self.lib.ocm_minimize(self.h, float(tolerance), int(max_iterations)))
+
     def set_timing(self, on):
         check(self.lib.ocm_set_timing(self.h, int(bool(on))))
 

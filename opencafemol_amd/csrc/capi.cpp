@@ -141,6 +141,13 @@ int ocm_step(ocm_context* ctx, int64_t nsteps) {
     });
 }
 
+int ocm_minimize(ocm_context* ctx, double tolerance, int32_t max_iterations) {
+    return guarded([&] {
+        eng(ctx).minimize(tolerance, max_iterations);
+        return OCM_OK;
+    });
+}
+
 int ocm_batch_get_state(ocm_context* ctx, int32_t s, int32_t what, uint32_t groups, ocm_state* out) {
     return guarded([&] {
         auto& e = eng(ctx);

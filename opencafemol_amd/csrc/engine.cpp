@@ -780,6 +780,54 @@ int Engine::step(long long n) {
     return status;
 }
 
+int Engine::minimize(double tol, int max_iter) {
+    if (!(tol >= 0.0)) invalid("tolerance must be >= 0");
+    if (max_iter < 0) invalid("max_iterations must be >= 0");
+    if (max_iter == 0) max_iter = 100000;
+    ensure_forces();
+    const double alpha = 5e-6, dmax = 0.01;  // nm per kJ/mol/nm, nm
+    std::vector<long long> f(3 * static_cast<size_t>(d_.nb));
+    int it = 0;
+    while (true) {
+        sync();
+        check_hip(hipMemcpy(f.data(), d_.f.fx, f.size() * sizeof(long long), hipMemcpyDeviceToHost), "forces");
+        bool done = true;
+        for (int s = 0; s < d_.nsys && done; ++s) {
+            const HostSystem& H = sys_[s];
+            double ss = 0.0;
+            for (int i = 0; i < H.n; ++i) {
+                const size_t b = bead_begin_[s] + i;
+                if (H.masses[i] == 0.0) continue;
+                for (int c = 0; c < 3; ++c) {
+                    const double v = f[c * static_cast<size_t>(d_.nb) + b] * kFixInv;
+                    ss += v * v;
+                }
+            }
+            if (std::sqrt(ss / (3.0 * H.n)) > tol) done = false;
+        }
+        if (done || it >= max_iter) break;
+        const int burst = std::min(50, max_iter - it);
+        for (int k = 0; k < burst; ++k) {
+            launch_relax(d_, stream_, alpha, dmax);
+            enqueue_forces(d_.f, true, 0xFFFFFFFFu);
+        }
+        it += burst;
+        sync();
+        int flags = 0;
+        check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
+        if (flags & kFlagListOverflow) {
+            int total = 0;
+            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
+            grow_list(total);
+            check_hip(hipMemset(d_.flags, 0, sizeof(int)), "memset");
+            launch_zero_forces(d_, d_.f, stream_);
+            enqueue_forces(d_.f, true, 0xFFFFFFFFu);
+        }
+    }
+    check_hip(hipMemset(d_.err, 0, sizeof(int) * d_.nsys), "memset");
+    return it;
+}
+
 // ------------------------------------------------------------ state I/O
 void Engine::set_positions(int s, const double* xyz) {
     const HostSystem& H = sys_.at(s);

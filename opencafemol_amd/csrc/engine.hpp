@@ -71,6 +71,9 @@ public:
     // Advances every system by n steps.  Returns OCM_OK or OCM_ERR_NUMERIC
     // (some system produced NaN/overflow; see system_status).
     int step(long long n);
+    // Capped steepest descent until the RMS force component of every system
+    // is <= tol or max_iter iterations ran.  Returns iterations done.
+    int minimize(double tol, int max_iter);
     void get_state(int s, int what, uint32_t groups, ocm_state* out);
     long long neighbor_pairs(int s, int force_index, int32_t* out, long long cap);
     int system_status(int s) const { return sys_status_.at(s); }

@@ -68,6 +68,7 @@ void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& 
 void launch_zero_energy(const DevPtrs& d, hipStream_t st);
 void launch_kinetic(const DevPtrs& d, hipStream_t st);
 void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members);
+void launch_relax(const DevPtrs& d, hipStream_t st, double alpha, double dmax);
 void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed);
 
 }  // namespace ocm

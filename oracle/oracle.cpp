@@ -572,3 +572,153 @@ int32_t oracle_abi_layout(int64_t* out, int32_t cap) {
     return n;
 }
 }
+
+// ----------------------------------------------------------------------------
+// Cell-list CPU step (CPU baseline): the same BAOAB and the same per-pair
+// formulas as above, with nonbonded pairs found through a cell list and forces
+// gathered per bead (full neighbor search, OpenMP over beads).  Used only to
+// time a CPU implementation of the step beside the GPU; parity is checked with
+// oracle_eval / oracle_baoab.
+namespace {
+
+struct CellPairs {
+    const ocm_force_desc* f;
+    std::vector<std::vector<int>> excl;   // sorted partners per bead
+};
+
+void pair_forces_cells(const ocm_system_desc* s, const CellPairs& cp, const double* x, double* F,
+                       int nthreads) {
+    const ocm_force_desc& f = *cp.f;
+    const int n = s->n_particles;
+    const Box box = make_box(s);
+    const bool pbc = f.use_periodic != 0 && box.periodic;
+    const double rc = f.cutoff, rc2 = rc * rc;
+    double lo[3], ext[3];
+    int nc[3];
+    for (int c = 0; c < 3; ++c) {
+        if (pbc) {
+            lo[c] = 0.0;
+            ext[c] = box.L[c];
+        } else {
+            double mn = 1e300, mx = -1e300;
+            for (int i = 0; i < n; ++i) { mn = std::min(mn, x[3 * i + c]); mx = std::max(mx, x[3 * i + c]); }
+            lo[c] = mn;
+            ext[c] = std::max(mx - mn, 1e-9) * (1.0 + 1e-12);
+        }
+        nc[c] = std::max(1, static_cast<int>(ext[c] / rc));
+        if (pbc && nc[c] < 3) nc[c] = 1;
+    }
+    const int ncell = nc[0] * nc[1] * nc[2];
+    std::vector<int> cell_of(n, -1), head(ncell, -1), next(n, -1);
+    for (int i = n - 1; i >= 0; --i) {
+        if (!participates(f, i)) continue;
+        int ci[3];
+        for (int c = 0; c < 3; ++c) {
+            double u = (x[3 * i + c] - lo[c]) / ext[c];
+            if (pbc) u -= std::floor(u);
+            int k = static_cast<int>(u * nc[c]);
+            ci[c] = std::max(0, std::min(nc[c] - 1, k));
+        }
+        const int cell = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
+        cell_of[i] = cell;
+        next[i] = head[cell];
+        head[cell] = i;
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        if (cell_of[i] < 0) continue;
+        const int cell = cell_of[i];
+        const int cx = cell % nc[0], cy = (cell / nc[0]) % nc[1], cz = cell / (nc[0] * nc[1]);
+        double acc[3] = {0, 0, 0};
+        for (int dz = -1; dz <= 1; ++dz) {
+            if (nc[2] == 1 && dz != 0) continue;
+            int z = cz + dz;
+            if (pbc) z = (z + nc[2]) % nc[2]; else if (z < 0 || z >= nc[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                if (nc[1] == 1 && dy != 0) continue;
+                int y = cy + dy;
+                if (pbc) y = (y + nc[1]) % nc[1]; else if (y < 0 || y >= nc[1]) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (nc[0] == 1 && dx != 0) continue;
+                    int xx = cx + dx;
+                    if (pbc) xx = (xx + nc[0]) % nc[0]; else if (xx < 0 || xx >= nc[0]) continue;
+                    for (int j = head[(z * nc[1] + y) * nc[0] + xx]; j >= 0; j = next[j]) {
+                        if (j == i || !allowed(f, i, j)) continue;
+                        const auto& ex = cp.excl[i];
+                        if (!ex.empty() && std::binary_search(ex.begin(), ex.end(), j)) continue;
+                        const V3 d = disp(x, i, j, box, pbc);
+                        const double r2 = r2_of(d);
+                        if (!(r2 < rc2)) continue;
+                        const double r = std::sqrt(r2);
+                        double e, dedr;
+                        pair_energy(f, i, j, r, &e, &dedr);
+                        acc[0] += d.x * dedr / r;
+                        acc[1] += d.y * dedr / r;
+                        acc[2] += d.z * dedr / r;
+                    }
+                }
+            }
+        }
+        F[3 * i] += acc[0]; F[3 * i + 1] += acc[1]; F[3 * i + 2] += acc[2];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// BAOAB like oracle_baoab with cell-list nonbonded forces on `nthreads` threads.
+int oracle_baoab_cells(const ocm_system_desc* s, const ocm_integrator_desc* it, double* x, double* v,
+                       int64_t step0, int64_t nsteps, int32_t nthreads) {
+    const int n = s->n_particles;
+    const Box box = make_box(s);
+    std::vector<CellPairs> cps;
+    for (int fi = 0; fi < s->n_forces; ++fi) {
+        const ocm_force_desc& f = s->forces[fi];
+        if (is_bonded(f.kind)) continue;
+        CellPairs cp{&f, std::vector<std::vector<int>>(n)};
+        for (int64_t e = 0; e < f.n_exclusions; ++e) {
+            const int a = f.exclusions[2 * e], b = f.exclusions[2 * e + 1];
+            cp.excl[a].push_back(b);
+            cp.excl[b].push_back(a);
+        }
+        for (auto& v2 : cp.excl) std::sort(v2.begin(), v2.end());
+        cps.push_back(std::move(cp));
+    }
+    auto forces = [&](std::vector<double>& F) {
+        std::fill(F.begin(), F.end(), 0.0);
+        for (int fi = 0; fi < s->n_forces; ++fi)
+            if (is_bonded(s->forces[fi].kind)) eval_bonded(s->forces[fi], x, box, F.data());
+        for (const auto& cp : cps) pair_forces_cells(s, cp, x, F.data(), nthreads);
+    };
+    const double kB = 1.380649e-23 * 6.02214076e23 / 1000.0;
+    const double kT = kB * it->temperature, h = 0.5 * it->dt;
+    const double c1 = std::exp(-it->friction * it->dt), c2 = 1.0 - c1 * c1;
+    std::vector<double> F(3 * n), im(n);
+    for (int i = 0; i < n; ++i) im[i] = s->masses[i] > 0 ? 1.0 / s->masses[i] : 0.0;
+    forces(F);
+    for (int64_t st = 0; st < nsteps; ++st) {
+        const uint64_t sidx = static_cast<uint64_t>(step0 + st);
+#pragma omp parallel for num_threads(nthreads)
+        for (int i = 0; i < n; ++i) {
+            if (im[i] == 0.0) continue;
+            double g[3];
+            gauss3(it->seed, sidx, static_cast<uint32_t>(i), 0u, g);
+            const double sig = std::sqrt(kT * c2 * im[i]);
+            for (int c = 0; c < 3; ++c) {
+                double vv = v[3 * i + c] + h * F[3 * i + c] * im[i];
+                double xx = x[3 * i + c] + h * vv;
+                vv = c1 * vv + sig * g[c];
+                xx = xx + h * vv;
+                v[3 * i + c] = vv;
+                x[3 * i + c] = xx;
+            }
+        }
+        forces(F);
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) v[3 * i + c] += h * F[3 * i + c] * im[i];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -118,3 +118,17 @@ def kinetic(system, v):
     sd, keep = system.descriptor()
     v = np.ascontiguousarray(v, dtype=np.float64)
     return lib().oracle_kinetic(C.byref(sd), _dp(v))
+
+
+def baoab_cells(system, integrator, x, v, step0, nsteps, nthreads=1):
+    L = lib()
+    L.oracle_baoab_cells.argtypes = [C.POINTER(abi.SystemDesc), C.POINTER(abi.IntegratorDesc), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.c_int64, C.c_int64, C.c_int32]
+    L.oracle_baoab_cells.restype = C.c_int
+    sd, keep = system.descriptor()
+    it = integrator.descriptor()
+    x = np.array(x, dtype=np.float64, order="C")
+    v = np.array(v, dtype=np.float64, order="C")
+    L.oracle_baoab_cells(C.byref(sd), C.byref(it), _dp(x), _dp(v), step0, nsteps, nthreads)
+    del keep
+    return x, v
