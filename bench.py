@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--workload", default="hps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--minimize-iters", type=int, default=3000)
+    ap.add_argument("--no-graph", action="store_true", help="eager launches (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent systems shard "
@@ -86,7 +87,7 @@ def main():
 
     systems, integs, xs, cfg = build_workload(args.workload)
     log(f"workload {cfg['workload']}: {sum(s.n for s in systems)} beads in {len(systems)} system(s)")
-    b = Batch(systems, integs)
+    b = Batch(systems, integs, use_graph=0 if args.no_graph else 1)
     for k, x in enumerate(xs):
         b.set_positions(k, x)
     t = time.perf_counter()

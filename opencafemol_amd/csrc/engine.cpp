@@ -531,6 +531,16 @@ void Engine::pack() {
     d_.member_slot = dalloc<int>(nm);
     d_.member_cell = dalloc<int>(nm);
     d_.member_rank = dalloc<int>(nm);
+    {
+        std::vector<int> cnt(nb, 0), start(nb, 0), lst(nm);
+        for (int m = 0; m < nm; ++m) ++cnt[member_bead[m]];
+        for (int b = 1; b < nb; ++b) start[b] = start[b - 1] + cnt[b - 1];
+        std::vector<int> fill(start);
+        for (int m = 0; m < nm; ++m) lst[fill[member_bead[m]]++] = m;
+        d_.bead_mem_start = dalloc<int>(nb); upload(d_.bead_mem_start, start);
+        d_.bead_mem_count = dalloc<int>(nb); upload(d_.bead_mem_count, cnt);
+        d_.bead_mem = dalloc<int>(nm); upload(d_.bead_mem, lst);
+    }
     d_.excl_start = dalloc<int>(nm); upload(d_.excl_start, excl_start);
     d_.excl_count = dalloc<int>(nm); upload(d_.excl_count, excl_count);
     d_.excl = dalloc<int>(excl.size()); upload(d_.excl, excl);
@@ -626,8 +636,12 @@ void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
 
 void Engine::enqueue_chunk(int k, int interval) {
     for (int s = 0; s < k; ++s) {
-        timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0); });
-        enqueue_forces(d_.f, s % interval == 0, 0xFFFFFFFFu);
+        const bool rebuild = s % interval == 0;
+        // the integrator refreshes the pair records itself on non-rebuild steps
+        timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0, rebuild ? 0 : 1, 1); });
+        if (d_.nm > 0 && rebuild) timed(3, [&] { launch_rebuild(d_, stream_); });
+        timed(1, [&] { launch_bonded(d_, stream_, false, d_.f, 0xFFFFFFFFu); });
+        timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu); });
     }
     timed(2, [&] { launch_half_kick(d_, stream_, k & 1); });
 }
@@ -734,10 +748,10 @@ int Engine::run_chunk(int k) {
                 it = graphs_.emplace(key, ge).first;
             }
             check_hip(hipGraphLaunch(it->second, stream_), "hipGraphLaunch");
-            rebuilds_ += (k + interval - 1) / interval;
         } else {
             enqueue_chunk(k, interval);
         }
+        if (d_.nm > 0) rebuilds_ += (k + interval - 1) / interval;
         sync();
         int flags = 0;
         std::vector<int> err(d_.nsys);

@@ -56,6 +56,43 @@ __global__ void k_fill_i64(long long* p, long long v, int n) {
     if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------- record refresh
+// Writes the pair record of every pair-group membership of bead b and, when
+// requested, flags the system if the bead moved more than skin/2 since the
+// last list build (the list is then rebuilt by the host's rollback).
+struct PackArgs {
+    int enabled;
+    int check;
+    const int* bead_mem_start;
+    const int* bead_mem_count;
+    const int* bead_mem;
+    const DevPairGroup* groups;
+    const int* member_group;
+    const int* member_type;
+    const int* member_slot;
+    const double* refx;
+    const double* refy;
+    const double* refz;
+    PairRec* rec;
+};
+
+__device__ inline void pack_bead(const PackArgs& pk, int b, double x, double y, double z, int* err) {
+    const int k0 = pk.bead_mem_start[b], k1 = k0 + pk.bead_mem_count[b];
+    for (int k = k0; k < k1; ++k) {
+        const int m = pk.bead_mem[k];
+        PairRec r;
+        r.x = x; r.y = y; r.z = z;
+        r.type = pk.member_type[m];
+        r.bead = b;
+        pk.rec[pk.member_slot[m]] = r;
+        if (pk.check) {
+            const DevPairGroup& G = pk.groups[pk.member_group[m]];
+            const double dx = x - pk.refx[m], dy = y - pk.refy[m], dz = z - pk.refz[m];
+            if (!(dist2(dx, dy, dz) <= G.half_skin2)) atomicOr(&err[G.sys], kErrStale);
+        }
+    }
+}
+
 // ------------------------------------------------------------- integrator
 // pending != 0: the closing half kick of the previous step is still owed, so
 // the opening kick uses a full dt.  gstep[pin] is this step's global step;
@@ -66,7 +103,7 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
                             double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
                             long long* __restrict__ fx, long long* __restrict__ fy,
                             long long* __restrict__ fz, long long* gstep, int pin, int pout,
-                            int pending, int* __restrict__ err) {
+                            int pending, int* __restrict__ err, PackArgs pk) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const long long step = gstep[pin];
     if (b == 0) gstep[pout] = step + 1;
@@ -76,7 +113,10 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     const double im = inv_m[b];
     const double f0 = fx[b] * kFixInv, f1 = fy[b] * kFixInv, f2 = fz[b] * kFixInv;
     fx[b] = 0; fy[b] = 0; fz[b] = 0;
-    if (im == 0.0) return;
+    if (im == 0.0) {
+        if (pk.enabled) pack_bead(pk, b, px[b], py[b], pz[b], err);
+        return;
+    }
     const double kick = pending ? 2.0 * S.half_dt : S.half_dt;
     double v0 = vx[b] + kick * f0 * im;
     double v1 = vy[b] + kick * f1 * im;
@@ -96,6 +136,7 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) atomicOr(&err[s], kErrNumeric);
     px[b] = x0; py[b] = x1; pz[b] = x2;
     vx[b] = v0; vy[b] = v1; vz[b] = v2;
+    if (pk.enabled) pack_bead(pk, b, x0, x1, x2, err);
 }
 
 // Closing half kick; also resets both global-step slots to the value reached.
@@ -199,16 +240,28 @@ __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, 
     return false;
 }
 
-// Walks the cell stencil of slot i and calls visit(slot_j) for every listed
-// partner (r^2 < rl^2 at the build positions, allowed, not excluded).
-template <typename Visit>
-__device__ inline void nl_walk(int i, const DevPairGroup& G, const int* __restrict__ slot_member,
-                               const int* __restrict__ member_cell, const int* __restrict__ member_cat,
-                               const int* __restrict__ excl_start, const int* __restrict__ excl_count,
-                               const int* __restrict__ excl, const int* __restrict__ cell_start,
-                               const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
-                               Visit visit) {
+// One wavefront per row i.  The 64 lanes sweep the slots of each stencil
+// cell together; candidates that pass (r^2 < rl^2 at the build positions,
+// allowed categories, not excluded) are counted with a ballot, and in the
+// FILL pass written in slot order at base + popcount of the lower lanes.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_nl_rows(
+    int ns, const DevPairGroup* __restrict__ groups, const int* __restrict__ member_group,
+    const int* __restrict__ slot_member, const int* __restrict__ member_cell, const int* __restrict__ member_cat,
+    const int* __restrict__ excl_start, const int* __restrict__ excl_count, const int* __restrict__ excl,
+    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
+    const int* __restrict__ row_start, int* __restrict__ row_count, const int* __restrict__ total,
+    long long capacity, int* __restrict__ col, int* __restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= ns) return;  // uniform per wavefront
+    if (FILL && static_cast<long long>(*total) > capacity) {
+        if (lane == 0) row_count[i] = 0;  // the pair kernel then reads nothing
+        if (i == 0 && lane == 0) atomicOr(flags, kFlagListOverflow);
+        return;
+    }
     const int m = slot_member[i];
+    const DevPairGroup& G = groups[member_group[m]];
     const int cell = member_cell[m] - G.cell_begin;
     const int cx = cell % G.ncell[0];
     const int cy = (cell / G.ncell[0]) % G.ncell[1];
@@ -216,6 +269,9 @@ __device__ inline void nl_walk(int i, const DevPairGroup& G, const int* __restri
     const uint32_t amask = G.allow[member_cat[m]];
     const int eb = excl_start[m], ec = excl_count[m];
     const PairRec ri = rec[i];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int base = FILL ? row_start[i] : 0;
+    int cnt = 0;
     for (int dz = -G.reach[2]; dz <= G.reach[2]; ++dz) {
         int z = cz + dz;
         if (z < 0) z += G.ncell[2]; else if (z >= G.ncell[2]) z -= G.ncell[2];
@@ -227,63 +283,30 @@ __device__ inline void nl_walk(int i, const DevPairGroup& G, const int* __restri
                 if (x < 0) x += G.ncell[0]; else if (x >= G.ncell[0]) x -= G.ncell[0];
                 const int c = G.cell_begin + (z * G.ncell[1] + y) * G.ncell[0] + x;
                 const int s0 = cell_start[c], s1 = s0 + cell_count[c];
-                for (int s = s0; s < s1; ++s) {
-                    if (s == i) continue;
-                    const int mj = slot_member[s];
-                    if (!((amask >> member_cat[mj]) & 1u)) continue;
-                    const PairRec rj = rec[s];
-                    double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
-                    if (G.periodic) {
-                        ddx = min_image(ddx, G.box[0], G.inv_box[0]);
-                        ddy = min_image(ddy, G.box[1], G.inv_box[1]);
-                        ddz = min_image(ddz, G.box[2], G.inv_box[2]);
+                for (int sb = s0; sb < s1; sb += 64) {
+                    const int s = sb + lane;
+                    bool pass = false;
+                    if (s < s1 && s != i) {
+                        const int mj = slot_member[s];
+                        if ((amask >> member_cat[mj]) & 1u) {
+                            const PairRec rj = rec[s];
+                            double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
+                            if (G.periodic) {
+                                ddx = min_image(ddx, G.box[0], G.inv_box[0]);
+                                ddy = min_image(ddy, G.box[1], G.inv_box[1]);
+                                ddz = min_image(ddz, G.box[2], G.inv_box[2]);
+                            }
+                            pass = dist2(ddx, ddy, ddz) < G.rl2 && !(ec && excluded(excl, eb, ec, mj));
+                        }
                     }
-                    if (!(dist2(ddx, ddy, ddz) < G.rl2)) continue;
-                    if (ec && excluded(excl, eb, ec, mj)) continue;
-                    visit(s);
+                    const unsigned long long mask = __ballot(pass);
+                    if (FILL && pass) col[base + cnt + __popcll(mask & below)] = s;
+                    cnt += __popcll(mask);
                 }
             }
         }
     }
-}
-
-__global__ void k_nl_count(int ns, const DevPairGroup* __restrict__ groups,
-                           const int* __restrict__ member_group, const int* __restrict__ slot_member,
-                           const int* __restrict__ member_cell, const int* __restrict__ member_cat,
-                           const int* __restrict__ excl_start, const int* __restrict__ excl_count,
-                           const int* __restrict__ excl, const int* __restrict__ cell_start,
-                           const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
-                           int* __restrict__ row_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    const DevPairGroup& G = groups[member_group[slot_member[i]]];
-    int cnt = 0;
-    nl_walk(i, G, slot_member, member_cell, member_cat, excl_start, excl_count, excl, cell_start,
-            cell_count, rec, [&](int) { ++cnt; });
-    row_count[i] = cnt;
-}
-
-__global__ void k_nl_fill(int ns, const DevPairGroup* __restrict__ groups,
-                          const int* __restrict__ member_group, const int* __restrict__ slot_member,
-                          const int* __restrict__ member_cell, const int* __restrict__ member_cat,
-                          const int* __restrict__ excl_start, const int* __restrict__ excl_count,
-                          const int* __restrict__ excl, const int* __restrict__ cell_start,
-                          const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
-                          const int* __restrict__ row_start, int* __restrict__ row_count,
-                          const int* __restrict__ total, long long capacity, int* __restrict__ col,
-                          int* __restrict__ flags) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    if (static_cast<long long>(*total) > capacity) {
-        row_count[i] = 0;  // the pair kernel then reads nothing
-        if (i == 0) atomicOr(flags, kFlagListOverflow);
-        return;
-    }
-    const DevPairGroup& G = groups[member_group[slot_member[i]]];
-    const int base = row_start[i];
-    int k = 0;
-    nl_walk(i, G, slot_member, member_cell, member_cat, excl_start, excl_count, excl, cell_start,
-            cell_count, rec, [&](int s) { col[base + k] = s; ++k; });
+    if (!FILL && lane == 0) row_count[i] = cnt;
 }
 
 // -------------------------------------------------------------------- scan
@@ -837,10 +860,12 @@ void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st) {
     k_fill_i64<<<nblk(d.nb, 256), 256, 0, st>>>(f.fz, 0, d.nb);
 }
 
-void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending) {
+void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check) {
+    PackArgs pk{pack && d.nm > 0, check, d.bead_mem_start, d.bead_mem_count, d.bead_mem, d.groups,
+                d.member_group, d.member_type, d.member_slot, d.refx, d.refy, d.refz, d.rec};
     k_integrate<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
                                                  d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
-                                                 pin, pout, pending, d.err);
+                                                 pin, pout, pending, d.err, pk);
 }
 
 void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur) {
@@ -871,14 +896,16 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
                                              d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
                                              d.rec, d.refx, d.refy, d.refz);
-    k_nl_count<<<nblk(d.nm, 128), 128, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
-                                                 d.member_cat, d.excl_start, d.excl_count, d.excl,
-                                                 d.cell_start, d.cell_count, d.rec, d.row_count);
+    const int rows_blocks = nblk(d.nm, 4);
+    k_nl_rows<false><<<rows_blocks, 256, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
+                                                   d.member_cat, d.excl_start, d.excl_count, d.excl, d.cell_start,
+                                                   d.cell_count, d.rec, d.row_start, d.row_count, d.scan_total + 1,
+                                                   d.list_capacity, d.col, d.flags);
     launch_scan(d, st, d.row_count, d.row_start, d.nm, d.scan_total + 1);
-    k_nl_fill<<<nblk(d.nm, 128), 128, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
-                                               d.member_cat, d.excl_start, d.excl_count, d.excl, d.cell_start,
-                                               d.cell_count, d.rec, d.row_start, d.row_count, d.scan_total + 1,
-                                               d.list_capacity, d.col, d.flags);
+    k_nl_rows<true><<<rows_blocks, 256, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
+                                                  d.member_cat, d.excl_start, d.excl_count, d.excl, d.cell_start,
+                                                  d.cell_count, d.rec, d.row_start, d.row_count, d.scan_total + 1,
+                                                  d.list_capacity, d.col, d.flags);
 }
 
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {

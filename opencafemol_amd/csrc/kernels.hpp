@@ -35,6 +35,7 @@ struct DevPtrs {
     DevPairGroup* groups = nullptr;
     int *member_group = nullptr, *member_bead = nullptr, *member_type = nullptr, *member_cat = nullptr;
     int *member_slot = nullptr, *member_cell = nullptr, *member_rank = nullptr;
+    int *bead_mem_start = nullptr, *bead_mem_count = nullptr, *bead_mem = nullptr;  // bead -> members
     int *excl_start = nullptr, *excl_count = nullptr, *excl = nullptr;
     double *refx = nullptr, *refy = nullptr, *refz = nullptr;
     PairRec* rec = nullptr;
@@ -59,7 +60,8 @@ struct DevPtrs {
 constexpr int kPairBlock = 256;
 
 void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st);
-void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending);
+// pack != 0: also refresh the pair records (and check list validity when check != 0)
+void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check);
 void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur);
 void launch_pack(const DevPtrs& d, hipStream_t st);
 void launch_rebuild(const DevPtrs& d, hipStream_t st);
