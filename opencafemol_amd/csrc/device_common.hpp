@@ -57,7 +57,7 @@ struct DevSys {
 // Pair-record of one member bead in slot (cell-sorted) order.
 struct alignas(16) PairRec {
     double x, y, z;
-    int type;   // index into the group's type table
+    int type;   // bits 0-15: index into the group's type table; bits 16-20: category
     int bead;   // global bead index
 };
 

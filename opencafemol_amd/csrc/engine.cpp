@@ -326,6 +326,7 @@ void Engine::pack() {
     // ---- pair groups
     std::vector<DevPairGroup> groups;
     std::vector<int> member_group, member_bead, member_type, member_cat, excl_start, excl_count, excl;
+    std::vector<int> cell_group;
     std::vector<float> types_f;
     std::vector<double> types_d;
     int ncell_total = 0;
@@ -427,7 +428,7 @@ void Engine::pack() {
                 bead_to_member[i] = static_cast<int>(member_bead.size());
                 member_group.push_back(gid);
                 member_bead.push_back(bead_begin_[s] + i);
-                member_type.push_back(ty);
+                member_type.push_back(ty | (cat << 16));  // category rides in bits 16-20
                 member_cat.push_back(cat);
             }
             G.member_count = static_cast<int>(member_bead.size()) - G.member_begin;
@@ -437,11 +438,11 @@ void Engine::pack() {
             max_types = std::max(max_types, G.n_types);
             // exclusions in member space, symmetric CSR sorted by partner
             std::vector<std::vector<int>> adj(G.member_count);
-            for (const auto& e : f0.exclusions) {
+            for (const auto& e : f0.exclusions) {  // partners stored as global bead ids
                 const int a = bead_to_member[e.first], b = bead_to_member[e.second];
                 if (a < 0 || b < 0) continue;
-                adj[a - G.member_begin].push_back(b);
-                adj[b - G.member_begin].push_back(a);
+                adj[a - G.member_begin].push_back(bead_begin_[s] + e.second);
+                adj[b - G.member_begin].push_back(bead_begin_[s] + e.first);
             }
             for (int m = 0; m < G.member_count; ++m) {
                 auto& v = adj[m];
@@ -479,6 +480,7 @@ void Engine::pack() {
             }
             G.cell_begin = ncell_total;
             ncell_total += G.ncell[0] * G.ncell[1] * G.ncell[2];
+            cell_group.resize(ncell_total, gid);
             group_list_estimate_.push_back(est_neighbors);
             group_member_begin_.push_back(G.member_begin);
             group_member_count_.push_back(G.member_count);
@@ -549,6 +551,9 @@ void Engine::pack() {
     d_.slot_member = dalloc<int>(nm);
     d_.cell_count = dalloc<int>(d_.ncell);
     d_.cell_start = dalloc<int>(d_.ncell);
+    d_.cell_group = dalloc<int>(d_.ncell);
+    if (cell_group.empty()) cell_group.assign(d_.ncell, 0);
+    upload(d_.cell_group, cell_group);
     d_.row_count = dalloc<int>(nm);
     d_.row_start = dalloc<int>(nm);
     check_hip(hipMemset(d_.row_count, 0, sizeof(int) * std::max(nm, 1)), "memset");

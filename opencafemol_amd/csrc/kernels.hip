@@ -240,73 +240,116 @@ __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, 
     return false;
 }
 
-// One wavefront per row i.  The 64 lanes sweep the slots of each stencil
-// cell together; candidates that pass (r^2 < rl^2 at the build positions,
-// allowed categories, not excluded) are counted with a ballot, and in the
-// FILL pass written in slot order at base + popcount of the lower lanes.
+// LDS-staged cell tiles.  One workgroup per home cell: the records of the
+// stencil cells are staged into LDS in windows of kTileCap, and every row of
+// the home cell (one wavefront per row) sweeps the window with its 64 lanes.
+// A candidate passes when r^2 < rl^2 at the build positions, its category is
+// allowed and it is not excluded; passes are counted with a ballot and, in
+// the FILL pass, written in window order at base + popcount of lower lanes.
+constexpr int kTileCap = 1024;
+constexpr int kTileRows = 256;
+constexpr int kMaxStencil = 27;
+
 template <bool FILL>
-__global__ __launch_bounds__(256) void k_nl_rows(
-    int ns, const DevPairGroup* __restrict__ groups, const int* __restrict__ member_group,
-    const int* __restrict__ slot_member, const int* __restrict__ member_cell, const int* __restrict__ member_cat,
-    const int* __restrict__ excl_start, const int* __restrict__ excl_count, const int* __restrict__ excl,
+__global__ __launch_bounds__(256) void k_nl_tiles(
+    const DevPairGroup* __restrict__ groups, const int* __restrict__ cell_group,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
-    const int* __restrict__ row_start, int* __restrict__ row_count, const int* __restrict__ total,
-    long long capacity, int* __restrict__ col, int* __restrict__ flags) {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (i >= ns) return;  // uniform per wavefront
+    const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
+    const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
+    const int* __restrict__ total, long long capacity, int* __restrict__ col, int* __restrict__ flags) {
+    __shared__ PairRec cand[kTileCap];
+    __shared__ int cand_slot[kTileCap];
+    __shared__ int row_run[kTileRows];
+    __shared__ int st_start[kMaxStencil], st_pre[kMaxStencil + 1];
+    const int home = blockIdx.x;
+    const int hn = cell_count[home];
+    if (hn == 0) return;
+    const int h0 = cell_start[home];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
     if (FILL && static_cast<long long>(*total) > capacity) {
-        if (lane == 0) row_count[i] = 0;  // the pair kernel then reads nothing
-        if (i == 0 && lane == 0) atomicOr(flags, kFlagListOverflow);
+        for (int r = tid; r < hn; r += blockDim.x) row_count[h0 + r] = 0;  // the pair kernel reads nothing
+        if (tid == 0) atomicOr(flags, kFlagListOverflow);
         return;
     }
-    const int m = slot_member[i];
-    const DevPairGroup& G = groups[member_group[m]];
-    const int cell = member_cell[m] - G.cell_begin;
+    const DevPairGroup& G = groups[cell_group[home]];
+    const int cell = home - G.cell_begin;
     const int cx = cell % G.ncell[0];
     const int cy = (cell / G.ncell[0]) % G.ncell[1];
     const int cz = cell / (G.ncell[0] * G.ncell[1]);
-    const uint32_t amask = G.allow[member_cat[m]];
-    const int eb = excl_start[m], ec = excl_count[m];
-    const PairRec ri = rec[i];
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int base = FILL ? row_start[i] : 0;
-    int cnt = 0;
-    for (int dz = -G.reach[2]; dz <= G.reach[2]; ++dz) {
-        int z = cz + dz;
+    const int w0 = 2 * G.reach[0] + 1, w1 = 2 * G.reach[1] + 1, w2 = 2 * G.reach[2] + 1;
+    const int nst = w0 * w1 * w2;
+    if (tid < nst) {
+        const int dx = tid % w0 - G.reach[0], dy = (tid / w0) % w1 - G.reach[1], dz = tid / (w0 * w1) - G.reach[2];
+        int x = cx + dx, y = cy + dy, z = cz + dz;
+        if (x < 0) x += G.ncell[0]; else if (x >= G.ncell[0]) x -= G.ncell[0];
+        if (y < 0) y += G.ncell[1]; else if (y >= G.ncell[1]) y -= G.ncell[1];
         if (z < 0) z += G.ncell[2]; else if (z >= G.ncell[2]) z -= G.ncell[2];
-        for (int dy = -G.reach[1]; dy <= G.reach[1]; ++dy) {
-            int y = cy + dy;
-            if (y < 0) y += G.ncell[1]; else if (y >= G.ncell[1]) y -= G.ncell[1];
-            for (int dx = -G.reach[0]; dx <= G.reach[0]; ++dx) {
-                int x = cx + dx;
-                if (x < 0) x += G.ncell[0]; else if (x >= G.ncell[0]) x -= G.ncell[0];
-                const int c = G.cell_begin + (z * G.ncell[1] + y) * G.ncell[0] + x;
-                const int s0 = cell_start[c], s1 = s0 + cell_count[c];
-                for (int sb = s0; sb < s1; sb += 64) {
-                    const int s = sb + lane;
+        const int c = G.cell_begin + (z * G.ncell[1] + y) * G.ncell[0] + x;
+        st_start[tid] = cell_start[c];
+        st_pre[tid + 1] = cell_count[c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        st_pre[0] = 0;
+        for (int k = 0; k < nst; ++k) st_pre[k + 1] += st_pre[k];
+    }
+    __syncthreads();
+    const int ntot = st_pre[nst];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r0 = 0; r0 < hn; r0 += kTileRows) {
+        const int nr = min(kTileRows, hn - r0);
+        for (int r = tid; r < nr; r += blockDim.x) row_run[r] = 0;
+        for (int wbeg = 0; wbeg < ntot; wbeg += kTileCap) {
+            const int nw = min(kTileCap, ntot - wbeg);
+            __syncthreads();
+            for (int k = tid; k < nw; k += blockDim.x) {
+                const int flat = wbeg + k;
+                int lo = 0, hi = nst - 1;  // last stencil cell with st_pre <= flat
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (st_pre[mid] <= flat) lo = mid; else hi = mid - 1;
+                }
+                const int sl = st_start[lo] + flat - st_pre[lo];
+                cand[k] = rec[sl];
+                cand_slot[k] = sl;
+            }
+            __syncthreads();
+            for (int r = wave; r < nr; r += nwave) {
+                const int i = h0 + r0 + r;
+                const PairRec ri = rec[i];
+                const uint32_t amask = G.allow[ri.type >> 16];
+                const int m = slot_member[i];
+                const int eb = excl_start[m], ec = excl_count[m];
+                const int base = FILL ? row_start[i] : 0;
+                int run = row_run[r];
+                for (int kb = 0; kb < nw; kb += 64) {
+                    const int k = kb + lane;
                     bool pass = false;
-                    if (s < s1 && s != i) {
-                        const int mj = slot_member[s];
-                        if ((amask >> member_cat[mj]) & 1u) {
-                            const PairRec rj = rec[s];
+                    int sl = -1;
+                    if (k < nw) {
+                        sl = cand_slot[k];
+                        const PairRec rj = cand[k];
+                        if (sl != i && ((amask >> (rj.type >> 16)) & 1u)) {
                             double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
                             if (G.periodic) {
                                 ddx = min_image(ddx, G.box[0], G.inv_box[0]);
                                 ddy = min_image(ddy, G.box[1], G.inv_box[1]);
                                 ddz = min_image(ddz, G.box[2], G.inv_box[2]);
                             }
-                            pass = dist2(ddx, ddy, ddz) < G.rl2 && !(ec && excluded(excl, eb, ec, mj));
+                            pass = dist2(ddx, ddy, ddz) < G.rl2 && !(ec && excluded(excl, eb, ec, rj.bead));
                         }
                     }
                     const unsigned long long mask = __ballot(pass);
-                    if (FILL && pass) col[base + cnt + __popcll(mask & below)] = s;
-                    cnt += __popcll(mask);
+                    if (FILL && pass) col[base + run + __popcll(mask & below)] = sl;
+                    run += __popcll(mask);
                 }
+                if (lane == 0) row_run[r] = run;
             }
         }
+        __syncthreads();
+        if (!FILL)
+            for (int r = tid; r < nr; r += blockDim.x) row_count[h0 + r0 + r] = row_run[r];
     }
-    if (!FILL && lane == 0) row_count[i] = cnt;
 }
 
 // -------------------------------------------------------------------- scan
@@ -716,8 +759,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
         const PairRec ri = rec[row];
         bead_i = ri.bead;
         const int beg = row_start[row], cnt = row_count[row];
-        const float* tfi = lds_types + ri.type * kTypeStride;
-        const double* tdi = types_d + static_cast<size_t>(G.type_begin + ri.type) * kTypeStride;
+        const int ti = ri.type & 0xFFFF;
+        const float* tfi = lds_types + ti * kTypeStride;
+        const double* tdi = types_d + static_cast<size_t>(G.type_begin + ti) * kTypeStride;
         for (int k = lane; k < cnt; k += TEAM) {
             const int s = col[beg + k];
             const PairRec rj = rec[s];
@@ -733,7 +777,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
                 if (row < s) {
                     double f;
                     double e[4] = {0, 0, 0, 0};
-                    const double* tdj = types_d + static_cast<size_t>(G.type_begin + rj.type) * kTypeStride;
+                    const double* tdj = types_d + static_cast<size_t>(G.type_begin + (rj.type & 0xFFFF)) * kTypeStride;
                     pair_eval<double>(kinds, G, nullptr, nullptr, tdi, tdj, r2, &f, e);
                     ae0 += to_fix_d(e[0], bad);
                     ae1 += to_fix_d(e[1], bad);
@@ -743,7 +787,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
             } else {
                 float f;
                 float e[4];
-                const float* tfj = lds_types + rj.type * kTypeStride;
+                const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
                 pair_eval<float>(kinds, G, tfi, tfj, nullptr, nullptr, static_cast<float>(r2), &f, e);
                 ax += to_fix(f * static_cast<float>(dx), bad);
                 ay += to_fix(f * static_cast<float>(dy), bad);
@@ -896,16 +940,13 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
                                              d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
                                              d.rec, d.refx, d.refy, d.refz);
-    const int rows_blocks = nblk(d.nm, 4);
-    k_nl_rows<false><<<rows_blocks, 256, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
-                                                   d.member_cat, d.excl_start, d.excl_count, d.excl, d.cell_start,
-                                                   d.cell_count, d.rec, d.row_start, d.row_count, d.scan_total + 1,
-                                                   d.list_capacity, d.col, d.flags);
+    k_nl_tiles<false><<<d.ncell, 256, 0, st>>>(d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
+                                                 d.slot_member, d.excl_start, d.excl_count, d.excl, d.row_start,
+                                                 d.row_count, d.scan_total + 1, d.list_capacity, d.col, d.flags);
     launch_scan(d, st, d.row_count, d.row_start, d.nm, d.scan_total + 1);
-    k_nl_rows<true><<<rows_blocks, 256, 0, st>>>(d.nm, d.groups, d.member_group, d.slot_member, d.member_cell,
-                                                  d.member_cat, d.excl_start, d.excl_count, d.excl, d.cell_start,
-                                                  d.cell_count, d.rec, d.row_start, d.row_count, d.scan_total + 1,
-                                                  d.list_capacity, d.col, d.flags);
+    k_nl_tiles<true><<<d.ncell, 256, 0, st>>>(d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
+                                                d.slot_member, d.excl_start, d.excl_count, d.excl, d.row_start,
+                                                d.row_count, d.scan_total + 1, d.list_capacity, d.col, d.flags);
 }
 
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {

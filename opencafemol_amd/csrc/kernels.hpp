@@ -40,7 +40,7 @@ struct DevPtrs {
     double *refx = nullptr, *refy = nullptr, *refz = nullptr;
     PairRec* rec = nullptr;
     int* slot_member = nullptr;
-    int *cell_count = nullptr, *cell_start = nullptr;
+    int *cell_count = nullptr, *cell_start = nullptr, *cell_group = nullptr;
     int *row_count = nullptr, *row_start = nullptr, *col = nullptr;
     long long list_capacity = 0;
     int* scan_partial = nullptr;
