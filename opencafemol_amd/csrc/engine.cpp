@@ -327,6 +327,7 @@ void Engine::pack() {
     std::vector<DevPairGroup> groups;
     std::vector<int> member_group, member_bead, member_type, member_cat, excl_start, excl_count, excl;
     std::vector<int> cell_group;
+    double tile_rows_max = 1.0;
     std::vector<float> types_f;
     std::vector<double> types_d;
     int ncell_total = 0;
@@ -478,6 +479,16 @@ void Engine::pack() {
                 for (int c = 0; c < 3; ++c) { G.ncell[c] = 1; G.reach[c] = 0; G.cell_scale[c] = 0; }
                 est_neighbors = static_cast<long long>(G.member_count) * std::min(G.member_count, 256);
             }
+            {
+                // rows expected in the densest cells (2x the mean occupancy)
+                const double cells = static_cast<double>(G.ncell[0]) * G.ncell[1] * G.ncell[2];
+                double per_cell = G.member_count / cells;
+                if (G.periodic) {
+                    const double vcell = H.box[0] * H.box[1] * H.box[2] / cells;
+                    per_cell = std::max(per_cell, vcell * G.member_count / (H.box[0] * H.box[1] * H.box[2]));
+                }
+                tile_rows_max = std::max(tile_rows_max, 2.0 * per_cell);
+            }
             G.cell_begin = ncell_total;
             ncell_total += G.ncell[0] * G.ncell[1] * G.ncell[2];
             cell_group.resize(ncell_total, gid);
@@ -491,6 +502,7 @@ void Engine::pack() {
     d_.ncell = std::max(ncell_total, 1);
     d_.ngroups = static_cast<int>(groups.size());
     d_.max_types = std::max(max_types, 1);
+    d_.tile_split = std::max(1, std::min(64, static_cast<int>(std::ceil(tile_rows_max / 16.0))));
 
     // ---- device allocation and upload
     const int n3 = 3 * nb;
@@ -696,10 +708,25 @@ void Engine::ensure_forces() {
             continue;
         }
         forces_dirty_ = false;
+        tune_tile_split();
         if (!team_tuned_) autotune_team();
         return;
     }
     throw Error(OCM_ERR_OOM, "neighbor list did not fit after repeated growth");
+}
+
+// Workgroups per home cell in the list build: enough 16-row chunks to cover
+// the most populated cell (with 25% headroom) of the current configuration.
+void Engine::tune_tile_split() {
+    if (d_.nm == 0) return;
+    std::vector<int> cnt(d_.ncell);
+    check_hip(hipMemcpy(cnt.data(), d_.cell_count, cnt.size() * sizeof(int), hipMemcpyDeviceToHost), "cells");
+    const int mx = *std::max_element(cnt.begin(), cnt.end());
+    const int split = std::max(1, std::min(64, (mx * 5 / 4 + 15) / 16));
+    if (split != d_.tile_split) {
+        d_.tile_split = split;
+        drop_graphs();
+    }
 }
 
 void Engine::autotune_team() {

@@ -250,9 +250,11 @@ constexpr int kTileCap = 1024;
 constexpr int kTileRows = 256;
 constexpr int kMaxStencil = 27;
 
+constexpr int kTileChunk = 16;  // rows per chunk; chunk q of a cell goes to split q % nsplit
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_nl_tiles(
-    const DevPairGroup* __restrict__ groups, const int* __restrict__ cell_group,
+    int nsplit, const DevPairGroup* __restrict__ groups, const int* __restrict__ cell_group,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
     const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
@@ -261,14 +263,20 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     __shared__ int cand_slot[kTileCap];
     __shared__ int row_run[kTileRows];
     __shared__ int st_start[kMaxStencil], st_pre[kMaxStencil + 1];
-    const int home = blockIdx.x;
+    const int home = blockIdx.x / nsplit, part = blockIdx.x % nsplit;
     const int hn = cell_count[home];
-    if (hn == 0) return;
+    if (part * kTileChunk >= hn) return;
     const int h0 = cell_start[home];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+    // rows of this block: chunks part, part + nsplit, ... of the home cell,
+    // numbered locally 0..nloc-1; local row L is cell row
+    // ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk
+    int nloc = 0;
+    for (int q = part; q * kTileChunk < hn; q += nsplit) nloc += min(kTileChunk, hn - q * kTileChunk);
     if (FILL && static_cast<long long>(*total) > capacity) {
-        for (int r = tid; r < hn; r += blockDim.x) row_count[h0 + r] = 0;  // the pair kernel reads nothing
-        if (tid == 0) atomicOr(flags, kFlagListOverflow);
+        for (int L = tid; L < nloc; L += blockDim.x)  // the pair kernel then reads nothing
+            row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = 0;
+        if (tid == 0 && home == 0 && part == 0) atomicOr(flags, kFlagListOverflow);
         return;
     }
     const DevPairGroup& G = groups[cell_group[home]];
@@ -296,8 +304,8 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     __syncthreads();
     const int ntot = st_pre[nst];
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int r0 = 0; r0 < hn; r0 += kTileRows) {
-        const int nr = min(kTileRows, hn - r0);
+    for (int r0 = 0; r0 < nloc; r0 += kTileRows) {
+        const int nr = min(kTileRows, nloc - r0);
         for (int r = tid; r < nr; r += blockDim.x) row_run[r] = 0;
         for (int wbeg = 0; wbeg < ntot; wbeg += kTileCap) {
             const int nw = min(kTileCap, ntot - wbeg);
@@ -315,7 +323,8 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
             }
             __syncthreads();
             for (int r = wave; r < nr; r += nwave) {
-                const int i = h0 + r0 + r;
+                const int L = r0 + r;
+                const int i = h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk;
                 const PairRec ri = rec[i];
                 const uint32_t amask = G.allow[ri.type >> 16];
                 const int m = slot_member[i];
@@ -348,7 +357,10 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
         }
         __syncthreads();
         if (!FILL)
-            for (int r = tid; r < nr; r += blockDim.x) row_count[h0 + r0 + r] = row_run[r];
+            for (int r = tid; r < nr; r += blockDim.x) {
+                const int L = r0 + r;
+                row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = row_run[r];
+            }
     }
 }
 
@@ -940,11 +952,12 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
                                              d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
                                              d.rec, d.refx, d.refy, d.refz);
-    k_nl_tiles<false><<<d.ncell, 256, 0, st>>>(d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
+    const int tiles = d.ncell * d.tile_split;
+    k_nl_tiles<false><<<tiles, 256, 0, st>>>(d.tile_split, d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
                                                  d.slot_member, d.excl_start, d.excl_count, d.excl, d.row_start,
                                                  d.row_count, d.scan_total + 1, d.list_capacity, d.col, d.flags);
     launch_scan(d, st, d.row_count, d.row_start, d.nm, d.scan_total + 1);
-    k_nl_tiles<true><<<d.ncell, 256, 0, st>>>(d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
+    k_nl_tiles<true><<<tiles, 256, 0, st>>>(d.tile_split, d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
                                                 d.slot_member, d.excl_start, d.excl_count, d.excl, d.row_start,
                                                 d.row_count, d.scan_total + 1, d.list_capacity, d.col, d.flags);
 }

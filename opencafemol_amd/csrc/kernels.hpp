@@ -43,6 +43,7 @@ struct DevPtrs {
     int *cell_count = nullptr, *cell_start = nullptr, *cell_group = nullptr;
     int *row_count = nullptr, *row_start = nullptr, *col = nullptr;
     long long list_capacity = 0;
+    int tile_split = 1;            // workgroups per home cell in the list build
     int* scan_partial = nullptr;
     int* scan_total = nullptr;     // [0] cells total, [1] list total
     float* types_f = nullptr;
