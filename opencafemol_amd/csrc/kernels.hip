@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     if (FILL && static_cast<long long>(*total) > capacity) {
         for (int L = tid; L < nloc; L += blockDim.x)  // the pair kernel then reads nothing
             row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = 0;
-        if (tid == 0 && home == 0 && part == 0) atomicOr(flags, kFlagListOverflow);
+        if (tid == 0) atomicOr(flags, kFlagListOverflow);  // any non-empty block may report it
         return;
     }
     const DevPairGroup& G = groups[cell_group[home]];
@@ -361,6 +361,7 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                 const int L = r0 + r;
                 row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = row_run[r];
             }
+        __syncthreads();  // row_run is reset by the next batch
     }
 }
 
