@@ -572,7 +572,7 @@ void Engine::pack() {
     check_hip(hipMemset(d_.row_start, 0, sizeof(int) * std::max(nm, 1)), "memset");
     long long est = 0;
     for (long long e : group_list_estimate_) est += e;
-    d_.list_capacity = std::max<long long>(1 << 16, est);
+    d_.list_capacity = std::max<long long>(1 << 16, est + 4LL * nm);  // rows padded to 4 entries
     if (d_.list_capacity > std::numeric_limits<int>::max()) d_.list_capacity = std::numeric_limits<int>::max();
     d_.col = dalloc<int>(d_.list_capacity);
     col_bytes_ = d_.list_capacity * static_cast<long long>(sizeof(int));

@@ -204,7 +204,20 @@ __global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
     }
     const int cell = G.cell_begin + (c[2] * G.ncell[1] + c[1]) * G.ncell[0] + c[0];
     member_cell[m] = cell;
-    member_rank[m] = atomicAdd(&cell_count[cell], 1);
+    // consecutive members (beads along a chain) mostly share a cell: one
+    // atomic per distinct cell of the wavefront instead of one per member
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(true);
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int lc = __shfl(cell, leader, 64);
+        const unsigned long long peers = __ballot(cell == lc) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cell_count[lc], __popcll(peers));
+        base = __shfl(base, leader, 64);
+        if (cell == lc) member_rank[m] = base + __popcll(peers & ((1ull << lane) - 1ull));
+        todo &= ~peers;
+    }
 }
 
 __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* __restrict__ member_type,
@@ -329,6 +342,13 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                 const uint32_t amask = G.allow[ri.type >> 16];
                 const int m = slot_member[i];
                 const int eb = excl_start[m], ec = excl_count[m];
+                // the row's excluded partners (bead ids) held wave-uniformly in
+                // scalar registers when there are at most 16 of them
+                const int mine = lane < ec ? excl[eb + lane] : -1;
+                int ex[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) ex[e] = __builtin_amdgcn_readlane(mine, e);
+                const bool few = ec <= 16;
                 const int base = FILL ? row_start[i] : 0;
                 int run = row_run[r];
                 for (int kb = 0; kb < nw; kb += 64) {
@@ -345,7 +365,17 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                                 ddy = min_image(ddy, G.box[1], G.inv_box[1]);
                                 ddz = min_image(ddz, G.box[2], G.inv_box[2]);
                             }
-                            pass = dist2(ddx, ddy, ddz) < G.rl2 && !(ec && excluded(excl, eb, ec, rj.bead));
+                            pass = dist2(ddx, ddy, ddz) < G.rl2;
+                            if (pass && ec) {
+                                if (few) {
+                                    bool hit = false;
+#pragma unroll
+                                    for (int e = 0; e < 16; ++e) hit |= (e < ec) & (ex[e] == rj.bead);
+                                    pass = !hit;
+                                } else {
+                                    pass = !excluded(excl, eb, ec, rj.bead);
+                                }
+                            }
                         }
                     }
                     const unsigned long long mask = __ballot(pass);
@@ -353,13 +383,15 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                     run += __popcll(mask);
                 }
                 if (lane == 0) row_run[r] = run;
+                // last window: pad the row to a multiple of 4 entries with -1
+                if (FILL && wbeg + kTileCap >= ntot && lane < ((4 - (run & 3)) & 3)) col[base + run + lane] = -1;
             }
         }
         __syncthreads();
         if (!FILL)
             for (int r = tid; r < nr; r += blockDim.x) {
                 const int L = r0 + r;
-                row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = row_run[r];
+                row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = (row_run[r] + 3) & ~3;
             }
         __syncthreads();  // row_run is reset by the next batch
     }
@@ -775,36 +807,48 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
         const int ti = ri.type & 0xFFFF;
         const float* tfi = lds_types + ti * kTypeStride;
         const double* tdi = types_d + static_cast<size_t>(G.type_begin + ti) * kTypeStride;
-        for (int k = lane; k < cnt; k += TEAM) {
-            const int s = col[beg + k];
-            const PairRec rj = rec[s];
-            double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-            if (G.periodic) {
-                dx = min_image(dx, G.box[0], G.inv_box[0]);
-                dy = min_image(dy, G.box[1], G.inv_box[1]);
-                dz = min_image(dz, G.box[2], G.inv_box[2]);
-            }
-            const double r2 = dist2(dx, dy, dz);
-            if (!(r2 < G.rc2)) continue;
-            if (ENERGY) {
-                if (row < s) {
-                    double f;
-                    double e[4] = {0, 0, 0, 0};
-                    const double* tdj = types_d + static_cast<size_t>(G.type_begin + (rj.type & 0xFFFF)) * kTypeStride;
-                    pair_eval<double>(kinds, G, nullptr, nullptr, tdi, tdj, r2, &f, e);
-                    ae0 += to_fix_d(e[0], bad);
-                    ae1 += to_fix_d(e[1], bad);
-                    ae2 += to_fix_d(e[2], bad);
-                    ae3 += to_fix_d(e[3], bad);
+        // rows are padded to multiples of 4 entries (-1 = empty); a lane takes
+        // 4 consecutive entries per round so 4 record gathers are in flight
+        const int4* col4 = reinterpret_cast<const int4*>(col + beg);
+        for (int k = lane; 4 * k < cnt; k += TEAM) {
+            const int4 c4 = col4[k];
+            const int cs[4] = {c4.x, c4.y, c4.z, c4.w};
+            PairRec rjs[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rjs[u] = rec[cs[u] >= 0 ? cs[u] : row];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int s = cs[u];
+                if (s < 0) continue;
+                const PairRec rj = rjs[u];
+                double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+                if (G.periodic) {
+                    dx = min_image(dx, G.box[0], G.inv_box[0]);
+                    dy = min_image(dy, G.box[1], G.inv_box[1]);
+                    dz = min_image(dz, G.box[2], G.inv_box[2]);
                 }
-            } else {
-                float f;
-                float e[4];
-                const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
-                pair_eval<float>(kinds, G, tfi, tfj, nullptr, nullptr, static_cast<float>(r2), &f, e);
-                ax += to_fix(f * static_cast<float>(dx), bad);
-                ay += to_fix(f * static_cast<float>(dy), bad);
-                az += to_fix(f * static_cast<float>(dz), bad);
+                const double r2 = dist2(dx, dy, dz);
+                if (!(r2 < G.rc2)) continue;
+                if (ENERGY) {
+                    if (row < s) {
+                        double f;
+                        double e[4] = {0, 0, 0, 0};
+                        const double* tdj = types_d + static_cast<size_t>(G.type_begin + (rj.type & 0xFFFF)) * kTypeStride;
+                        pair_eval<double>(kinds, G, nullptr, nullptr, tdi, tdj, r2, &f, e);
+                        ae0 += to_fix_d(e[0], bad);
+                        ae1 += to_fix_d(e[1], bad);
+                        ae2 += to_fix_d(e[2], bad);
+                        ae3 += to_fix_d(e[3], bad);
+                    }
+                } else {
+                    float f;
+                    float e[4];
+                    const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
+                    pair_eval<float>(kinds, G, tfi, tfj, nullptr, nullptr, static_cast<float>(r2), &f, e);
+                    ax += to_fix(f * static_cast<float>(dx), bad);
+                    ay += to_fix(f * static_cast<float>(dy), bad);
+                    az += to_fix(f * static_cast<float>(dz), bad);
+                }
             }
         }
     }
@@ -860,6 +904,7 @@ __global__ void k_neighbor_dump(int group, const DevPairGroup* __restrict__ grou
     const PairRec ri = rec[row];
     const int beg = row_start[row], cnt = row_count[row];
     for (int k = 0; k < cnt; ++k) {
+        if (col[beg + k] < 0) continue;
         const PairRec rj = rec[col[beg + k]];
         if (!(ri.bead < rj.bead)) continue;
         double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
