@@ -327,7 +327,6 @@ void Engine::pack() {
     std::vector<DevPairGroup> groups;
     std::vector<int> member_group, member_bead, member_type, member_cat, excl_start, excl_count, excl;
     std::vector<int> cell_group;
-    double tile_rows_max = 1.0;
     std::vector<float> types_f;
     std::vector<double> types_d;
     int ncell_total = 0;
@@ -479,16 +478,6 @@ void Engine::pack() {
                 for (int c = 0; c < 3; ++c) { G.ncell[c] = 1; G.reach[c] = 0; G.cell_scale[c] = 0; }
                 est_neighbors = static_cast<long long>(G.member_count) * std::min(G.member_count, 256);
             }
-            {
-                // rows expected in the densest cells (2x the mean occupancy)
-                const double cells = static_cast<double>(G.ncell[0]) * G.ncell[1] * G.ncell[2];
-                double per_cell = G.member_count / cells;
-                if (G.periodic) {
-                    const double vcell = H.box[0] * H.box[1] * H.box[2] / cells;
-                    per_cell = std::max(per_cell, vcell * G.member_count / (H.box[0] * H.box[1] * H.box[2]));
-                }
-                tile_rows_max = std::max(tile_rows_max, 2.0 * per_cell);
-            }
             G.cell_begin = ncell_total;
             ncell_total += G.ncell[0] * G.ncell[1] * G.ncell[2];
             cell_group.resize(ncell_total, gid);
@@ -502,7 +491,7 @@ void Engine::pack() {
     d_.ncell = std::max(ncell_total, 1);
     d_.ngroups = static_cast<int>(groups.size());
     d_.max_types = std::max(max_types, 1);
-    d_.tile_split = std::max(1, std::min(64, static_cast<int>(std::ceil(tile_rows_max / 16.0))));
+
 
     // ---- device allocation and upload
     const int n3 = 3 * nb;
@@ -564,6 +553,8 @@ void Engine::pack() {
     d_.cell_count = dalloc<int>(d_.ncell);
     d_.cell_start = dalloc<int>(d_.ncell);
     d_.cell_group = dalloc<int>(d_.ncell);
+    d_.tile_items = dalloc<int2>(static_cast<size_t>(nm) / 16 + std::min(d_.ncell, std::max(nm, 1)) + 16);
+    d_.tile_nitems = dalloc<int>(1);
     if (cell_group.empty()) cell_group.assign(d_.ncell, 0);
     upload(d_.cell_group, cell_group);
     d_.row_count = dalloc<int>(nm);
@@ -708,52 +699,45 @@ void Engine::ensure_forces() {
             continue;
         }
         forces_dirty_ = false;
-        tune_tile_split();
         if (!team_tuned_) autotune_team();
         return;
     }
     throw Error(OCM_ERR_OOM, "neighbor list did not fit after repeated growth");
 }
 
-// Workgroups per home cell in the list build: enough 16-row chunks to cover
-// the most populated cell (with 25% headroom) of the current configuration.
-void Engine::tune_tile_split() {
-    if (d_.nm == 0) return;
-    std::vector<int> cnt(d_.ncell);
-    check_hip(hipMemcpy(cnt.data(), d_.cell_count, cnt.size() * sizeof(int), hipMemcpyDeviceToHost), "cells");
-    const int mx = *std::max_element(cnt.begin(), cnt.end());
-    const int split = std::max(1, std::min(64, (mx * 5 / 4 + 15) / 16));
-    if (split != d_.tile_split) {
-        d_.tile_split = split;
-        drop_graphs();
-    }
-}
-
+// Project autotuner for the pair kernel's launch shape: lanes per row (team)
+// and the number of workgroups of the fixed grid.  Forces are fixed-point
+// sums, so every shape gives identical bits; the fastest on the current list
+// wins.
 void Engine::autotune_team() {
     team_tuned_ = true;
     if (d_.nm == 0) return;
-    // Forces are fixed-point sums, so every team size gives identical bits;
-    // pick the fastest on the current list.
-    const int cands[4] = {4, 8, 16, 32};
     hipEvent_t a, b;
     check_hip(hipEventCreate(&a), "event");
     check_hip(hipEventCreate(&b), "event");
     float best = 1e30f;
-    int best_team = d_.team;
-    for (int t : cands) {
+    int best_team = d_.team, best_grid = d_.pair_grid;
+    const int teams[4] = {8, 16, 32, 64};
+    const int grids[4] = {512, 1024, 2048, 1 << 30};
+    for (int t : teams) {
         build_pair_blocks(t);
-        launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);  // warm-up into the scratch buffer
-        check_hip(hipEventRecord(a, stream_), "event");
-        for (int r = 0; r < 3; ++r) launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);
-        check_hip(hipEventRecord(b, stream_), "event");
-        check_hip(hipEventSynchronize(b), "event");
-        float ms = 0;
-        check_hip(hipEventElapsedTime(&ms, a, b), "event");
-        if (ms < best) { best = ms; best_team = t; }
+        for (int g : grids) {
+            d_.pair_grid = g;
+            launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);  // warm-up into the scratch buffer
+            check_hip(hipEventRecord(a, stream_), "event");
+            for (int r = 0; r < 3; ++r) launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);
+            check_hip(hipEventRecord(b, stream_), "event");
+            check_hip(hipEventSynchronize(b), "event");
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, a, b), "event");
+            if (ms < best) { best = ms; best_team = t; best_grid = g; }
+            if (g >= d_.n_pair_blocks) break;
+        }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     build_pair_blocks(best_team);
+    d_.pair_grid = best_grid;
     // the tuning launches may have flagged numeric trouble in the scratch pass only
     check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
     sync();

@@ -96,7 +96,6 @@ private:
     void sync();
     void grow_list(long long needed);
     void autotune_team();
-    void tune_tile_split();
     void drop_graphs();
     template <typename F> void timed(int slot, F&& f);
     void collect_timing();

@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace ocm {
 
 // ------------------------------------------------------------ small helpers
@@ -260,37 +262,46 @@ __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, 
 // allowed and it is not excluded; passes are counted with a ballot and, in
 // the FILL pass, written in window order at base + popcount of lower lanes.
 constexpr int kTileCap = 1024;
-constexpr int kTileRows = 256;
 constexpr int kMaxStencil = 27;
 
-constexpr int kTileChunk = 16;  // rows per chunk; chunk q of a cell goes to split q % nsplit
+constexpr int kTileChunk = 16;  // rows per work item
+
+// Work items of the tile build: one per (non-empty cell, 16-row chunk),
+// appended by k_tile_items after the cell scan; the tile kernels run on a
+// fixed grid and loop over them.
+__global__ void k_tile_items(int ncell, const int* __restrict__ cell_count, int2* __restrict__ items,
+                             int* __restrict__ n_items) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int n = (cell_count[c] + kTileChunk - 1) / kTileChunk;
+    if (n == 0) return;
+    const int at = atomicAdd(n_items, n);
+    for (int q = 0; q < n; ++q) items[at + q] = make_int2(c, q);
+}
 
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_nl_tiles(
-    int nsplit, const DevPairGroup* __restrict__ groups, const int* __restrict__ cell_group,
+    const int2* __restrict__ items, const int* __restrict__ n_items, const DevPairGroup* __restrict__ groups,
+    const int* __restrict__ cell_group,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
     const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
     const int* __restrict__ total, long long capacity, int* __restrict__ col, int* __restrict__ flags) {
     __shared__ PairRec cand[kTileCap];
     __shared__ int cand_slot[kTileCap];
-    __shared__ int row_run[kTileRows];
+    __shared__ int row_run[kTileChunk];
     __shared__ int st_start[kMaxStencil], st_pre[kMaxStencil + 1];
-    const int home = blockIdx.x / nsplit, part = blockIdx.x % nsplit;
-    const int hn = cell_count[home];
-    if (part * kTileChunk >= hn) return;
-    const int h0 = cell_start[home];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
-    // rows of this block: chunks part, part + nsplit, ... of the home cell,
-    // numbered locally 0..nloc-1; local row L is cell row
-    // ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk
-    int nloc = 0;
-    for (int q = part; q * kTileChunk < hn; q += nsplit) nloc += min(kTileChunk, hn - q * kTileChunk);
+    const int nit = *n_items;
+    for (int item = blockIdx.x; item < nit; item += gridDim.x) {
+    __syncthreads();  // LDS of the previous item is no longer read
+    const int home = items[item].x;
+    const int h0 = cell_start[home] + items[item].y * kTileChunk;
+    const int nr = min(kTileChunk, cell_count[home] - items[item].y * kTileChunk);
     if (FILL && static_cast<long long>(*total) > capacity) {
-        for (int L = tid; L < nloc; L += blockDim.x)  // the pair kernel then reads nothing
-            row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = 0;
-        if (tid == 0) atomicOr(flags, kFlagListOverflow);  // any non-empty block may report it
-        return;
+        for (int r = tid; r < nr; r += blockDim.x) row_count[h0 + r] = 0;  // the pair kernel reads nothing
+        if (tid == 0) atomicOr(flags, kFlagListOverflow);
+        continue;
     }
     const DevPairGroup& G = groups[cell_group[home]];
     const int cell = home - G.cell_begin;
@@ -317,84 +328,76 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     __syncthreads();
     const int ntot = st_pre[nst];
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int r0 = 0; r0 < nloc; r0 += kTileRows) {
-        const int nr = min(kTileRows, nloc - r0);
-        for (int r = tid; r < nr; r += blockDim.x) row_run[r] = 0;
-        for (int wbeg = 0; wbeg < ntot; wbeg += kTileCap) {
-            const int nw = min(kTileCap, ntot - wbeg);
-            __syncthreads();
-            for (int k = tid; k < nw; k += blockDim.x) {
-                const int flat = wbeg + k;
-                int lo = 0, hi = nst - 1;  // last stencil cell with st_pre <= flat
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (st_pre[mid] <= flat) lo = mid; else hi = mid - 1;
-                }
-                const int sl = st_start[lo] + flat - st_pre[lo];
-                cand[k] = rec[sl];
-                cand_slot[k] = sl;
+    for (int r = tid; r < nr; r += blockDim.x) row_run[r] = 0;
+    for (int wbeg = 0; wbeg < ntot; wbeg += kTileCap) {
+        const int nw = min(kTileCap, ntot - wbeg);
+        __syncthreads();
+        for (int k = tid; k < nw; k += blockDim.x) {
+            const int flat = wbeg + k;
+            int lo = 0, hi = nst - 1;  // last stencil cell with st_pre <= flat
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (st_pre[mid] <= flat) lo = mid; else hi = mid - 1;
             }
-            __syncthreads();
-            for (int r = wave; r < nr; r += nwave) {
-                const int L = r0 + r;
-                const int i = h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk;
-                const PairRec ri = rec[i];
-                const uint32_t amask = G.allow[ri.type >> 16];
-                const int m = slot_member[i];
-                const int eb = excl_start[m], ec = excl_count[m];
-                // the row's excluded partners (bead ids) held wave-uniformly in
-                // scalar registers when there are at most 16 of them
-                const int mine = lane < ec ? excl[eb + lane] : -1;
-                int ex[16];
+            const int sl = st_start[lo] + flat - st_pre[lo];
+            cand[k] = rec[sl];
+            cand_slot[k] = sl;
+        }
+        __syncthreads();
+        for (int r = wave; r < nr; r += nwave) {
+            const int i = h0 + r;
+            const PairRec ri = rec[i];
+            const uint32_t amask = G.allow[ri.type >> 16];
+            const int m = slot_member[i];
+            const int eb = excl_start[m], ec = excl_count[m];
+            // the row's excluded partners (bead ids) held wave-uniformly when <= 16
+            const int mine = lane < ec ? excl[eb + lane] : -1;
+            int ex[16];
 #pragma unroll
-                for (int e = 0; e < 16; ++e) ex[e] = __builtin_amdgcn_readlane(mine, e);
-                const bool few = ec <= 16;
-                const int base = FILL ? row_start[i] : 0;
-                int run = row_run[r];
-                for (int kb = 0; kb < nw; kb += 64) {
-                    const int k = kb + lane;
-                    bool pass = false;
-                    int sl = -1;
-                    if (k < nw) {
-                        sl = cand_slot[k];
-                        const PairRec rj = cand[k];
-                        if (sl != i && ((amask >> (rj.type >> 16)) & 1u)) {
-                            double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
-                            if (G.periodic) {
-                                ddx = min_image(ddx, G.box[0], G.inv_box[0]);
-                                ddy = min_image(ddy, G.box[1], G.inv_box[1]);
-                                ddz = min_image(ddz, G.box[2], G.inv_box[2]);
-                            }
-                            pass = dist2(ddx, ddy, ddz) < G.rl2;
-                            if (pass && ec) {
-                                if (few) {
-                                    bool hit = false;
+            for (int e = 0; e < 16; ++e) ex[e] = __builtin_amdgcn_readlane(mine, e);
+            const bool few = ec <= 16;
+            const int base = FILL ? row_start[i] : 0;
+            int run = row_run[r];
+            for (int kb = 0; kb < nw; kb += 64) {
+                const int k = kb + lane;
+                bool pass = false;
+                int sl = -1;
+                if (k < nw) {
+                    sl = cand_slot[k];
+                    const PairRec rj = cand[k];
+                    if (sl != i && ((amask >> (rj.type >> 16)) & 1u)) {
+                        double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
+                        if (G.periodic) {
+                            ddx = min_image(ddx, G.box[0], G.inv_box[0]);
+                            ddy = min_image(ddy, G.box[1], G.inv_box[1]);
+                            ddz = min_image(ddz, G.box[2], G.inv_box[2]);
+                        }
+                        pass = dist2(ddx, ddy, ddz) < G.rl2;
+                        if (pass && ec) {
+                            if (few) {
+                                bool hit = false;
 #pragma unroll
-                                    for (int e = 0; e < 16; ++e) hit |= (e < ec) & (ex[e] == rj.bead);
-                                    pass = !hit;
-                                } else {
-                                    pass = !excluded(excl, eb, ec, rj.bead);
-                                }
+                                for (int e = 0; e < 16; ++e) hit |= (e < ec) & (ex[e] == rj.bead);
+                                pass = !hit;
+                            } else {
+                                pass = !excluded(excl, eb, ec, rj.bead);
                             }
                         }
                     }
-                    const unsigned long long mask = __ballot(pass);
-                    if (FILL && pass) col[base + run + __popcll(mask & below)] = sl;
-                    run += __popcll(mask);
                 }
-                if (lane == 0) row_run[r] = run;
-                // last window: pad the row to a multiple of 4 entries with -1
-                if (FILL && wbeg + kTileCap >= ntot && lane < ((4 - (run & 3)) & 3)) col[base + run + lane] = -1;
+                const unsigned long long mask = __ballot(pass);
+                if (FILL && pass) col[base + run + __popcll(mask & below)] = sl;
+                run += __popcll(mask);
             }
+            if (lane == 0) row_run[r] = run;
+            // last window: pad the row to a multiple of 4 entries with -1
+            if (FILL && wbeg + kTileCap >= ntot && lane < ((4 - (run & 3)) & 3)) col[base + run + lane] = -1;
         }
-        __syncthreads();
-        if (!FILL)
-            for (int r = tid; r < nr; r += blockDim.x) {
-                const int L = r0 + r;
-                row_count[h0 + ((L / kTileChunk) * nsplit + part) * kTileChunk + L % kTileChunk] = (row_run[r] + 3) & ~3;
-            }
-        __syncthreads();  // row_run is reset by the next batch
     }
+    __syncthreads();
+    if (!FILL)
+        for (int r = tid; r < nr; r += blockDim.x) row_count[h0 + r] = (row_run[r] + 3) & ~3;
+    }  // items
 }
 
 // -------------------------------------------------------------------- scan
@@ -780,14 +783,19 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     const PairRec* __restrict__ rec, const int* __restrict__ row_start,
     const int* __restrict__ row_count, const int* __restrict__ col,
     const float* __restrict__ types_f, const double* __restrict__ types_d, ForceBuf out,
-    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err) {
+    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks) {
     extern __shared__ float lds_types[];
-    const int2 bt = blocks[blockIdx.x];
+    int loaded = -1;
+    // fixed grid: each workgroup walks the block table with stride gridDim.x
+    for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    const int2 bt = blocks[bi];
     const DevPairGroup& G = groups[bt.x];
-    if (!ENERGY) {
+    if (!ENERGY && bt.x != loaded) {  // bt.x is uniform over the workgroup
+        __syncthreads();
         const float* src = types_f + static_cast<size_t>(G.type_begin) * kTypeStride;
         for (int k = threadIdx.x; k < G.n_types * kTypeStride; k += blockDim.x) lds_types[k] = src[k];
         __syncthreads();
+        loaded = bt.x;
     }
     int kinds = 0;
 #pragma unroll
@@ -876,6 +884,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
         }
     }
     if (bad) atomicOr(&err[G.sys], kErrNumeric);
+    }  // block table
 }
 
 // ------------------------------------------------------------- energies
@@ -998,14 +1007,17 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
                                              d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
                                              d.rec, d.refx, d.refy, d.refz);
-    const int tiles = d.ncell * d.tile_split;
-    k_nl_tiles<false><<<tiles, 256, 0, st>>>(d.tile_split, d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
-                                                 d.slot_member, d.excl_start, d.excl_count, d.excl, d.row_start,
-                                                 d.row_count, d.scan_total + 1, d.list_capacity, d.col, d.flags);
+    k_fill_i32<<<1, 64, 0, st>>>(d.tile_nitems, 0, 1);
+    k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_count, d.tile_items, d.tile_nitems);
+    k_nl_tiles<false><<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group,
+                                                    d.cell_start, d.cell_count, d.rec, d.slot_member, d.excl_start,
+                                                    d.excl_count, d.excl, d.row_start, d.row_count, d.scan_total + 1,
+                                                    d.list_capacity, d.col, d.flags);
     launch_scan(d, st, d.row_count, d.row_start, d.nm, d.scan_total + 1);
-    k_nl_tiles<true><<<tiles, 256, 0, st>>>(d.tile_split, d.groups, d.cell_group, d.cell_start, d.cell_count, d.rec,
-                                                d.slot_member, d.excl_start, d.excl_count, d.excl, d.row_start,
-                                                d.row_count, d.scan_total + 1, d.list_capacity, d.col, d.flags);
+    k_nl_tiles<true><<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group,
+                                                   d.cell_start, d.cell_count, d.rec, d.slot_member, d.excl_start,
+                                                   d.excl_count, d.excl, d.row_start, d.row_count, d.scan_total + 1,
+                                                   d.list_capacity, d.col, d.flags);
 }
 
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {
@@ -1022,10 +1034,11 @@ void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf
 template <bool E>
 static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, uint32_t groups) {
     const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
-    const dim3 grid(d.n_pair_blocks), block(kPairBlock);
+    const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
 #define OCM_PAIR(T)                                                                                   \
     k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, \
-                                           d.col, d.types_f, d.types_d, f, d.energy, groups, d.err)
+                                           d.col, d.types_f, d.types_d, f, d.energy, groups, d.err,   \
+                                           d.n_pair_blocks)
     switch (d.team) {
     case 4: OCM_PAIR(4); break;
     case 8: OCM_PAIR(8); break;

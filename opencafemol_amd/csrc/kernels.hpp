@@ -43,7 +43,10 @@ struct DevPtrs {
     int *cell_count = nullptr, *cell_start = nullptr, *cell_group = nullptr;
     int *row_count = nullptr, *row_start = nullptr, *col = nullptr;
     long long list_capacity = 0;
-    int tile_split = 1;            // workgroups per home cell in the list build
+    int2* tile_items = nullptr;    // (cell, chunk) work items of the list build
+    int* tile_nitems = nullptr;
+    int tile_grid = 2048;          // workgroups of the list-build kernels
+    int pair_grid = 2048;          // workgroups of the pair kernel
     int* scan_partial = nullptr;
     int* scan_total = nullptr;     // [0] cells total, [1] list total
     float* types_f = nullptr;
