@@ -85,6 +85,7 @@ struct DevPairGroup {
     double exv_eps, exv_corr;
     double dh_K, dh_inv_lambda, dh_corr;
     double ljr_rsw, lja_rsw, exv_rsw, dh_rsw;   // switching distances (<= 0: none)
+    double lja_inv_sw;     // 1 / (cutoff - lja_rsw)
     uint32_t allow[32];    // category allow masks
 };
 

@@ -390,7 +390,10 @@ void Engine::pack() {
                 G.dh_inv_lambda = 1.0 / byk[1]->globals[1];
                 G.dh_corr = byk[1]->globals[2];
             }
-            if (byk[3]) G.lja_rsw = byk[3]->switch_distance;
+            if (byk[3]) {
+                G.lja_rsw = byk[3]->switch_distance;
+                if (G.lja_rsw > 0) G.lja_inv_sw = 1.0 / (f0.cutoff - G.lja_rsw);
+            }
             // categories
             G.n_cat = f0.n_categories;
             for (int a = 0; a < G.n_cat; ++a) {

@@ -703,6 +703,15 @@ __global__ void k_bonded(int nt, const BTerm* __restrict__ terms, const DevSys* 
 }
 
 // ------------------------------------------------------------------- pair
+// Reciprocals: the FP32 force pass uses the hardware v_rcp/v_rsq (1 ulp);
+// the FP64 energy pass divides exactly.
+template <typename R> __device__ inline R r_rcp(R v);
+template <> __device__ inline float r_rcp<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+template <> __device__ inline double r_rcp<double>(double v) { return 1.0 / v; }
+template <typename R> __device__ inline R r_rsqrt(R v);
+template <> __device__ inline float r_rsqrt<float>(float v) { return __builtin_amdgcn_rsqf(v); }
+template <> __device__ inline double r_rsqrt<double>(double v) { return 1.0 / sqrt(v); }
+
 // Energy terms and (dE/dr)/r of all kinds of a pair group for one pair.
 // R = float (force pass, FP32 type rows tf) or double (energy pass, rows td).
 template <typename R>
@@ -713,10 +722,13 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
         if constexpr (sizeof(R) == 4) return i_side ? tfi[k] : tfj[k];
         else return i_side ? tdi[k] : tdj[k];
     };
-    const R ir2 = R(1) / r2;
+    const R ir2 = r_rcp<R>(r2);
     R f = 0;
-    R r = 0;
-    if ((kinds & kDh) || ((kinds & kLja) && G.lja_rsw > 0)) r = r_sqrt<R>(r2);
+    R r = 0, ir = 0;
+    if ((kinds & kDh) || ((kinds & kLja) && G.lja_rsw > 0)) {
+        ir = r_rsqrt<R>(r2);
+        r = r2 * ir;
+    }
     if (kinds & kExv) {
         const R s = P(true, 0) + P(false, 0);
         const R s2 = s * s;
@@ -732,15 +744,18 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
         const R il = static_cast<R>(G.dh_inv_lambda);
         const R ex = r_exp<R>(-r * il);
         const R K = static_cast<R>(G.dh_K);
-        const R ir = R(1) / r;
         f += -qq * K * ex * (ir + il) * ir2;
         e[1] = qq * K * (ex * ir - static_cast<R>(G.dh_corr));
     }
     const R c13 = static_cast<R>(1.2599210498948731648);  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
+    R s6 = 0, s12 = 0, sb_r = -1;
     if (kinds & kLjr) {
         const R eb = R(0.5) * (P(true, 2) + P(false, 2));
         const R sb = R(0.5) * (P(true, 3) + P(false, 3));
-        const R s2 = sb * sb * ir2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        const R s2 = sb * sb * ir2;
+        s6 = s2 * s2 * s2;
+        s12 = s6 * s6;
+        sb_r = sb;
         if (c13 * sb * sb - r2 >= R(0)) {
             f += eb * (R(-48) * s12 + R(24) * s6) * ir2;
             e[2] = eb * (R(4) * (s12 - s6) + R(1));
@@ -751,7 +766,11 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
     if (kinds & kLja) {
         const R eb = R(0.5) * (P(true, 4) + P(false, 4));
         const R sb = R(0.5) * (P(true, 5) + P(false, 5));
-        const R s2 = sb * sb * ir2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        if (sb != sb_r) {  // Ashbaugh-Hatch pairs share sigma with the repulsive term
+            const R s2 = sb * sb * ir2;
+            s6 = s2 * s2 * s2;
+            s12 = s6 * s6;
+        }
         const R th2 = c13 * sb * sb;
         R u = 0, du = 0;  // du = (dU/dr)/r
         if (r2 - th2 >= R(0)) {
@@ -762,12 +781,12 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
         if (G.lja_rsw > 0) {
             const R rsw = static_cast<R>(G.lja_rsw);
             if (r > rsw) {
-                const R inv = R(1) / (static_cast<R>(G.rc) - rsw);
+                const R inv = static_cast<R>(G.lja_inv_sw);
                 const R x = (r - rsw) * inv;
                 const R x2 = x * x, x3 = x2 * x;
                 const R S = R(1) - x3 * (R(10) - R(15) * x + R(6) * x2);
                 const R dS = R(-30) * x2 * (R(1) - x) * (R(1) - x) * inv;
-                du = du * S + u * dS / r;
+                du = du * S + u * dS * ir;
                 u = u * S;
             }
         }
