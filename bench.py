@@ -67,6 +67,51 @@ def cpu_baseline(system, integ, x, v, dt_ps, budget_s=10.0):
                       f"(oracle/oracle.cpp oracle_baoab_cells), OpenMP {threads} threads"}
 
 
+def replicas_bench(args):
+    """256 Go chains (50-500 beads, 280-400 K ladder) packed in one batch vs the
+    same replicas stepped one at a time through single-system contexts."""
+    from opencafemol_amd import abi
+    from opencafemol_amd.context import Batch, Context
+    systems, integs, xs, cfg = build_workload("replicas")
+    kB = 1.380649e-23 * 6.02214076e23 / 1000.0
+    rng = np.random.default_rng(5)
+    vs = [rng.normal(size=(s.n, 3)) * np.sqrt(kB * it.temperature / s.masses)[:, None] for s, it in zip(systems, integs)]
+    b = Batch(systems, integs, use_graph=0 if args.no_graph else 1)
+    for k in range(len(systems)):
+        b.set_positions(k, xs[k])
+        b.set_velocities(k, vs[k])
+    code = b.step(args.warmup)
+    if code != abi.OCM_OK:
+        raise SystemExit(f"warmup failed with status {code}")
+    t0 = time.perf_counter()
+    code = b.step(args.steps)
+    el = time.perf_counter() - t0
+    if code != abi.OCM_OK:
+        raise SystemExit(f"timed run failed with status {code}")
+    dt_ps = integs[0].dt
+    agg = len(systems) * args.steps / el * dt_ps * 86400.0 / 1000.0
+    log(f"batched: {args.steps} steps of {len(systems)} replicas in {el:.3f} s -> {agg:.0f} ns/day aggregate")
+    b.close()
+    serial_steps = max(200, args.steps // 10)
+    tot = 0.0
+    for k in range(len(systems)):
+        c = Context(systems[k], integs[k], use_graph=0 if args.no_graph else 1)
+        c.set_positions(xs[k])
+        c.set_velocities(vs[k])
+        c.step(200)
+        t = time.perf_counter()
+        c.step(serial_steps)
+        tot += time.perf_counter() - t
+        c.close()
+    serial = len(systems) * serial_steps / tot * dt_ps * 86400.0 / 1000.0
+    log(f"serial: {len(systems)} replicas x {serial_steps} steps one at a time in {tot:.2f} s -> {serial:.0f} ns/day")
+    out = {"metric": "aggregate ns/day, 256 Go replicas (50-500 beads, 280-400 K)", "value": agg, "unit": "ns/day",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1000,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+           "data": "synthetic", "config": cfg, "serial_plain_ns_day": serial, "batched_over_serial": agg / serial}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +126,10 @@ def main():
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent systems shard "
               f"across GPUs without communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "replicas":
+        replicas_bench(args)
+        return
 
     from opencafemol_amd import abi
     from opencafemol_amd.context import Batch
