@@ -73,14 +73,16 @@ def fld_globals(key):
     return np.array(list(FLD[key]) + [_FLD_EMIN[key]])
 
 
-def fld_class(first, second):
-    """Residue-pair class of a dihedral whose central bond joins `first` and
-    `second` (ReadTOMLForceFieldGenerator.cpp:636-766)."""
+def fld_classes(first, second):
+    """Residue-pair classes a dihedral whose central bond joins `first` and
+    `second` is added to (ReadTOMLForceFieldGenerator.cpp:636-766): R1-GLY for
+    every X-GLY, R2-PRO for every X-PRO and additionally GLY-PRO for GLY-PRO,
+    otherwise the first residue's R3 class."""
     if second == "GLY":
-        return ("R1", "GLY")
+        return [("R1", "GLY")]
     if second == "PRO":
-        return ("GLY", "PRO") if first == "GLY" else ("R2", "PRO")
-    return (first, "R3")
+        return [("R2", "PRO"), ("GLY", "PRO")] if first == "GLY" else [("R2", "PRO")]
+    return [(first, "R3")]
 
 
 def flexible_local_angle_forces(triplets, residues, k_kcal, use_periodic=False):
@@ -100,10 +102,10 @@ def flexible_local_angle_forces(triplets, residues, k_kcal, use_periodic=False):
 
 def flexible_local_dihedral_forces(quads, residues, k_kcal, use_periodic=False):
     quads = np.asarray(quads, dtype=np.int32).reshape(-1, 4)
-    classes = [fld_class(residues[q[1]], residues[q[2]]) for q in quads]
+    classes = [fld_classes(residues[q[1]], residues[q[2]]) for q in quads]
     out = []
     for key in FLD:
-        sel = np.array([c == key for c in classes], dtype=bool)
+        sel = np.array([key in c for c in classes], dtype=bool)
         if not sel.any():
             continue
         out.append(ForceSpec(abi.FLEXIBLE_LOCAL_DIHEDRAL, f"FlexibleLocalDihedral ({key[0]}-{key[1]})",

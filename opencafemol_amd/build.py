@@ -52,6 +52,28 @@ def build_hip(verbose=False):
         print("built", os.path.join(LIB_DIR, "libocm.so"))
 
 
+HOST_SOURCES = ["toml.cpp", "forcefield.cpp", "aicg2_tables.cpp", "read_toml.cpp", "simulator.cpp", "host_capi.cpp"]
+HOST = os.path.join(HERE, "host")
+BIN_DIR = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
+ROCM_LIB = "/opt/rocm/lib"
+
+
+def build_host(verbose=False):
+    """Host front end (C++17, g++): libocm_host.so over libocm.so, and the
+    ocm_run command-line driver."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    lib = os.path.join(LIB_DIR, "libocm_host.so")
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
+          *[os.path.join(HOST, f) for f in HOST_SOURCES], "-L" + LIB_DIR, "-locm", "-L" + ROCM_LIB,
+          "-Wl,-rpath-link," + ROCM_LIB, "-Wl,-rpath,$ORIGIN", "-o", lib])
+    _run([CXX, "-O2", "-std=c++17", os.path.join(HOST, "main.cpp"), "-L" + LIB_DIR, "-locm_host", "-locm",
+          "-L" + ROCM_LIB, "-Wl,-rpath-link," + ROCM_LIB, "-Wl,-rpath,$ORIGIN/../lib",
+          "-o", os.path.join(BIN_DIR, "ocm_run")])
+    if verbose:
+        print("built", lib, "and", os.path.join(BIN_DIR, "ocm_run"))
+
+
 def build_oracle(verbose=False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     if verbose:
@@ -63,9 +85,12 @@ def main(argv):
         build_hip(True)
     elif "--oracle" in argv:
         build_oracle(True)
+    elif "--host" in argv:
+        build_host(True)
     else:
         build_oracle(True)
         build_hip(True)
+        build_host(True)
 
 
 if __name__ == "__main__":

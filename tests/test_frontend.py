@@ -1,0 +1,214 @@
+"""C++ front end (TOML readers -> generators -> SystemGenerator) against an
+independent Python construction of the same input, the reference's sample
+inputs when they are available, and the readers' error behaviour."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from opencafemol_amd import abi, host_abi
+from tests import sh3_system
+from tests.golden import sh3_toml
+
+REF_SAMPLES = "/root/reference/sample/toml_input"
+
+
+def _write(tmp_path, text, name="in.toml"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def _by_kind(spec):
+    out = {}
+    for f in spec.forces:
+        out.setdefault(f.kind, []).append(f)
+    return out
+
+
+def _bonded_rows(forces):
+    rows = []
+    for f in forces:
+        g = tuple(np.round(f.globals, 12)) if f.globals is not None else ()
+        for i, p in zip(f.indices, f.params):
+            rows.append((tuple(int(v) for v in i), tuple(np.round(p, 12)), g))
+    return sorted(rows)
+
+
+def test_sh3_toml_matches_python_construction(tmp_path, oracle):
+    path = sh3_toml.write(str(tmp_path / "sh3.toml"))
+    cpp, integ, x, v = host_abi.load_toml(path)
+    py, _ = sh3_system.build()
+    assert cpp.n == py.n == 56
+    np.testing.assert_allclose(cpp.masses, py.masses)
+    np.testing.assert_allclose(x, sh3_system.native_positions_nm(), rtol=0, atol=1e-12)
+    kc, kp = _by_kind(cpp), _by_kind(py)
+    for kind in (abi.HARMONIC_BOND, abi.GO_CONTACT, abi.GAUSSIAN_BOND, abi.GAUSSIAN_DIHEDRAL,
+                 abi.FLEXIBLE_LOCAL_ANGLE, abi.FLEXIBLE_LOCAL_DIHEDRAL):
+        assert _bonded_rows(kc[kind]) == _bonded_rows([f for f in kp[kind] if f.n_terms]), kind
+    assert len(kc[abi.FLEXIBLE_LOCAL_ANGLE]) == 20 and len(kc[abi.FLEXIBLE_LOCAL_DIHEDRAL]) == 23
+    (ec,), (ep,) = kc[abi.EXCLUDED_VOLUME], kp[abi.EXCLUDED_VOLUME]
+    np.testing.assert_allclose(ec.particle_params, ep.particle_params)
+    assert ec.cutoff == pytest.approx(ep.cutoff, rel=1e-14)
+    assert ec.globals[0] == pytest.approx(0.6 * 4.184)
+    assert {tuple(e) for e in ec.exclusions} == {tuple(e) for e in ep.exclusions}
+    assert cpp.group_ids == {"HarmonicBond": 0, "GoContact": 1, "GaussianBond": 2, "FlexibleLocalAngle": 3,
+                             "GaussianDihedral": 4, "FlexibleLocalDihedral": 5, "ExcludedVolume": 6}
+    # integrator: delta_t and gamma_t in cafetime units (ReadTOMLInput.cpp:877-906)
+    assert integ.dt == pytest.approx(0.01 * 0.0488882129, rel=1e-9)
+    assert integ.friction == pytest.approx(0.01 / 0.0488882129, rel=1e-9)
+    assert integ.seed == 12345
+    # same energies group by group through the oracle
+    xt = x + np.random.default_rng(0).normal(scale=0.01, size=x.shape)
+    Fc, Ec = oracle.evaluate(cpp, xt)
+    Fp, Ep = oracle.evaluate(py, xt)
+    np.testing.assert_allclose(Ec, Ep, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(Fc, Fp, rtol=1e-10, atol=1e-8)
+
+
+def test_velocities_read_in_cafetime_units(tmp_path):
+    path = sh3_toml.write(str(tmp_path / "sh3v.toml"), with_velocity=True)
+    _, _, _, v = host_abi.load_toml(path)
+    np.testing.assert_allclose(v[0], np.array([0.01, -0.02, 0.03]) * 0.1 / 0.0488882129, rtol=1e-9)
+
+
+HPS = """
+[files]
+output.prefix = "hps"
+output.format = "dcd"
+[simulator]
+boundary_type = "PeriodicCuboid"
+delta_t = 0.2
+total_step = 1000_000
+save_step = 1000
+[[systems]]
+boundary_shape.lower = [0.0, 0.0, 0.0]
+boundary_shape.upper = [80.0, 80.0, 200.0]
+attributes.ionic_strength = 0.1
+attributes.temperature = 300.0
+particles = [
+{m = 100.0, pos = [10.0, 10.0, 100.0], group = "A"},
+{m = 100.0, pos = [13.8, 10.0, 100.0], group = "A"},
+{m = 100.0, pos = [17.6, 10.0, 100.0], group = "B"},
+{m = 100.0, pos = [21.4, 10.0, 100.0]},
+]
+[[forcefields]]
+[[forcefields.local]]
+interaction = "BondLength"
+potential = "Harmonic"
+topology = "bond"
+env.k = 110.4
+parameters = [
+{indices = [0, 1], v0 = 3.8, k = "k"},
+{indices = [0, 1], offset = 1, v0 = 3.8, k = "k"},
+{indices = [0, 1], offset = [2, 2], v0 = 3.8, k = "k"},
+]
+[[forcefields.global]]
+interaction = "Pair"
+potential = "LennardJonesRepulsive"
+ignore.particles_within.bond = 1
+cutoff = 4.0
+parameters = [
+{index = 0, epsilon = 0.2, sigma = 6.0},
+{index = 1, epsilon = 0.2, sigma = 5.0},
+{index = 2, epsilon = 0.2, sigma = 7.0},
+{index = 0, offset = 3, epsilon = 0.2, sigma = 4.5},
+]
+[[forcefields.global]]
+interaction = "Pair"
+potential = "LennardJonesAttractive"
+ignore.particles_within.bond = 1
+ignore.group.inter = [["B", "A"]]
+cutoff = 4.0
+parameters = [
+{index = 0, epsilon = 0.1, sigma = 6.0},
+{index = 1, epsilon = 0.05, sigma = 5.0},
+{index = 2, epsilon = 0.0, sigma = 7.0},
+]
+[[forcefields.global]]
+interaction = "Pair"
+potential = "DebyeHuckel"
+ignore.particles_within.bond = 3
+parameters = [
+{index = 1, charge = 1.0},
+{index = 3, charge = -1.0},
+]
+"""
+
+
+def test_hps_input_conversions(tmp_path):
+    s, integ, x, v = host_abi.load_toml(_write(tmp_path, HPS))
+    assert s.box == pytest.approx((8.0, 8.0, 20.0))
+    kinds = _by_kind(s)
+    (hb,) = kinds[abi.HARMONIC_BOND]
+    assert hb.indices.tolist() == [[0, 1], [1, 2], [2, 3]]                       # scalar and array offsets
+    assert hb.params[:, 1] == pytest.approx(110.4 * 4.184 * 100 * 2)             # env alias, k doubled
+    (ljr,) = kinds[abi.LJ_REPULSIVE]
+    assert ljr.particle_params[3].tolist() == pytest.approx([0.2 * 4.184, 0.45])
+    assert ljr.cutoff == pytest.approx(4.0 * (0.7 + 0.6) / 2)                    # ratio x mean of two largest
+    assert ljr.exclusions.tolist() == [[0, 1], [1, 2], [2, 3]]
+    (lja,) = kinds[abi.LJ_ATTRACTIVE]
+    assert lja.switch_distance == pytest.approx(0.9 * lja.cutoff)
+    # groups named in ignore.group: others = 0, A = 1, B = 2; the pair is matched in either order
+    assert lja.category.tolist() == [1, 1, 2, -1]
+    assert lja.allow.tolist() == [[1, 1, 1], [1, 1, 0], [1, 0, 1]]
+    (dh,) = kinds[abi.DEBYE_HUCKEL]
+    assert dh.category.tolist() == [-1, 0, -1, 0]
+    assert dh.globals[1] == pytest.approx(0.9486, abs=2e-4)                        # Debye length at 300 K, 0.1 M
+    assert dh.cutoff == pytest.approx(5.5 * dh.globals[1])
+    assert dh.exclusions.tolist() == [[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]]
+    assert integ.dt == pytest.approx(0.2 * 0.0488882129, rel=1e-9)
+
+
+@pytest.mark.parametrize("mutate,needle", [
+    (lambda t: t.replace("{indices = [0, 1], v0", "{indices = [0, 9], v0"), "exceeds the system's largest index"),
+    (lambda t: t.replace('attributes.ionic_strength = 0.1\n', ''), "ionic_strength"),
+    (lambda t: t.replace('potential = "DebyeHuckel"', 'potential = "iSoLFAttractive"'), "outside this build"),
+    (lambda t: t.replace('ignore.particles_within.bond = 3', 'ignore.molecule = "Others"'), "Others"),
+    (lambda t: t.replace('attributes.temperature = 300.0', 'attributes.temperature = 300.0\nattributes.ensemble.type = "NPT"'), "barostat"),
+    (lambda t: t.replace('k = "k"}', 'k = "kk"}'), "named variable"),
+    (lambda t: t.replace('offset = [2, 2]', 'offset = [2, 2, 2]'), "offset"),
+])
+def test_reader_errors(tmp_path, mutate, needle):
+    with pytest.raises(abi.OcmError) as e:
+        host_abi.load_toml(_write(tmp_path, mutate(HPS)))
+    assert needle in str(e.value)
+
+
+def test_toml_syntax_and_include(tmp_path):
+    inc = _write(tmp_path, '[[forcefields]]\n[[forcefields.local]]\ninteraction = "BondLength"\n'
+                           'potential = "Harmonic"\nparameters = [{indices = [0, 1], v0 = 3.8, k = 1.0}]\n', "ff.toml")
+    main = ('include = "%s"\n[files]\noutput = {prefix = "x", format = "pdb"}\n'
+            '[simulator]\nboundary_type = "Unlimited" # comment\ndelta_t = 1e-1\ntotal_step = 1_000\nsave_step = 10\n'
+            "[[systems]]\nattributes.temperature = 3.0e2\nparticles = [\n  {m = 1, pos = [0, 0, 0]},  # first\n"
+            "  {mass = 2.5, position = [1.0, 0, 0]},\n]\n") % inc
+    s, integ, x, v = host_abi.load_toml(_write(tmp_path, main))
+    assert s.masses.tolist() == [1.0, 2.5]
+    assert s.forces[0].params[0, 0] == pytest.approx(0.38)
+    assert integ.temperature == 300.0
+
+
+def test_suffix_check(tmp_path):
+    with pytest.raises(abi.OcmError) as e:
+        host_abi.load_toml(_write(tmp_path, "x = 1", "in.txt"))
+    assert "suffix" in str(e.value)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SAMPLES), reason="reference samples not present")
+def test_reference_sample_inputs_parse():
+    s, integ, x, v = host_abi.load_toml(os.path.join(REF_SAMPLES, "sh3_AICG2+.toml"))
+    assert s.n == 56 and len(s.forces) == 48 and v is not None
+    exv = [f for f in s.forces if f.kind == abi.EXCLUDED_VOLUME][0]
+    assert exv.cutoff == pytest.approx(2.0 * (0.2348 + 0.2265), rel=1e-12)
+    s, integ, x, v = host_abi.load_toml(os.path.join(REF_SAMPLES, "FUS_100_HPS.toml"))
+    assert s.n == 16300 and s.box is not None
+    ljr = [f for f in s.forces if f.kind == abi.LJ_REPULSIVE][0]
+    sig = np.sort(ljr.particle_params[:, 1])
+    assert ljr.cutoff == pytest.approx(6.19 * (sig[-1] + sig[-2]) / 2, rel=1e-12)
+
+
+def test_cli_rejects_bad_usage():
+    exe = os.path.join(os.path.dirname(host_abi.LIB_PATH), "..", "bin", "ocm_run")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr

@@ -1,0 +1,70 @@
+"""C++ front end on the GPU: the ocm_run driver end to end, and forces of
+a TOML-built system against the oracle."""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from opencafemol_amd import abi, host_abi
+from opencafemol_amd.context import Context
+from tests.golden import sh3_toml
+from tests.tolerances import assert_energies_close, assert_forces_close
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(host_abi.LIB_PATH), "..", "bin", "ocm_run")
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    sys.__stderr__.write(f"\n[gpu-case] {request.node.nodeid}\n")
+    sys.__stderr__.flush()
+    yield
+
+
+def test_toml_system_forces_match_oracle(tmp_path, oracle):
+    path = sh3_toml.write(str(tmp_path / "sh3.toml"))
+    s, it, x, v = host_abi.load_toml(path)
+    xt = x + np.random.default_rng(3).normal(scale=0.02, size=x.shape)
+    with Context(s, it) as c:
+        c.set_positions(xt)
+        st = c.get_state(abi.STATE_FORCES | abi.STATE_ENERGY)
+    Fo, Eo = oracle.evaluate(s, xt)
+    assert_forces_close(st["forces"], Fo)
+    assert_energies_close(st["energy_by_group"], Eo)
+
+
+def test_cli_runs_sh3_and_writes_pdb_and_energy(tmp_path):
+    path = sh3_toml.write(str(tmp_path / "sh3.toml"), total_step=500, save_step=100, with_velocity=True)
+    r = subprocess.run([EXE, path], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pdb = (tmp_path / "sh3.pdb").read_text()
+    assert pdb.count("MODEL") == 5 and pdb.count("ATOM") == 5 * 56
+    ene = (tmp_path / "sh3.ene").read_text().splitlines()
+    assert ene[0] == "# unit of energy : kcal/mol"
+    cols = ene[1].split()
+    assert cols[:2] == ["#", "timestep"] and cols[-1] == "kinetic_energy"
+    rows = [list(map(float, l.split())) for l in ene[2:]]
+    assert len(rows) == 5 and all(np.isfinite(r).all() for r in rows)
+    assert rows[0][0] == 0 and rows[-1][0] == 400
+
+
+def test_cli_writes_dcd_for_periodic_input(tmp_path):
+    from tests.test_frontend import HPS
+    text = HPS.replace("total_step = 1000_000", "total_step = 300").replace("save_step = 1000", "save_step = 100")
+    p = tmp_path / "hps.toml"
+    p.write_text(text)
+    r = subprocess.run([EXE, str(p)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    data = (tmp_path / "hps_position.dcd").read_bytes()
+    assert struct.unpack("<i4s", data[:8]) == (84, b"CORD")
+    nset, first, interval, nsteps = struct.unpack("<4i", data[8:24])
+    assert (nset, interval, nsteps) == (3, 100, 300)
+    natom = struct.unpack("<i", data[8 + 84 + 8 + 84 + 4:8 + 84 + 8 + 84 + 8])[0]
+    assert natom == 4
+    header = 4 + 84 + 4 + 4 + 84 + 4 + 4 + 4 + 4
+    frame = (4 + 48 + 4) + 3 * (4 + 4 * natom + 4)
+    assert len(data) == header + 3 * frame
+    assert (tmp_path / "hps_velocity.dcd").stat().st_size == header + 3 * (3 * (4 + 4 * natom + 4))

@@ -47,12 +47,12 @@ def test_min_energies_match_oracle(oracle):
 
 
 @pytest.mark.parametrize("first,second,expect", [
-    ("ALA", "GLY", ("R1", "GLY")), ("GLY", "GLY", ("R1", "GLY")), ("GLY", "PRO", ("GLY", "PRO")),
-    ("ALA", "PRO", ("R2", "PRO")), ("VAL", "ALA", ("VAL", "R3")), ("GLY", "ALA", ("GLY", "R3")),
+    ("ALA", "GLY", [("R1", "GLY")]), ("GLY", "GLY", [("R1", "GLY")]), ("GLY", "PRO", [("R2", "PRO"), ("GLY", "PRO")]),
+    ("ALA", "PRO", [("R2", "PRO")]), ("VAL", "ALA", [("VAL", "R3")]), ("GLY", "ALA", [("GLY", "R3")]),
 ])
 def test_fld_class_selection(first, second, expect):
-    # ReadTOMLForceFieldGenerator.cpp:636-766
-    assert aicg2.fld_class(first, second) == expect
+    # ReadTOMLForceFieldGenerator.cpp:636-766 (R2-PRO also takes GLY-PRO)
+    assert aicg2.fld_classes(first, second) == expect
 
 
 def test_group_ids_follow_system_generator():
@@ -93,7 +93,7 @@ def _interaction_groups(params_present, group_of, ignore_pairs):
     for a_i, a in enumerate(names):
         out.append((groups[a], others))
         for b in names[a_i:]:
-            if (a, b) not in ignore_pairs:
+            if (a, b) not in ignore_pairs and (b, a) not in ignore_pairs:  # Utility::contains, either order
                 out.append((groups[a], groups[b]))
     return out
 
@@ -120,7 +120,7 @@ def test_category_allow_matrix_equals_interaction_groups(seed):
     present = rng.random(n) < 0.8
     names = ["A", "B", "C"]
     group_of = [names[k] if k < 3 else None for k in rng.integers(0, 5, n)]
-    ignore = {("A", "A"), ("A", "C")} if seed % 2 else set()
+    ignore = {("A", "A"), ("C", "A")} if seed % 2 else set()
     sets = _interaction_groups(list(present), group_of, ignore)
     want = _pairs_from_sets(sets, n)
     # category encoding: 0 = others, 1.. = named groups that appear in ignores
