@@ -84,7 +84,7 @@ total_step = 1000_000
 save_step = 1000
 [[systems]]
 boundary_shape.lower = [0.0, 0.0, 0.0]
-boundary_shape.upper = [80.0, 80.0, 200.0]
+boundary_shape.upper = [120.0, 120.0, 200.0]
 attributes.ionic_strength = 0.1
 attributes.temperature = 300.0
 particles = [
@@ -139,7 +139,7 @@ parameters = [
 
 def test_hps_input_conversions(tmp_path):
     s, integ, x, v = host_abi.load_toml(_write(tmp_path, HPS))
-    assert s.box == pytest.approx((8.0, 8.0, 20.0))
+    assert s.box == pytest.approx((12.0, 12.0, 20.0))
     kinds = _by_kind(s)
     (hb,) = kinds[abi.HARMONIC_BOND]
     assert hb.indices.tolist() == [[0, 1], [1, 2], [2, 3]]                       # scalar and array offsets
