@@ -1,7 +1,7 @@
 """MD-step throughput on one MI355X: ns/day of the HPS slab (100 chains x 140
 beads = 14,000 beads, periodic 15 x 15 x 150 nm, dt 10 fs, 300 K).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hps|go|replicas]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hps|go|replicas|serve]
 
 Protocol: build the seeded slab, relax it on the device (ocm_minimize), draw
 Maxwell-Boltzmann velocities, run W untimed steps, then time exactly K steps
@@ -112,6 +112,136 @@ def replicas_bench(args):
     print(json.dumps(out))
 
 
+def _pct(v, q):
+    """Nearest-rank percentile (as the server reports)."""
+    v = sorted(v)
+    if not v:
+        return 0.0
+    k = int(np.ceil(q * len(v)))
+    return v[min(len(v) - 1, max(k, 1) - 1)]
+
+
+def serve_bench(args):
+    """Served job stream: 60% Go chains (50-500 beads), 40% HPS slabs
+    (2,000-20,000 beads), one frame per 10,000 steps.  Three runs of the same
+    jobs: (1) serial plain execution, one context per job, back to back;
+    (2) the server with every job submitted at once (capacity); (3) the server
+    fed Poisson arrivals at `--serve-load` times the serial service rate, with
+    job latency (submit -> completion) against a FIFO of the serial run's
+    measured job times on the same arrivals."""
+    from opencafemol_amd import abi, configs
+    from opencafemol_amd.context import Batch, Context, Server
+    lo, hi = (int(v) for v in args.serve_steps.split(","))
+    t = time.perf_counter()
+    jobs = configs.served_job_mix(args.serve_jobs, seed=args.seed, steps=(lo, hi))
+    hps = [j for j in jobs if j["kind"] == "hps"]
+    log(f"{len(jobs)} jobs ({len(hps)} HPS, {len(jobs) - len(hps)} Go), "
+        f"{sum(j['system'].n for j in jobs)} beads, built in {time.perf_counter() - t:.1f} s")
+    if hps:   # pre-relax the packed slabs in one batch
+        t = time.perf_counter()
+        with Batch([j["system"] for j in hps], [j["integrator"] for j in hps]) as b:
+            for k, j in enumerate(hps):
+                b.set_positions(k, j["positions"])
+            b.minimize(tolerance=10.0, max_iterations=args.minimize_iters)
+            for k, j in enumerate(hps):
+                j["positions"] = b.state(k, abi.STATE_POSITIONS)["positions"]
+        log(f"relaxed {len(hps)} slabs in {time.perf_counter() - t:.1f} s")
+    sim_ns = [j["nsteps"] * j["integrator"].dt * 1e-3 for j in jobs]
+    opt = {"use_graph": 0 if args.no_graph else 1}
+
+    def submit(srv, j):
+        return srv.submit(j["system"], j["integrator"], j["positions"], None, nsteps=j["nsteps"],
+                          frame_interval=j["frame_interval"])
+
+    # warm-up: library load, first kernels
+    with Server(**opt) as srv:
+        srv.drain([submit(srv, dict(jobs[0], nsteps=jobs[0]["frame_interval"]))])
+
+    # (1) serial plain execution
+    serial_t, serial_x, n_frames = [], [], 0
+    for j in jobs:
+        t = time.perf_counter()
+        with Context(j["system"], j["integrator"], **opt) as c:
+            c.set_positions(j["positions"])
+            c.set_velocities_to_temperature(j["integrator"].temperature, j["integrator"].seed)
+            done = 0
+            while done < j["nsteps"]:
+                if c.step(j["frame_interval"]) != abi.OCM_OK:
+                    raise SystemExit("serial run failed")
+                done += j["frame_interval"]
+                c.get_state(abi.STATE_POSITIONS)
+                n_frames += 1
+            serial_x.append(c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES))
+        serial_t.append(time.perf_counter() - t)
+    serial_total = sum(serial_t)
+    serial_ns_day = sum(sim_ns) / serial_total * 86400.0
+    log(f"serial: {serial_total:.2f} s -> {serial_ns_day:.0f} ns/day aggregate")
+
+    # (2) capacity: everything submitted at once
+    with Server(**opt) as srv:
+        t = time.perf_counter()
+        ids = [submit(srv, j) for j in jobs]
+        res = srv.drain(ids)
+        sat_s = time.perf_counter() - t
+        sat_stats = srv.stats()
+    identical = all(np.array_equal(res[i]["positions"], sx["positions"]) and
+                    np.array_equal(res[i]["velocities"], sx["velocities"]) for i, sx in zip(ids, serial_x))
+    sat_ns_day = sum(sim_ns) / sat_s * 86400.0
+    log(f"served, all at once: {sat_s:.2f} s -> {sat_ns_day:.0f} ns/day aggregate "
+        f"({sat_stats['repacks']} batches, identical to serial: {identical})")
+
+    # (3) Poisson stream
+    rng = np.random.default_rng(args.seed + 1)
+    rate = args.serve_load * len(jobs) / serial_total       # jobs per second
+    arrivals = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, len(jobs) - 1))])
+    order = rng.permutation(len(jobs))
+    with Server(**opt) as srv:
+        t0 = time.perf_counter()
+        ids, pending, results, k = [], set(), {}, 0
+        while k < len(jobs) or pending:
+            now = time.perf_counter() - t0
+            while k < len(jobs) and arrivals[k] <= now:
+                jid = submit(srv, jobs[order[k]])
+                ids.append(jid)
+                pending.add(jid)
+                k += 1
+            wait_ms = 20 if k >= len(jobs) else max(0, min(20, int((arrivals[k] - now) * 1000)))
+            if srv.next_frame(timeout_ms=wait_ms) is None:
+                for jid in list(pending):
+                    r = srv.wait(jid, timeout_ms=0)
+                    if r is not None:
+                        results[jid] = r
+                        pending.discard(jid)
+        stream_s = time.perf_counter() - t0
+        st = srv.stats()
+    lat = [(results[i]["end_ms"] - results[i]["submit_ms"]) / 1000.0 for i in ids]
+    fifo_end, fifo_lat = 0.0, []
+    for k2, jidx in enumerate(order):
+        start = max(arrivals[k2], fifo_end)
+        fifo_end = start + serial_t[jidx]
+        fifo_lat.append(fifo_end - arrivals[k2])
+    stream = {"arrival_rate_per_s": rate, "load_vs_serial": args.serve_load,
+              "latency_p50_s": _pct(lat, 0.5), "latency_p99_s": _pct(lat, 0.99),
+              "serial_fifo_latency_p50_s": _pct(fifo_lat, 0.5), "serial_fifo_latency_p99_s": _pct(fifo_lat, 0.99),
+              "wait_p50_s": st["wait_p50_ms"] / 1000.0, "wait_p99_s": st["wait_p99_ms"] / 1000.0,
+              "makespan_s": stream_s, "serial_fifo_makespan_s": fifo_end,
+              "aggregate_ns_day": sum(sim_ns) / stream_s * 86400.0, "batches": st["repacks"]}
+    log(f"stream: p50 {stream['latency_p50_s']:.2f} s, p99 {stream['latency_p99_s']:.2f} s "
+        f"(serial FIFO p50 {stream['serial_fifo_latency_p50_s']:.2f} s, p99 {stream['serial_fifo_latency_p99_s']:.2f} s)")
+    cfg = {"workload": "served_job_stream", "jobs": len(jobs), "hps_jobs": len(hps),
+           "go_beads": [50, 500], "hps_beads": [2000, 20000], "steps_per_job": [lo, hi], "frame_interval": 10000,
+           "frames": n_frames, "total_beads": int(sum(j["system"].n for j in jobs)), "dt_ps": 0.01,
+           "temperature_K": 300.0}
+    out = {"metric": "served job stream: aggregate ns/day (all jobs submitted at once) and p50/p99 job latency "
+                     "(Poisson arrivals)", "value": sat_ns_day, "unit": "ns/day", "n_gpus": 1,
+           "steps": int(sum(j["nsteps"] for j in jobs)), "warmup": 0, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+           "data": "synthetic", "config": cfg, "serial_plain_ns_day": serial_ns_day,
+           "served_over_serial": sat_ns_day / serial_ns_day, "identical_to_serial": bool(identical),
+           "stream": stream}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,14 +251,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--minimize-iters", type=int, default=3000)
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for profilers)")
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-steps", default="10000,50000", help="per-job step range (multiples of 10000)")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="arrival rate / serial service rate")
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent systems shard "
-              f"across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported; this build measures one "
+              f"GPU's share (independent systems and jobs would shard across GPUs without communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "replicas":
         replicas_bench(args)
+        return
+    if args.workload == "serve":
+        serve_bench(args)
         return
 
     from opencafemol_amd import abi

@@ -39,6 +39,7 @@ extern "C" {
 #define OCM_ERR_NUMERIC      4  /* NaN or fixed-point force overflow detected  */
 #define OCM_ERR_NO_DEVICE    5  /* no usable gfx950 device                     */
 #define OCM_ERR_STATE        6  /* call not valid in the current state         */
+#define OCM_ERR_TIMEOUT      7  /* nothing ready within the given time          */
 
 /* ----------------------------------------------------------- force kinds */
 /* Bonded kinds: `indices` holds n_terms * arity bead indices and `params`
@@ -227,6 +228,80 @@ int ocm_batch_get_state(ocm_context* ctx, int32_t system, int32_t what,
 /* Per-system error status after the last ocm_step (OCM_OK or OCM_ERR_NUMERIC). */
 int ocm_batch_system_status(ocm_context* ctx, int32_t system);
 int32_t ocm_batch_size(ocm_context* ctx);
+
+/* ----------------------------------------------------------------- server */
+/* A request loop over batches: jobs (one system each) are admitted as job
+ * slots and the device-memory budget allow, packed into one batch with the
+ * jobs already running and advanced together; the batch is repacked whenever
+ * a job enters or leaves.  Frames are copied out asynchronously and queued.
+ * A job's trajectory and result are the same bits as when it runs alone.
+ * No counterpart in the reference, which runs one system per process
+ * (src/OpenAICG2Plus.cpp:12-93); each job is that run's MD loop
+ * (src/Simulator.cpp:49-76) without its file output.
+ * Thread safety: submit, next_frame, wait and get_stats may be called from
+ * any host threads; one worker thread owns the device.                      */
+typedef struct ocm_server ocm_server;
+
+typedef struct ocm_server_options {
+    ocm_options engine;          /* options of each packed batch             */
+    int64_t hbm_budget_bytes;    /* device bytes for admitted jobs, by estimate;
+                                    0 = 80% of the free memory at creation    */
+    int32_t max_active_jobs;     /* job slots; 0 = 256                        */
+    int32_t max_queued_frames;   /* uncollected frames before the worker waits;
+                                    0 = 4096.  Callers must drain frames.     */
+    int64_t quantum_steps;       /* longest stretch between admission checks;
+                                    0 = 2000                                  */
+} ocm_server_options;
+
+typedef struct ocm_server_stats {
+    int64_t submitted, completed, failed, active, queued;
+    int64_t repacks;             /* batches built                             */
+    int64_t frames;              /* frames snapshotted                        */
+    double  latency_p50_ms;      /* submit -> completion, completed jobs      */
+    double  latency_p99_ms;
+    double  wait_p50_ms;         /* submit -> admission                       */
+    double  wait_p99_ms;
+    double  simulated_ns;        /* sum over ended jobs of steps * dt         */
+    double  busy_ms;             /* worker time spent stepping or repacking   */
+    double  repack_ms;           /* of which repacking                        */
+    double  makespan_ms;         /* first submission -> last completion       */
+    int64_t device_bytes;        /* current batch allocation                  */
+    int64_t budget_bytes;
+} ocm_server_stats;
+
+typedef struct ocm_job_info {
+    int32_t status;              /* OCM_OK, OCM_ERR_NUMERIC, ...              */
+    int32_t n_particles;
+    int64_t steps_done;
+    double  submit_ms, start_ms, end_ms;   /* since server creation          */
+} ocm_job_info;
+
+void ocm_server_options_default(ocm_server_options* opt);
+int ocm_server_create(const ocm_server_options* options, ocm_server** out);
+/* Stops the worker (running jobs are abandoned) and frees everything.      */
+int ocm_server_destroy(ocm_server* srv);
+/* Queues a job: n particles at `positions` (n*3 nm); `velocities` n*3 nm/ps,
+ * or NULL for Maxwell-Boltzmann velocities at the integrator temperature
+ * from its seed.  Runs nsteps steps and queues a frame (positions) every
+ * frame_interval steps (0 = none).  OCM_ERR_OOM if the job alone exceeds
+ * the budget.                                                              */
+int ocm_server_submit(ocm_server* srv, const ocm_system_desc* system,
+                      const ocm_integrator_desc* integrator,
+                      const double* positions, const double* velocities,
+                      int32_t n, int64_t nsteps, int64_t frame_interval,
+                      int64_t* job_id);
+/* Pops the oldest frame into xyz (cap particles).  timeout_ms < 0 waits
+ * without limit.  OCM_ERR_TIMEOUT if none arrived; OCM_ERR_INVALID if cap is
+ * too small (*n gets the size and the frame stays queued).                 */
+int ocm_server_next_frame(ocm_server* srv, int32_t timeout_ms, int64_t* job,
+                          int64_t* step, double* xyz, int32_t cap, int32_t* n);
+/* Waits for a job to end and all of its frames to be queued, then copies its
+ * final positions and velocities (either may be NULL; cap particles) and
+ * forgets the job.  OCM_ERR_TIMEOUT if it is still running.               */
+int ocm_server_wait(ocm_server* srv, int64_t job, int32_t timeout_ms,
+                    double* positions, double* velocities, int32_t cap,
+                    ocm_job_info* info);
+int ocm_server_get_stats(ocm_server* srv, ocm_server_stats* out);
 
 const char* ocm_last_error(void);
 const char* ocm_version(void);

@@ -16,6 +16,7 @@ OCM_ERR_HIP = 3
 OCM_ERR_NUMERIC = 4
 OCM_ERR_NO_DEVICE = 5
 OCM_ERR_STATE = 6
+OCM_ERR_TIMEOUT = 7
 
 HARMONIC_BOND = 1
 GAUSSIAN_BOND = 2
@@ -45,7 +46,9 @@ EXPORTED = [
     "ocm_get_neighbor_pairs", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
     "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
     "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
-    "ocm_version", "ocm_minimize",
+    "ocm_version", "ocm_minimize", "ocm_server_options_default", "ocm_server_create",
+    "ocm_server_destroy", "ocm_server_submit", "ocm_server_next_frame", "ocm_server_wait",
+    "ocm_server_get_stats",
 ]
 
 
@@ -86,6 +89,25 @@ class State(C.Structure):
                 ("forces", C.POINTER(C.c_double)), ("energy_by_group", C.c_double * 32),
                 ("potential_energy", C.c_double), ("kinetic_energy", C.c_double),
                 ("step", C.c_int64)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("engine", Options), ("hbm_budget_bytes", C.c_int64), ("max_active_jobs", C.c_int32),
+                ("max_queued_frames", C.c_int32), ("quantum_steps", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("active", C.c_int64), ("queued", C.c_int64), ("repacks", C.c_int64),
+                ("frames", C.c_int64), ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("wait_p50_ms", C.c_double), ("wait_p99_ms", C.c_double), ("simulated_ns", C.c_double),
+                ("busy_ms", C.c_double), ("repack_ms", C.c_double), ("makespan_ms", C.c_double),
+                ("device_bytes", C.c_int64), ("budget_bytes", C.c_int64)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_particles", C.c_int32), ("steps_done", C.c_int64),
+                ("submit_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double)]
 
 
 class OcmError(RuntimeError):
@@ -129,6 +151,16 @@ def load_library(path=LIB_PATH):
         "ocm_last_error": (C.c_char_p, []),
         "ocm_version": (C.c_char_p, []),
         "ocm_minimize": (C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+        "ocm_server_options_default": (None, [P(ServerOptions)]),
+        "ocm_server_create": (C.c_int, [P(ServerOptions), P(C.c_void_p)]),
+        "ocm_server_destroy": (C.c_int, [C.c_void_p]),
+        "ocm_server_submit": (C.c_int, [C.c_void_p, P(SystemDesc), P(IntegratorDesc), P(C.c_double),
+                                        P(C.c_double), C.c_int32, C.c_int64, C.c_int64, P(C.c_int64)]),
+        "ocm_server_next_frame": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_double),
+                                            C.c_int32, P(C.c_int32)]),
+        "ocm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, P(C.c_double), P(C.c_double),
+                                      C.c_int32, P(JobInfo)]),
+        "ocm_server_get_stats": (C.c_int, [C.c_void_p, P(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
-HIP_SOURCES = ["kernels.hip", "engine.cpp", "capi.cpp"]
+HIP_SOURCES = ["kernels.hip", "engine.cpp", "server.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled for every device object.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -46,7 +46,7 @@ def build_hip(verbose=False):
             if p.returncode != 0:
                 raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
         tmp_lib = os.path.join(tmp, "libocm.so")
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp_lib, *objs])
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-pthread", "-o", tmp_lib, *objs])
         shutil.move(tmp_lib, os.path.join(LIB_DIR, "libocm.so"))
     if verbose:
         print("built", os.path.join(LIB_DIR, "libocm.so"))

@@ -190,6 +190,7 @@ def lattice_chains(n_chains, chain_len, box, slab_z, rng, spacing=0.5):
                     break
     out[:, 0] = np.array(starts)
     length = np.ones(n_chains, dtype=np.int64)
+    stalls = np.zeros(n_chains, dtype=np.int64)
     while True:
         grow = np.nonzero(length < chain_len)[0]
         if len(grow) == 0:
@@ -209,12 +210,30 @@ def lattice_chains(n_chains, chain_len, box, slab_z, rng, spacing=0.5):
                 length[c] += 1
                 placed = True
                 break
-            if not placed:  # dead end: release the last 5 beads and regrow
-                back = min(5, length[c] - 1)
-                for b in range(back):
-                    p = out[c, length[c] - 1]
+            if placed:
+                continue
+            stalls[c] += 1
+            if stalls[c] > 100:
+                # trapped in a pocket shorter than the chain (the 14,000-bead
+                # slab sees at most 6 dead ends per chain): restart it from a free site
+                for b in range(length[c]):
+                    p = out[c, b]
                     occ[p[0], p[1], p[2]] = False
-                    length[c] -= 1
+                cand = free_sites(1)
+                while len(cand) == 0:
+                    cand = free_sites(1)
+                start = cand[0]
+                occ[start[0], start[1], start[2]] = True
+                out[c, 0] = start
+                length[c] = 1
+                stalls[c] = 0
+                continue
+            # dead end: release the last 5 beads and regrow
+            back = min(5, length[c] - 1)
+            for b in range(back):
+                p = out[c, length[c] - 1]
+                occ[p[0], p[1], p[2]] = False
+                length[c] -= 1
     del active
     z0 = 0.5 * (box[2] - nz * spacing)
     pos = out.reshape(-1, 3).astype(np.float64) * spacing + np.array([0.25 * spacing, 0.25 * spacing, z0])
@@ -280,6 +299,35 @@ def go_replicas(n_replicas=256, n_min=50, n_max=500, seed=0):
     for r in range(n_replicas):
         out.append(go_chain(int(sizes[r]), seed=seed * 100003 + r, temperature=temps[r]))
     return out
+
+
+def hps_slab_of_size(n_beads, seed=0, chain_len=140):
+    """HPS slab of about n_beads beads (chains of chain_len) at the density of
+    the 14,000-bead slab: x-y extent scaled with the bead count (at least 8 nm,
+    so the 3.5 nm Debye-Hueckel cutoff stays under half the box), z 150 nm."""
+    n_chains = max(1, int(round(n_beads / chain_len)))
+    xy = max(8.0, 15.0 * float(np.sqrt(n_chains * chain_len / 14000.0)))
+    return hps_slab(n_chains=n_chains, chain_len=chain_len, box=(xy, xy, 150.0), slab_z=20.0, seed=seed)
+
+
+def served_job_mix(n_jobs=24, seed=0, steps=(10000, 50000), frame_interval=10000, go_fraction=0.6):
+    """Job stream: Go chains of 50-500 beads (fraction go_fraction) and HPS
+    slabs of 2,000-20,000 beads, each running a whole number of frame
+    intervals drawn uniformly from [steps[0], steps[1]]."""
+    rng = np.random.default_rng(seed)
+    fi = frame_interval
+    jobs = []
+    for k in range(n_jobs):
+        if rng.random() < go_fraction:
+            s, it, x = go_chain(int(rng.integers(50, 501)), seed=seed * 1000 + k)
+            kind = "go"
+        else:
+            s, it, x = hps_slab_of_size(int(rng.integers(2000, 20001)), seed=seed * 1000 + k)
+            kind = "hps"
+        nsteps = int(rng.integers(steps[0] // fi, steps[1] // fi + 1)) * fi
+        jobs.append({"kind": kind, "system": s, "integrator": it, "positions": x, "nsteps": nsteps,
+                     "frame_interval": fi})
+    return jobs
 
 
 def aicg2_chain(n=56, seed=0, temperature=300.0, periodic_box=None):

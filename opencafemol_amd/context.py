@@ -1,4 +1,4 @@
-"""Python handle over the C ABI: Context (one system) and Batch (many)."""
+"""Python handle over the C ABI: Context (one system), Batch (many), Server (job stream)."""
 import ctypes as C
 
 import numpy as np
@@ -135,3 +135,122 @@ class Context(Batch):
         check(self.lib.ocm_get_neighbor_pairs(self.h, force_index, out.ctypes.data_as(C.POINTER(C.c_int32)),
                                               cap, C.byref(cnt)))
         return out[:min(cap, cnt.value)]
+
+
+class Server:
+    """Continuous-batching job server (``ocm_server_*``).
+
+    ``submit`` queues a job and returns its id; ``next_frame`` pops a queued
+    trajectory frames; ``wait`` collects a job's final state.  The worker
+    stops when the frame queue is full, so a caller that waits on jobs must
+    also drain frames (``drain`` does both).
+    """
+
+    def __init__(self, hbm_budget_bytes=0, max_active_jobs=0, max_queued_frames=0, quantum_steps=0,
+                 **engine_options):
+        self.lib = abi.load_library()
+        o = abi.ServerOptions()
+        self.lib.ocm_server_options_default(C.byref(o))
+        for k, v in engine_options.items():
+            setattr(o.engine, k, v)
+        if hbm_budget_bytes:
+            o.hbm_budget_bytes = hbm_budget_bytes
+        if max_active_jobs:
+            o.max_active_jobs = max_active_jobs
+        if max_queued_frames:
+            o.max_queued_frames = max_queued_frames
+        if quantum_steps:
+            o.quantum_steps = quantum_steps
+        h = C.c_void_p()
+        check(self.lib.ocm_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self.n_of = {}
+        self.max_n = 1
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ocm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, system, integrator, positions, velocities=None, nsteps=1, frame_interval=0):
+        sd, keep = system.descriptor()
+        it = integrator.descriptor()
+        x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        vp = None
+        if velocities is not None:
+            v = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
+            vp = v.ctypes.data_as(C.POINTER(C.c_double))
+        jid = C.c_int64()
+        check(self.lib.ocm_server_submit(self.h, C.byref(sd), C.byref(it), x.ctypes.data_as(C.POINTER(C.c_double)),
+                                         vp, x.shape[0], int(nsteps), int(frame_interval), C.byref(jid)))
+        del keep
+        self.n_of[jid.value] = x.shape[0]
+        self.max_n = max(self.max_n, x.shape[0])
+        return jid.value
+
+    def next_frame(self, timeout_ms=-1):
+        """(job, step, positions) or None on timeout."""
+        cap = self.max_n
+        buf = np.zeros((cap, 3))
+        job, step, n = C.c_int64(), C.c_int64(), C.c_int32()
+        code = self.lib.ocm_server_next_frame(self.h, int(timeout_ms), C.byref(job), C.byref(step),
+                                              buf.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n))
+        if code == abi.OCM_ERR_TIMEOUT:
+            return None
+        check(code)
+        return job.value, step.value, buf[:n.value].copy()
+
+    def wait(self, job, timeout_ms=-1):
+        """{"status", "positions", "velocities", "steps", "submit_ms", "start_ms", "end_ms"} or None."""
+        n = self.n_of[job]
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        info = abi.JobInfo()
+        code = self.lib.ocm_server_wait(self.h, int(job), int(timeout_ms), x.ctypes.data_as(C.POINTER(C.c_double)),
+                                        v.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(info))
+        if code == abi.OCM_ERR_TIMEOUT:
+            return None
+        check(code)
+        del self.n_of[job]
+        return {"status": info.status, "positions": x, "velocities": v, "steps": info.steps_done,
+                "submit_ms": info.submit_ms, "start_ms": info.start_ms, "end_ms": info.end_ms}
+
+    def drain(self, jobs, on_frame=None):
+        """Collects frames until every job in ``jobs`` has ended; returns {job: result}."""
+        pending = set(jobs)
+        results = {}
+        while pending:
+            f = self.next_frame(timeout_ms=20)
+            if f is not None:
+                if on_frame is not None:
+                    on_frame(*f)
+                continue
+            for j in list(pending):
+                r = self.wait(j, timeout_ms=0)
+                if r is not None:
+                    results[j] = r
+                    pending.discard(j)
+        while True:
+            f = self.next_frame(timeout_ms=0)
+            if f is None:
+                break
+            if on_frame is not None:
+                on_frame(*f)
+        return results
+
+    def stats(self):
+        st = abi.ServerStats()
+        check(self.lib.ocm_server_get_stats(self.h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in abi.ServerStats._fields_}

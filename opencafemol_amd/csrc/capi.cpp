@@ -4,12 +4,17 @@
 // std::runtime_error into an exit status (src/OpenAICG2Plus.cpp:84-93).
 #include "../../include/ocm.h"
 #include "engine.hpp"
+#include "server.hpp"
 
 #include <new>
 #include <string>
 
 struct ocm_context {
     ocm::Engine* eng;
+};
+
+struct ocm_server {
+    ocm::Server* srv;
 };
 
 namespace {
@@ -43,6 +48,10 @@ void need(const void* p, const char* what) {
 }
 void check_sys(ocm::Engine& e, int s) {
     if (s < 0 || s >= e.n_systems()) throw ocm::Error(OCM_ERR_INVALID, "system index out of range");
+}
+ocm::Server& server(ocm_server* s) {
+    if (!s || !s->srv) throw ocm::Error(OCM_ERR_INVALID, "NULL server");
+    return *s->srv;
 }
 void check_n(ocm::Engine& e, int s, int32_t n) {
     if (n != e.n_particles(s)) throw ocm::Error(OCM_ERR_INVALID, "array length does not match the particle count");
@@ -214,6 +223,116 @@ int ocm_batch_system_status(ocm_context* ctx, int32_t s) {
 int32_t ocm_batch_size(ocm_context* ctx) {
     if (!ctx || !ctx->eng) return 0;
     return ctx->eng->n_systems();
+}
+
+void ocm_server_options_default(ocm_server_options* o) {
+    if (!o) return;
+    ocm_options_default(&o->engine);
+    o->hbm_budget_bytes = 0;
+    o->max_active_jobs = 256;
+    o->max_queued_frames = 4096;
+    o->quantum_steps = 2000;
+}
+
+int ocm_server_create(const ocm_server_options* options, ocm_server** out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        ocm_server_options opt;
+        ocm_server_options_default(&opt);
+        if (options) opt = *options;
+        auto* s = new ocm::Server(opt);
+        *out = new ocm_server{s};
+        return OCM_OK;
+    });
+}
+
+int ocm_server_destroy(ocm_server* srv) {
+    return guarded([&] {
+        if (!srv) return OCM_OK;
+        delete srv->srv;
+        delete srv;
+        return OCM_OK;
+    });
+}
+
+int ocm_server_submit(ocm_server* srv, const ocm_system_desc* system, const ocm_integrator_desc* integrator,
+                      const double* positions, const double* velocities, int32_t n, int64_t nsteps,
+                      int64_t frame_interval, int64_t* job_id) {
+    return guarded([&] {
+        auto& S = server(srv);
+        need(system, "system");
+        need(integrator, "integrator");
+        need(positions, "positions");
+        need(job_id, "job_id");
+        ocm::HostSystem hs = ocm::copy_system(*system, *integrator);
+        if (n != hs.n) throw ocm::Error(OCM_ERR_INVALID, "array length does not match the particle count");
+        std::vector<double> x(positions, positions + 3 * static_cast<size_t>(n));
+        std::vector<double> v;
+        if (velocities) v.assign(velocities, velocities + 3 * static_cast<size_t>(n));
+        *job_id = S.submit(std::move(hs), std::move(x), std::move(v), nsteps, frame_interval);
+        return OCM_OK;
+    });
+}
+
+int ocm_server_next_frame(ocm_server* srv, int32_t timeout_ms, int64_t* job, int64_t* step, double* xyz,
+                          int32_t cap, int32_t* n) {
+    return guarded([&] {
+        auto& S = server(srv);
+        need(n, "n");
+        if (cap > 0) need(xyz, "xyz");
+        ocm::ServerFrame f;
+        long long needed = 0;
+        *n = 0;
+        try {
+            if (!S.next_frame(timeout_ms, f, cap, &needed)) {
+                g_last_error = "no frame within the timeout";
+                return OCM_ERR_TIMEOUT;
+            }
+        } catch (...) {
+            *n = static_cast<int32_t>(needed);
+            throw;
+        }
+        *n = static_cast<int32_t>(needed);
+        if (job) *job = f.job;
+        if (step) *step = f.step;
+        std::copy(f.x.begin(), f.x.end(), xyz);
+        return OCM_OK;
+    });
+}
+
+int ocm_server_wait(ocm_server* srv, int64_t job, int32_t timeout_ms, double* positions, double* velocities,
+                    int32_t cap, ocm_job_info* info) {
+    return guarded([&] {
+        auto& S = server(srv);
+        ocm::ServerJob r;
+        if (!S.wait(job, timeout_ms, r)) {
+            g_last_error = "job still running";
+            return OCM_ERR_TIMEOUT;
+        }
+        if (info) {
+            info->status = r.status;
+            info->n_particles = r.sys.n;
+            info->steps_done = r.done;
+            info->submit_ms = r.t_submit;
+            info->start_ms = r.t_start;
+            info->end_ms = r.t_end;
+        }
+        if ((positions || velocities) && cap < r.sys.n)
+            throw ocm::Error(OCM_ERR_INVALID, "result buffer too small (the job is collected)");
+        if (positions) std::copy(r.x.begin(), r.x.end(), positions);
+        if (velocities) std::copy(r.v.begin(), r.v.end(), velocities);
+        if (r.status != OCM_OK) g_last_error = r.message;
+        return OCM_OK;
+    });
+}
+
+int ocm_server_get_stats(ocm_server* srv, ocm_server_stats* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = server(srv).stats();
+        return OCM_OK;
+    });
 }
 
 const char* ocm_last_error(void) { return g_last_error.c_str(); }

@@ -191,6 +191,7 @@ template <typename T> T* Engine::dalloc(size_t n) {
     const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
     check_hip(hipMalloc(&p, bytes), "hipMalloc");
     allocs_.push_back(p);
+    device_bytes_ += bytes;
     return static_cast<T*>(p);
 }
 
@@ -752,9 +753,13 @@ int Engine::run_chunk(int k) {
         check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
         check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
         const int interval = std::min(interval_, k);
-        const bool graph = opt_.use_graph && !timing_;
+        // Stretch lengths set by callers vary; graphs are kept for at most
+        // kMaxGraphs shapes besides the full chunk, other shapes run eagerly.
+        constexpr std::size_t kMaxGraphs = 8;
+        const auto key = std::make_pair(k, interval);
+        const bool graph = opt_.use_graph && !timing_ &&
+                           (k == opt_.chunk_steps || graphs_.count(key) || graphs_.size() < kMaxGraphs);
         if (graph) {
-            const auto key = std::make_pair(k, interval);
             auto it = graphs_.find(key);
             if (it == graphs_.end()) {
                 hipGraph_t g;
@@ -859,6 +864,25 @@ int Engine::minimize(double tol, int max_iter) {
     }
     check_hip(hipMemset(d_.err, 0, sizeof(int) * d_.nsys), "memset");
     return it;
+}
+
+void Engine::enqueue_position_copy(double* dst) {
+    const size_t bytes = static_cast<size_t>(d_.nb) * sizeof(double);
+    check_hip(hipMemcpyAsync(dst, d_.px, bytes, hipMemcpyDeviceToDevice, stream_), "position copy");
+    check_hip(hipMemcpyAsync(dst + d_.nb, d_.py, bytes, hipMemcpyDeviceToDevice, stream_), "position copy");
+    check_hip(hipMemcpyAsync(dst + 2 * static_cast<size_t>(d_.nb), d_.pz, bytes, hipMemcpyDeviceToDevice, stream_),
+              "position copy");
+}
+
+void Engine::set_steps_done(int s, long long steps_done) {
+    sync();
+    long long gs[2] = {0, 0};
+    check_hip(hipMemcpy(gs, d_.gstep, sizeof(gs), hipMemcpyDeviceToHost), "gstep");
+    sys_step_origin_.at(s) = gs[0] - steps_done;
+    DevSys D;
+    check_hip(hipMemcpy(&D, d_.sys + s, sizeof(DevSys), hipMemcpyDeviceToHost), "sys");
+    D.step_origin = sys_step_origin_[s];
+    check_hip(hipMemcpy(d_.sys + s, &D, sizeof(DevSys), hipMemcpyHostToDevice), "sys");
 }
 
 // ------------------------------------------------------------ state I/O

@@ -77,6 +77,15 @@ public:
     void get_state(int s, int what, uint32_t groups, ocm_state* out);
     long long neighbor_pairs(int s, int force_index, int32_t* out, long long cap);
     int system_status(int s) const { return sys_status_.at(s); }
+    // Continues system s's step count (and its noise stream) from steps_done.
+    void set_steps_done(int s, long long steps_done);
+    std::size_t device_bytes() const { return device_bytes_; }
+    int n_beads_total() const { return d_.nb; }
+    int bead_begin(int s) const { return bead_begin_.at(s); }
+    hipStream_t stream() const { return stream_; }
+    // Enqueues on the engine stream a copy of every bead position into dst
+    // (device memory, 3*nb doubles: the x block, then y, then z).
+    void enqueue_position_copy(double* dst);
 
     void set_timing(bool on) { timing_ = on; }
     TimingSlot timing(int k) const { return timing_slots_.at(k); }
@@ -133,6 +142,7 @@ private:
 
     std::map<std::pair<int, int>, hipGraphExec_t> graphs_;
     long long col_bytes_ = 0;
+    std::size_t device_bytes_ = 0;
 };
 
 }  // namespace ocm

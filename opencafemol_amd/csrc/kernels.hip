@@ -96,9 +96,17 @@ __device__ inline void pack_bead(const PackArgs& pk, int b, double x, double y, 
 }
 
 // ------------------------------------------------------------- integrator
-// pending != 0: the closing half kick of the previous step is still owed, so
-// the opening kick uses a full dt.  gstep[pin] is this step's global step;
-// thread 0 writes gstep[pout] = gstep[pin] + 1 for the next step.
+// One half kick v + (dt/2) F/m, as one FMA.  The closing kick of a step and the
+// opening kick of the next are applied as two such roundings whether they run
+// in one launch or in two, so the state after any step is the same bits
+// however the steps are grouped into calls.
+__device__ __forceinline__ double half_kick(double v, double h, double f, double im) {
+    return fma(h * f, im, v);
+}
+
+// pending != 0: the closing half kick of the previous step is still owed and
+// is applied first.  gstep[pin] is this step's global step; thread 0 writes
+// gstep[pout] = gstep[pin] + 1 for the next step.
 __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* __restrict__ sys_of,
                             const int* __restrict__ local_id, const double* __restrict__ inv_m,
                             double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
@@ -119,10 +127,15 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
         if (pk.enabled) pack_bead(pk, b, px[b], py[b], pz[b], err);
         return;
     }
-    const double kick = pending ? 2.0 * S.half_dt : S.half_dt;
-    double v0 = vx[b] + kick * f0 * im;
-    double v1 = vy[b] + kick * f1 * im;
-    double v2 = vz[b] + kick * f2 * im;
+    double v0 = vx[b], v1 = vy[b], v2 = vz[b];
+    if (pending) {
+        v0 = half_kick(v0, S.half_dt, f0, im);
+        v1 = half_kick(v1, S.half_dt, f1, im);
+        v2 = half_kick(v2, S.half_dt, f2, im);
+    }
+    v0 = half_kick(v0, S.half_dt, f0, im);
+    v1 = half_kick(v1, S.half_dt, f1, im);
+    v2 = half_kick(v2, S.half_dt, f2, im);
     double x0 = px[b] + S.half_dt * v0;
     double x1 = py[b] + S.half_dt * v1;
     double x2 = pz[b] + S.half_dt * v2;
@@ -157,9 +170,9 @@ __global__ void k_half_kick(int nb, const DevSys* __restrict__ sys, const int* _
     const double im = inv_m[b];
     if (im == 0.0) return;
     const double h = sys[sys_of[b]].half_dt;
-    vx[b] += h * (fx[b] * kFixInv) * im;
-    vy[b] += h * (fy[b] * kFixInv) * im;
-    vz[b] += h * (fz[b] * kFixInv) * im;
+    vx[b] = half_kick(vx[b], h, fx[b] * kFixInv, im);
+    vy[b] = half_kick(vy[b], h, fy[b] * kFixInv, im);
+    vz[b] = half_kick(vz[b], h, fz[b] * kFixInv, im);
 }
 
 // ------------------------------------------------------------ record pack

@@ -566,7 +566,12 @@ int32_t oracle_abi_layout(int64_t* out, int32_t cap) {
         (int64_t)offsetof(ocm_system_desc, box), (int64_t)offsetof(ocm_system_desc, forces),
         (int64_t)sizeof(ocm_integrator_desc), (int64_t)sizeof(ocm_options),
         (int64_t)offsetof(ocm_options, skin), (int64_t)sizeof(ocm_state),
-        (int64_t)offsetof(ocm_state, energy_by_group), (int64_t)offsetof(ocm_state, step)};
+        (int64_t)offsetof(ocm_state, energy_by_group), (int64_t)offsetof(ocm_state, step),
+        (int64_t)sizeof(ocm_server_options), (int64_t)offsetof(ocm_server_options, hbm_budget_bytes),
+        (int64_t)offsetof(ocm_server_options, quantum_steps), (int64_t)sizeof(ocm_server_stats),
+        (int64_t)offsetof(ocm_server_stats, latency_p50_ms), (int64_t)offsetof(ocm_server_stats, budget_bytes),
+        (int64_t)sizeof(ocm_job_info), (int64_t)offsetof(ocm_job_info, steps_done),
+        (int64_t)offsetof(ocm_job_info, end_ms)};
     const int32_t n = (int32_t)(sizeof(v) / sizeof(v[0]));
     for (int32_t i = 0; i < n && i < cap; ++i) out[i] = v[i];
     return n;

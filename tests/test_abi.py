@@ -45,6 +45,10 @@ def test_struct_layout_matches_c(oracle):
               F.n_exclusions.offset, F.exclusions.offset, C.sizeof(S), S.box.offset, S.forces.offset,
               C.sizeof(abi.IntegratorDesc), C.sizeof(O), O.skin.offset, C.sizeof(T),
               T.energy_by_group.offset, T.step.offset]
+    SO, SS, J = abi.ServerOptions, abi.ServerStats, abi.JobInfo
+    expect += [C.sizeof(SO), SO.hbm_budget_bytes.offset, SO.quantum_steps.offset, C.sizeof(SS),
+               SS.latency_p50_ms.offset, SS.budget_bytes.offset, C.sizeof(J), J.steps_done.offset,
+               J.end_ms.offset]
     assert got == expect
 
 
@@ -121,3 +125,24 @@ def test_no_device_is_reported_on_a_cpu_host():
     s, it, x = configs.go_chain(20, seed=1)
     code, msg = _create(s)
     assert code == abi.OCM_ERR_NO_DEVICE
+
+
+def test_server_options_default():
+    lib = abi.load_library()
+    o = abi.ServerOptions()
+    lib.ocm_server_options_default(C.byref(o))
+    assert (o.hbm_budget_bytes, o.max_active_jobs, o.max_queued_frames, o.quantum_steps) == (0, 256, 4096, 2000)
+    assert (o.engine.rebuild_interval, o.engine.chunk_steps) == (20, 200)
+
+
+def test_server_reports_no_device_on_a_cpu_host():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    lib = abi.load_library()
+    h = C.c_void_p()
+    assert lib.ocm_server_create(None, C.byref(h)) == abi.OCM_ERR_NO_DEVICE
+    assert not h.value
+    # calls on a NULL server are rejected, not dereferenced
+    st = abi.ServerStats()
+    assert lib.ocm_server_get_stats(None, C.byref(st)) == abi.OCM_ERR_INVALID
+    assert lib.ocm_server_destroy(None) == abi.OCM_OK

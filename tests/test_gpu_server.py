@@ -1,0 +1,166 @@
+"""Continuous-batching server through the C ABI (ocm_server_*).
+
+The claim under test: a job's frames and final state are the same bits when
+it runs alone in a plain context and when it runs in the server's mix, with
+jobs arriving while others run, waiting for slots or memory, and the batch
+repacked around them.  The step grouping test pins the property this rests on:
+the state after a step does not depend on how steps are split into calls.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from opencafemol_amd import abi, configs
+from opencafemol_amd.context import Context, Server
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    sys.__stderr__.write(f"\n[gpu-case] {request.node.nodeid}\n")
+    sys.__stderr__.flush()
+    yield
+
+
+def _relaxed_hps(seed):
+    s, it, x = configs.hps_slab(n_chains=8, chain_len=25, box=(7.0, 7.0, 25.0), slab_z=5.0, seed=seed)
+    with Context(s, it) as c:
+        c.set_positions(x)
+        c.minimize(tolerance=50.0, max_iterations=400)
+        x = c.get_state(abi.STATE_POSITIONS)["positions"]
+    return s, it, x
+
+
+def _jobs():
+    """(system, integrator, positions, velocities or None, nsteps, frame_interval)."""
+    jobs = []
+    reps = configs.go_replicas(5, n_min=50, n_max=300, seed=7)
+    nsteps = [1200, 700, 1500, 400, 900]
+    frames = [300, 0, 250, 100, 450]
+    for k, (s, it, x) in enumerate(reps):
+        v = None if k % 2 == 0 else np.random.default_rng(k).normal(scale=0.3, size=x.shape)
+        jobs.append((s, it, x, v, nsteps[k], frames[k]))
+    s, it, x = _relaxed_hps(3)
+    jobs.append((s, it, x, None, 1000, 200))
+    return jobs
+
+
+def _alone(job):
+    """Frames {step: positions} and the final state from a plain context."""
+    s, it, x, v, nsteps, fi = job
+    frames = {}
+    with Context(s, it) as c:
+        c.set_positions(x)
+        if v is None:
+            c.set_velocities_to_temperature(it.temperature, it.seed)
+        else:
+            c.set_velocities(v)
+        done = 0
+        while done < nsteps:
+            n = min(nsteps, (done // fi + 1) * fi) - done if fi else nsteps - done
+            assert c.step(n) == abi.OCM_OK
+            done += n
+            if fi and done % fi == 0:
+                frames[done] = c.get_state(abi.STATE_POSITIONS)["positions"]
+        st = c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES)
+    return frames, st["positions"], st["velocities"]
+
+
+def _serve(jobs, stagger, **server_opt):
+    """Runs the jobs through one server; the last `stagger` jobs are submitted
+    only after the first frame has come back, while the others are running."""
+    frames = {}
+    with Server(**server_opt) as srv:
+        ids = []
+        first = len(jobs) - stagger
+        for s, it, x, v, n, fi in jobs[:first]:
+            ids.append(srv.submit(s, it, x, v, nsteps=n, frame_interval=fi))
+        if stagger:
+            f = srv.next_frame(timeout_ms=120000)
+            assert f is not None
+            frames[(f[0], f[1])] = f[2]
+            for s, it, x, v, n, fi in jobs[first:]:
+                ids.append(srv.submit(s, it, x, v, nsteps=n, frame_interval=fi))
+
+        def keep(job, step, pos):
+            assert (job, step) not in frames
+            frames[(job, step)] = pos
+
+        results = srv.drain(ids, on_frame=keep)
+        stats = srv.stats()
+    return ids, frames, results, stats
+
+
+def _assert_same(jobs, ids, frames, results):
+    for job, jid in zip(jobs, ids):
+        ref_frames, xr, vr = _alone(job)
+        r = results[jid]
+        assert r["status"] == abi.OCM_OK
+        assert r["steps"] == job[4]
+        assert np.array_equal(r["positions"], xr), f"job {jid}: final positions differ"
+        assert np.array_equal(r["velocities"], vr), f"job {jid}: final velocities differ"
+        got = {step: pos for (j, step), pos in frames.items() if j == jid}
+        assert sorted(got) == sorted(ref_frames), f"job {jid}: frame steps differ"
+        for step, pos in ref_frames.items():
+            assert np.array_equal(got[step], pos), f"job {jid}: frame at step {step} differs"
+
+
+def test_step_grouping_does_not_change_bits():
+    s, it, x = configs.go_chain(120, seed=11)
+    out = []
+    for split in ([300], [7, 150, 143], [1] * 5 + [295], [200, 100]):
+        with Context(s, it, chunk_steps=64, rebuild_interval=10) as c:
+            c.set_positions(x)
+            c.set_velocities_to_temperature(300.0, 5)
+            for n in split:
+                c.step(n)
+            st = c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES)
+            assert st["step"] == 300
+            out.append(st)
+    for st in out[1:]:
+        assert np.array_equal(st["positions"], out[0]["positions"])
+        assert np.array_equal(st["velocities"], out[0]["velocities"])
+
+
+def test_server_jobs_alone_equal_in_mix():
+    jobs = _jobs()
+    ids, frames, results, stats = _serve(jobs, stagger=3, max_active_jobs=3, quantum_steps=350)
+    _assert_same(jobs, ids, frames, results)
+    assert stats["completed"] == len(jobs) and stats["failed"] == 0
+    assert stats["frames"] == sum(n // fi for *_, n, fi in jobs if fi)
+    assert stats["repacks"] >= 2
+    assert stats["latency_p99_ms"] >= stats["latency_p50_ms"] > 0
+    assert stats["simulated_ns"] == pytest.approx(sum(n * it.dt for _, it, _, _, n, _ in jobs) * 1e-3)
+
+
+def test_server_memory_budget_serialises_jobs():
+    jobs = _jobs()[:3]
+    # each job's estimate is 1 MiB of fixed overhead plus under 1 MiB for these
+    # sizes, so a 2 MiB budget holds one job at a time
+    ids, frames, results, stats = _serve(jobs, stagger=0, hbm_budget_bytes=2 * 2**20)
+    _assert_same(jobs, ids, frames, results)
+    for a, b in zip(ids, ids[1:]):
+        assert results[b]["start_ms"] >= results[a]["end_ms"]
+    assert stats["repacks"] == len(jobs)
+    assert stats["wait_p99_ms"] > 0
+    with Server(hbm_budget_bytes=2**20) as srv:
+        s, it, x, v, n, fi = jobs[0]
+        with pytest.raises(abi.OcmError) as e:
+            srv.submit(s, it, x, v, nsteps=n)
+        assert e.value.code == abi.OCM_ERR_OOM
+
+
+def test_server_numeric_failure_stays_in_its_job():
+    jobs = _jobs()[:3]
+    s, it, x = configs.go_chain(30, seed=1)
+    x = x.copy()
+    x[10] = x[20] + 1e-4
+    bad = (s, it, x, None, 500, 100)
+    mixed = [jobs[0], bad, jobs[1], jobs[2]]
+    ids, frames, results, stats = _serve(mixed, stagger=0)
+    assert results[ids[1]]["status"] == abi.OCM_ERR_NUMERIC
+    assert stats["failed"] == 1 and stats["completed"] == 3
+    good = [0, 2, 3]
+    _assert_same([mixed[k] for k in good], [ids[k] for k in good], frames, results)
