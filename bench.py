@@ -1,7 +1,7 @@
 """MD-step throughput on one MI355X: ns/day of the HPS slab (100 chains x 140
 beads = 14,000 beads, periodic 15 x 15 x 150 nm, dt 10 fs, 300 K).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hps|go|replicas|serve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hps|go|replicas|serve|copy]
 
 Protocol: build the seeded slab, relax it on the device (ocm_minimize), draw
 Maxwell-Boltzmann velocities, run W untimed steps, then time exactly K steps
@@ -242,6 +242,31 @@ def serve_bench(args):
     print(json.dumps(out))
 
 
+def copy_bench(args):
+    """Plain device copy of a known size (256 MiB read + 256 MiB written per
+    launch, torch's elementwise kernel): the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters of the pair kernel to bytes."""
+    import torch
+    n = 64 * 2**20
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    t = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    t.record()
+    for _ in range(args.steps):
+        torch.neg(src, out=dst)
+    e.record()
+    torch.cuda.synchronize()
+    ms = t.elapsed_time(e) / args.steps
+    gbs = 2 * 4 * n / (ms * 1e-3) / 1e9
+    print(json.dumps({"workload": "copy_256MiB", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "ms_per_launch": ms, "GB/s": gbs}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +293,9 @@ def main():
     if args.workload == "serve":
         serve_bench(args)
         return
+    if args.workload == "copy":
+        copy_bench(args)
+        return
 
     from opencafemol_amd import abi
     from opencafemol_amd.context import Batch
@@ -293,9 +321,11 @@ def main():
         raise SystemExit(f"warmup failed with status {code}")
     log(f"warmup {args.warmup} steps done")
 
+    rebuilds0 = b.list_stats()[2]
     t0 = time.perf_counter()
     code = b.step(args.steps)
     t1 = time.perf_counter()
+    rebuilds_timed = b.list_stats()[2] - rebuilds0
     if code != abi.OCM_OK:
         raise SystemExit(f"timed run failed with status {code}")
     el = t1 - t0
@@ -324,7 +354,8 @@ def main():
     kernels = {"pair_us": avg_us, "bonded_us": bond_ms / max(bond_n, 1) * 1000, "integrator_us":
                int_ms / max(int_n, 1) * 1000, "list_build_us": nl_ms / max(nl_n, 1) * 1000,
                "pack_us": pk_ms / max(pk_n, 1) * 1000, "list_entries": entries, "list_rows": members,
-               "entries_per_s": entries / (avg_us * 1e-6)}
+               "entries_per_s": entries / (avg_us * 1e-6), "list_rebuilds_timed": rebuilds_timed,
+               "steps_per_rebuild": args.steps / max(rebuilds_timed, 1)}
 
     cpu = None
     if not args.no_cpu and len(systems) == 1:

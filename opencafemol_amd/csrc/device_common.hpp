@@ -74,7 +74,7 @@ struct DevPairGroup {
     int reach[3];          // 1 when the cell grid has >= 3 cells along an axis
     int egroup[4];         // energy group of the EXV, DH, LJR, LJA member forces
     int n_cat;
-    int pad0;
+    int stride;            // list entries reserved per row (multiple of 4)
     double rc2;            // cutoff^2 (nm^2)
     double rl2;            // (cutoff + skin)^2
     double half_skin2;     // (skin/2)^2

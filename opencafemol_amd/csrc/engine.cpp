@@ -529,7 +529,16 @@ void Engine::pack() {
     d_.terms = dalloc<BTerm>(terms.size()); upload(d_.terms, terms);
     d_.fla = dalloc<FlaTable>(fla.size()); upload(d_.fla, fla);
     d_.fld = dalloc<FldTable>(fld.size()); upload(d_.fld, fld);
-    d_.groups = dalloc<DevPairGroup>(groups.size()); upload(d_.groups, groups);
+    // rows of a group get a fixed number of list slots: the estimate per row,
+    // widened on overflow (grow_rows)
+    for (int g = 0; g < static_cast<int>(groups.size()); ++g) {
+        const long long per_row = group_list_estimate_[g] / std::max(groups[g].member_count, 1) + 8;
+        groups[g].stride = static_cast<int>(std::min<long long>((per_row + 3) & ~3LL, 1 << 20));
+    }
+    host_groups_ = groups;
+    d_.groups = dalloc<DevPairGroup>(std::max<size_t>(groups.size(), 1)); upload(d_.groups, groups);
+    d_.group_need = dalloc<int>(std::max<size_t>(groups.size(), 1));
+    check_hip(hipMemset(d_.group_need, 0, sizeof(int) * std::max<size_t>(groups.size(), 1)), "memset");
     const int nm = d_.nm;
     d_.member_group = dalloc<int>(nm); upload(d_.member_group, member_group);
     d_.member_bead = dalloc<int>(nm); upload(d_.member_bead, member_bead);
@@ -564,13 +573,7 @@ void Engine::pack() {
     d_.row_count = dalloc<int>(nm);
     d_.row_start = dalloc<int>(nm);
     check_hip(hipMemset(d_.row_count, 0, sizeof(int) * std::max(nm, 1)), "memset");
-    check_hip(hipMemset(d_.row_start, 0, sizeof(int) * std::max(nm, 1)), "memset");
-    long long est = 0;
-    for (long long e : group_list_estimate_) est += e;
-    d_.list_capacity = std::max<long long>(1 << 16, est + 4LL * nm);  // rows padded to 4 entries
-    if (d_.list_capacity > std::numeric_limits<int>::max()) d_.list_capacity = std::numeric_limits<int>::max();
-    d_.col = dalloc<int>(d_.list_capacity);
-    col_bytes_ = d_.list_capacity * static_cast<long long>(sizeof(int));
+    layout_rows();
     const int scan_n = std::max(d_.ncell, nm);
     d_.scan_partial = dalloc<int>(scan_n / 2048 + 2);
     d_.scan_total = dalloc<int>(2);
@@ -674,16 +677,49 @@ void Engine::restore() {
     check_hip(hipMemcpyAsync(d_.gstep, bstep_, 2 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
 }
 
-void Engine::grow_list(long long needed) {
-    long long cap = static_cast<long long>(needed * 1.25) + (1 << 16);
-    if (cap > std::numeric_limits<int>::max()) cap = std::numeric_limits<int>::max();
-    if (cap <= d_.list_capacity) throw Error(OCM_ERR_OOM, "neighbor list exceeds 2^31 entries");
+// Row r of group g owns list slots [row_start[r], row_start[r] + stride_g);
+// groups follow each other in col.  (Re)allocates col and uploads row_start.
+void Engine::layout_rows() {
+    std::vector<int> row_start(std::max(d_.nm, 1), 0);
+    long long total = 0;
+    for (const DevPairGroup& G : host_groups_) {
+        for (int r = 0; r < G.member_count; ++r) {
+            if (total > std::numeric_limits<int>::max() - G.stride)
+                throw Error(OCM_ERR_OOM, "neighbor list exceeds 2^31 entries");
+            row_start[G.member_begin + r] = static_cast<int>(total);
+            total += G.stride;
+        }
+    }
+    upload(d_.row_start, row_start);
+    if (d_.col) {
+        auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(d_.col));
+        if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+        d_.col = nullptr;
+    }
+    d_.list_capacity = std::max<long long>(total, 4);
+    d_.col = dalloc<int>(d_.list_capacity);
+    col_bytes_ = d_.list_capacity * static_cast<long long>(sizeof(int));
+}
+
+// After a build reported rows longer than their slots: widen those groups to
+// 1.25x the longest row seen (plus 8) and lay the list out again.
+void Engine::grow_rows() {
     sync();
-    auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(d_.col));
-    if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
-    d_.col = dalloc<int>(cap);
-    d_.list_capacity = cap;
-    col_bytes_ = cap * static_cast<long long>(sizeof(int));
+    const int ng = static_cast<int>(host_groups_.size());
+    std::vector<int> need(std::max(ng, 1), 0);
+    check_hip(hipMemcpy(need.data(), d_.group_need, sizeof(int) * need.size(), hipMemcpyDeviceToHost), "need");
+    bool grew = false;
+    for (int g = 0; g < ng; ++g) {
+        if (need[g] <= host_groups_[g].stride) continue;
+        const long long w = (static_cast<long long>(need[g] * 1.25) + 8 + 3) & ~3LL;
+        if (w > (1 << 24)) throw Error(OCM_ERR_OOM, "a neighbor-list row exceeds 2^24 entries");
+        host_groups_[g].stride = static_cast<int>(w);
+        grew = true;
+    }
+    if (!grew) throw Error(OCM_ERR_HIP, "list overflow reported without a longer row");
+    upload(d_.groups, host_groups_);
+    check_hip(hipMemset(d_.group_need, 0, sizeof(int) * std::max(ng, 1)), "memset");
+    layout_rows();
     drop_graphs();
 }
 
@@ -697,9 +733,7 @@ void Engine::ensure_forces() {
         int flags = 0;
         check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
         if (flags & kFlagListOverflow) {
-            int total = 0;
-            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
-            grow_list(total);
+            grow_rows();
             continue;
         }
         forces_dirty_ = false;
@@ -782,10 +816,8 @@ int Engine::run_chunk(int k) {
         check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
         check_hip(hipMemcpy(err.data(), d_.err, sizeof(int) * d_.nsys, hipMemcpyDeviceToHost), "err");
         if (flags & kFlagListOverflow) {
-            int total = 0;
-            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
             restore();
-            grow_list(total);
+            grow_rows();
             continue;
         }
         bool stale = false, numeric = false;
@@ -854,9 +886,7 @@ int Engine::minimize(double tol, int max_iter) {
         int flags = 0;
         check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
         if (flags & kFlagListOverflow) {
-            int total = 0;
-            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
-            grow_list(total);
+            grow_rows();
             check_hip(hipMemset(d_.flags, 0, sizeof(int)), "memset");
             launch_zero_forces(d_, d_.f, stream_);
             enqueue_forces(d_.f, true, 0xFFFFFFFFu);
@@ -938,12 +968,11 @@ void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
     if ((what & OCM_STATE_FORCES) && out->forces) {
         const uint32_t present = sys_group_mask_[s];
         ForceBuf src = d_.f;
+        ensure_forces();   // builds (and if needed widens) the list for the current positions
         if ((present & groups) != present) {
             launch_zero_forces(d_, d_.f2, stream_);
-            enqueue_forces(d_.f2, true, groups);
+            enqueue_forces(d_.f2, false, groups);
             src = d_.f2;
-        } else {
-            ensure_forces();
         }
         sync();
         std::vector<long long> fx(H.n), fy(H.n), fz(H.n);
@@ -966,9 +995,7 @@ void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
         int flags = 0;
         check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
         if (flags & kFlagListOverflow) {
-            int total = 0;
-            check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
-            grow_list(total);
+            grow_rows();
             check_hip(hipMemset(d_.flags, 0, sizeof(int)), "memset");
             get_state(s, OCM_STATE_ENERGY, groups, out);
             return;
@@ -1017,8 +1044,10 @@ long long Engine::neighbor_pairs(int s, int force_index, int32_t* out, long long
 
 void Engine::list_stats(long long* entries, long long* members, long long* rebuilds) {
     sync();
-    int total = 0;
-    check_hip(hipMemcpy(&total, d_.scan_total + 1, sizeof(int), hipMemcpyDeviceToHost), "total");
+    std::vector<int> cnt(std::max(d_.nm, 1), 0);
+    if (d_.nm > 0) check_hip(hipMemcpy(cnt.data(), d_.row_count, sizeof(int) * d_.nm, hipMemcpyDeviceToHost), "rows");
+    long long total = 0;
+    for (int k = 0; k < d_.nm; ++k) total += cnt[k];
     if (entries) *entries = total;
     if (members) *members = d_.nm;
     if (rebuilds) *rebuilds = rebuilds_;

@@ -103,7 +103,8 @@ private:
     void restore();
     void check_hip(hipError_t e, const char* what);
     void sync();
-    void grow_list(long long needed);
+    void layout_rows();
+    void grow_rows();
     void autotune_team();
     void drop_graphs();
     template <typename F> void timed(int slot, F&& f);
@@ -121,6 +122,7 @@ private:
     std::vector<int> bead_begin_;
     std::vector<std::vector<int>> force_group_of_;   // [sys][force] -> pair group or -1
     std::vector<int> group_member_begin_, group_member_count_;
+    std::vector<DevPairGroup> host_groups_;
     std::vector<long long> group_list_estimate_;
     std::vector<uint32_t> sys_group_mask_;
     std::vector<int> sys_status_;

@@ -22,10 +22,20 @@ namespace ocm {
 // ------------------------------------------------------------ small helpers
 // Out-of-range or non-finite contributions are flagged and replaced by 0, so
 // the float-to-integer conversion is only ever applied to representable values.
+//
+// FP32 values go to fixed point as floor(v * 2^32): floor(v) is the high word
+// and the fraction, scaled by 2^32, the low word.  v - floor(v) is exact except
+// for negative |v| < 1, where it is rounded once (by at most 2^-25); the low
+// word is clamped below 2^32 for the case it rounds up to 1.  Requires |v| < 2^30.
+__device__ inline long long fix_floor(float v) {
+    const float h = floorf(v);
+    const float l = fminf((v - h) * 4294967296.0f, 4294967040.0f);
+    return (static_cast<long long>(static_cast<int>(h)) << 32) + static_cast<long long>(static_cast<unsigned>(l));
+}
 __device__ inline long long to_fix(float v, int& bad) {
     const bool ok = fabsf(v) < kFixLimitF;
     bad |= !ok;
-    return ok ? static_cast<long long>(v * 4294967296.0f) : 0LL;
+    return fix_floor(ok ? v : 0.0f);
 }
 __device__ inline long long to_fix_d(double v, int& bad) {
     const bool ok = fabs(v) < 1073741824.0;
@@ -292,14 +302,18 @@ __global__ void k_tile_items(int ncell, const int* __restrict__ cell_count, int2
     for (int q = 0; q < n; ++q) items[at + q] = make_int2(c, q);
 }
 
-template <bool FILL>
+// One pass: every row owns G.stride slots of col starting at row_start (fixed
+// per row), so rows are written as they are found.  A row longer than its
+// slots writes only what fits, reports its length through group_need and the
+// overflow flag, and gets row_count 0; the host then widens the rows of that
+// group and builds again.
 __global__ __launch_bounds__(256) void k_nl_tiles(
     const int2* __restrict__ items, const int* __restrict__ n_items, const DevPairGroup* __restrict__ groups,
     const int* __restrict__ cell_group,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
     const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
-    const int* __restrict__ total, long long capacity, int* __restrict__ col, int* __restrict__ flags) {
+    int* __restrict__ group_need, int* __restrict__ col, int* __restrict__ flags) {
     __shared__ PairRec cand[kTileCap];
     __shared__ int cand_slot[kTileCap];
     __shared__ int row_run[kTileChunk];
@@ -311,12 +325,9 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     const int home = items[item].x;
     const int h0 = cell_start[home] + items[item].y * kTileChunk;
     const int nr = min(kTileChunk, cell_count[home] - items[item].y * kTileChunk);
-    if (FILL && static_cast<long long>(*total) > capacity) {
-        for (int r = tid; r < nr; r += blockDim.x) row_count[h0 + r] = 0;  // the pair kernel reads nothing
-        if (tid == 0) atomicOr(flags, kFlagListOverflow);
-        continue;
-    }
-    const DevPairGroup& G = groups[cell_group[home]];
+    const int gid = cell_group[home];
+    const DevPairGroup& G = groups[gid];
+    const int cap = G.stride;
     const int cell = home - G.cell_begin;
     const int cx = cell % G.ncell[0];
     const int cy = (cell / G.ncell[0]) % G.ncell[1];
@@ -369,7 +380,7 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
 #pragma unroll
             for (int e = 0; e < 16; ++e) ex[e] = __builtin_amdgcn_readlane(mine, e);
             const bool few = ec <= 16;
-            const int base = FILL ? row_start[i] : 0;
+            const int base = row_start[i];
             int run = row_run[r];
             for (int kb = 0; kb < nw; kb += 64) {
                 const int k = kb + lane;
@@ -399,17 +410,29 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                     }
                 }
                 const unsigned long long mask = __ballot(pass);
-                if (FILL && pass) col[base + run + __popcll(mask & below)] = sl;
+                const int pos = run + __popcll(mask & below);
+                if (pass && pos < cap) col[base + pos] = sl;
                 run += __popcll(mask);
             }
             if (lane == 0) row_run[r] = run;
             // last window: pad the row to a multiple of 4 entries with -1
-            if (FILL && wbeg + kTileCap >= ntot && lane < ((4 - (run & 3)) & 3)) col[base + run + lane] = -1;
+            if (wbeg + kTileCap >= ntot) {
+                const int padded = (run + 3) & ~3;
+                if (padded <= cap && lane < padded - run) col[base + run + lane] = -1;
+            }
         }
     }
     __syncthreads();
-    if (!FILL)
-        for (int r = tid; r < nr; r += blockDim.x) row_count[h0 + r] = (row_run[r] + 3) & ~3;
+    for (int r = tid; r < nr; r += blockDim.x) {
+        const int padded = (row_run[r] + 3) & ~3;
+        if (padded <= cap) {
+            row_count[h0 + r] = padded;
+        } else {
+            row_count[h0 + r] = 0;  // the pair kernel reads nothing; the chunk is rebuilt wider
+            atomicMax(&group_need[gid], padded);
+            atomicOr(flags, kFlagListOverflow);
+        }
+    }
     }  // items
 }
 
@@ -718,6 +741,31 @@ __global__ void k_bonded(int nt, const BTerm* __restrict__ terms, const DevSys* 
 // ------------------------------------------------------------------- pair
 // Reciprocals: the FP32 force pass uses the hardware v_rcp/v_rsq (1 ulp);
 // the FP64 energy pass divides exactly.
+// exp in the force pass: the hardware 2^x (v_exp_f32) of x*log2(e); its
+// relative error stays near 1e-7 over the Debye-Hueckel range |x| <= 5.
+template <typename R> __device__ inline R r_exp_fast(R v);
+template <> __device__ inline float r_exp_fast<float>(float v) { return __builtin_amdgcn_exp2f(v * 1.44269504088896341f); }
+template <> __device__ inline double r_exp_fast<double>(double v) { return exp(v); }
+
+// Potential constants of a pair group in the arithmetic type of a pass, read
+// once per block of rows rather than per pair.
+template <typename R> struct PairK {
+    R exv_eps, exv_corr, dh_K, dh_il, dh_corr, lja_rsw, lja_inv_sw;
+    bool lja_switch;
+};
+template <typename R> __device__ inline PairK<R> pair_constants(const DevPairGroup& G) {
+    PairK<R> k;
+    k.exv_eps = static_cast<R>(G.exv_eps);
+    k.exv_corr = static_cast<R>(G.exv_corr);
+    k.dh_K = static_cast<R>(G.dh_K);
+    k.dh_il = static_cast<R>(G.dh_inv_lambda);
+    k.dh_corr = static_cast<R>(G.dh_corr);
+    k.lja_rsw = static_cast<R>(G.lja_rsw);
+    k.lja_inv_sw = static_cast<R>(G.lja_inv_sw);
+    k.lja_switch = G.lja_rsw > 0;
+    return k;
+}
+
 template <typename R> __device__ inline R r_rcp(R v);
 template <> __device__ inline float r_rcp<float>(float v) { return __builtin_amdgcn_rcpf(v); }
 template <> __device__ inline double r_rcp<double>(double v) { return 1.0 / v; }
@@ -728,7 +776,7 @@ template <> __device__ inline double r_rsqrt<double>(double v) { return 1.0 / sq
 // Energy terms and (dE/dr)/r of all kinds of a pair group for one pair.
 // R = float (force pass, FP32 type rows tf) or double (energy pass, rows td).
 template <typename R>
-__device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* __restrict__ tfi,
+__device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __restrict__ tfi,
                                  const float* __restrict__ tfj, const double* __restrict__ tdi,
                                  const double* __restrict__ tdj, R r2, R* dedr_r, R e[4]) {
     auto P = [&](bool i_side, int k) -> R {
@@ -738,7 +786,7 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
     const R ir2 = r_rcp<R>(r2);
     R f = 0;
     R r = 0, ir = 0;
-    if ((kinds & kDh) || ((kinds & kLja) && G.lja_rsw > 0)) {
+    if ((kinds & kDh) || ((kinds & kLja) && G.lja_switch)) {
         ir = r_rsqrt<R>(r2);
         r = r2 * ir;
     }
@@ -748,17 +796,17 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
         const R q2 = s2 * ir2;
         const R q6 = q2 * q2 * q2, q12 = q6 * q6;
         const R s6 = s2 * s2 * s2;
-        const R eps = static_cast<R>(G.exv_eps);
+        const R eps = G.exv_eps;
         f += R(-12) * eps * q12 * ir2;
-        e[0] = eps * (q12 - s6 * s6 * static_cast<R>(G.exv_corr));
+        e[0] = eps * (q12 - s6 * s6 * G.exv_corr);
     }
     if (kinds & kDh) {
         const R qq = P(true, 1) * P(false, 1);
-        const R il = static_cast<R>(G.dh_inv_lambda);
-        const R ex = r_exp<R>(-r * il);
-        const R K = static_cast<R>(G.dh_K);
+        const R il = G.dh_il;
+        const R ex = r_exp_fast<R>(-r * il);
+        const R K = G.dh_K;
         f += -qq * K * ex * (ir + il) * ir2;
-        e[1] = qq * K * (ex * ir - static_cast<R>(G.dh_corr));
+        e[1] = qq * K * (ex * ir - G.dh_corr);
     }
     const R c13 = static_cast<R>(1.2599210498948731648);  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
     R s6 = 0, s12 = 0, sb_r = -1;
@@ -791,10 +839,10 @@ __device__ inline void pair_eval(int kinds, const DevPairGroup& G, const float* 
             du += eb * (R(-48) * s12 + R(24) * s6) * ir2;
         }
         if (th2 - r2 >= R(0)) u -= eb;
-        if (G.lja_rsw > 0) {
-            const R rsw = static_cast<R>(G.lja_rsw);
+        if (G.lja_switch) {
+            const R rsw = G.lja_rsw;
             if (r > rsw) {
-                const R inv = static_cast<R>(G.lja_inv_sw);
+                const R inv = G.lja_inv_sw;
                 const R x = (r - rsw) * inv;
                 const R x2 = x * x, x3 = x2 * x;
                 const R S = R(1) - x3 * (R(10) - R(15) * x + R(6) * x2);
@@ -822,6 +870,12 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
     const int2 bt = blocks[bi];
     const DevPairGroup& G = groups[bt.x];
+    const bool periodic = G.periodic;
+    const double bx = G.box[0], by = G.box[1], bz = G.box[2];
+    const double ibx = G.inv_box[0], iby = G.inv_box[1], ibz = G.inv_box[2];
+    const double rc2 = G.rc2;
+    const PairK<float> Kf = pair_constants<float>(G);
+    const PairK<double> Kd = pair_constants<double>(G);
     if (!ENERGY && bt.x != loaded) {  // bt.x is uniform over the workgroup
         __syncthreads();
         const float* src = types_f + static_cast<size_t>(G.type_begin) * kTypeStride;
@@ -862,19 +916,19 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
                 if (s < 0) continue;
                 const PairRec rj = rjs[u];
                 double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-                if (G.periodic) {
-                    dx = min_image(dx, G.box[0], G.inv_box[0]);
-                    dy = min_image(dy, G.box[1], G.inv_box[1]);
-                    dz = min_image(dz, G.box[2], G.inv_box[2]);
+                if (periodic) {
+                    dx = min_image(dx, bx, ibx);
+                    dy = min_image(dy, by, iby);
+                    dz = min_image(dz, bz, ibz);
                 }
                 const double r2 = dist2(dx, dy, dz);
-                if (!(r2 < G.rc2)) continue;
+                if (!(r2 < rc2)) continue;
                 if (ENERGY) {
                     if (row < s) {
                         double f;
                         double e[4] = {0, 0, 0, 0};
                         const double* tdj = types_d + static_cast<size_t>(G.type_begin + (rj.type & 0xFFFF)) * kTypeStride;
-                        pair_eval<double>(kinds, G, nullptr, nullptr, tdi, tdj, r2, &f, e);
+                        pair_eval<double>(kinds, Kd, nullptr, nullptr, tdi, tdj, r2, &f, e);
                         ae0 += to_fix_d(e[0], bad);
                         ae1 += to_fix_d(e[1], bad);
                         ae2 += to_fix_d(e[2], bad);
@@ -884,10 +938,16 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
                     float f;
                     float e[4];
                     const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
-                    pair_eval<float>(kinds, G, tfi, tfj, nullptr, nullptr, static_cast<float>(r2), &f, e);
-                    ax += to_fix(f * static_cast<float>(dx), bad);
-                    ay += to_fix(f * static_cast<float>(dy), bad);
-                    az += to_fix(f * static_cast<float>(dz), bad);
+                    const float r2f = static_cast<float>(r2);
+                    pair_eval<float>(kinds, Kf, tfi, tfj, nullptr, nullptr, r2f, &f, e);
+                    // |f d| <= |f| r: one range test covers the three components
+                    // (and fails for NaN); a failing pair contributes nothing
+                    const bool ok = f * f * r2f < kFixLimitF * kFixLimitF;
+                    bad |= !ok;
+                    const float g = ok ? f : 0.0f;
+                    ax += fix_floor(g * static_cast<float>(dx));
+                    ay += fix_floor(g * static_cast<float>(dy));
+                    az += fix_floor(g * static_cast<float>(dz));
                 }
             }
         }
@@ -1041,15 +1101,9 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
                                              d.rec, d.refx, d.refy, d.refz);
     k_fill_i32<<<1, 64, 0, st>>>(d.tile_nitems, 0, 1);
     k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_count, d.tile_items, d.tile_nitems);
-    k_nl_tiles<false><<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group,
-                                                    d.cell_start, d.cell_count, d.rec, d.slot_member, d.excl_start,
-                                                    d.excl_count, d.excl, d.row_start, d.row_count, d.scan_total + 1,
-                                                    d.list_capacity, d.col, d.flags);
-    launch_scan(d, st, d.row_count, d.row_start, d.nm, d.scan_total + 1);
-    k_nl_tiles<true><<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group,
-                                                   d.cell_start, d.cell_count, d.rec, d.slot_member, d.excl_start,
-                                                   d.excl_count, d.excl, d.row_start, d.row_count, d.scan_total + 1,
-                                                   d.list_capacity, d.col, d.flags);
+    k_nl_tiles<<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group, d.cell_start,
+                                            d.cell_count, d.rec, d.slot_member, d.excl_start, d.excl_count, d.excl,
+                                            d.row_start, d.row_count, d.group_need, d.col, d.flags);
 }
 
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {

@@ -29,6 +29,7 @@ struct DevPtrs {
     long long* gstep = nullptr;   // two slots, see k_integrate
     int* err = nullptr;           // nsys error words
     int* flags = nullptr;         // batch-wide flag word
+    int* group_need = nullptr;    // per pair group: longest row seen that did not fit
     BTerm* terms = nullptr;
     FlaTable* fla = nullptr;
     FldTable* fld = nullptr;

@@ -25,7 +25,7 @@ def _announce(request):
 
 
 def _relaxed_hps(seed):
-    s, it, x = configs.hps_slab(n_chains=8, chain_len=25, box=(7.0, 7.0, 25.0), slab_z=5.0, seed=seed)
+    s, it, x = configs.hps_slab(n_chains=8, chain_len=25, box=(8.0, 8.0, 25.0), slab_z=5.0, seed=seed)
     with Context(s, it) as c:
         c.set_positions(x)
         c.minimize(tolerance=50.0, max_iterations=400)
