@@ -1,7 +1,7 @@
 """MD-step throughput on one MI355X: ns/day of the HPS slab (100 chains x 140
 beads = 14,000 beads, periodic 15 x 15 x 150 nm, dt 10 fs, 300 K).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hps|go|replicas|serve|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hps|condensate|go|replicas|serve|copy]
 
 Protocol: build the seeded slab, relax it on the device (ocm_minimize), draw
 Maxwell-Boltzmann velocities, run W untimed steps, then time exactly K steps
@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "ns/day (single system + 256-replica aggregate); p50/p99 job latency"
+# HBM bytes per list entry of the pair kernel, from the FETCH_SIZE / WRITE_SIZE
+# counter runs summarized in profiles/ (calibrated against a copy kernel)
+TRAFFIC_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_traffic.json")}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -38,6 +41,12 @@ def build_workload(name):
         s, it, x = configs.hps_slab()
         cfg = {"workload": "hps_slab_100x140", "beads": 14000, "chains": 100, "box_nm": [15.0, 15.0, 150.0],
                "slab_nm": 20.0, "cutoff_ah_nm": 2.0, "cutoff_dh_nm": 3.5, "dt_ps": 0.01, "temperature_K": 300.0}
+        return [s], [it], [x], cfg
+    if name == "condensate":
+        s, it, x = configs.hps_condensate()
+        cfg = {"workload": "hps_condensate_1000x1000", "beads": 1000000, "chains": 1000,
+               "box_nm": [60.0, 60.0, 600.0], "start": "serpentine lattice, 0.7 nm", "cutoff_ah_nm": 2.0,
+               "cutoff_dh_nm": 3.5, "skin_nm": 0.3, "rebuild_interval": 20, "dt_ps": 0.01, "temperature_K": 300.0}
         return [s], [it], [x], cfg
     if name == "go":
         s, it, x = configs.go_chain(62, seed=0)
@@ -276,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--minimize-iters", type=int, default=3000)
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for profilers)")
+    ap.add_argument("--skin", type=float, default=None, help="neighbor-list skin (nm); default: library's")
+    ap.add_argument("--rebuild-interval", type=int, default=None, help="steps between list rebuilds")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-steps", default="10000,50000", help="per-job step range (multiples of 10000)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="arrival rate / serial service rate")
@@ -302,7 +313,14 @@ def main():
 
     systems, integs, xs, cfg = build_workload(args.workload)
     log(f"workload {cfg['workload']}: {sum(s.n for s in systems)} beads in {len(systems)} system(s)")
-    b = Batch(systems, integs, use_graph=0 if args.no_graph else 1)
+    lopt = {}
+    if args.workload == "condensate":   # the configuration states its list parameters
+        lopt = {"skin": 0.3, "rebuild_interval": 20}
+    if args.skin is not None:
+        lopt["skin"] = args.skin
+    if args.rebuild_interval is not None:
+        lopt["rebuild_interval"] = args.rebuild_interval
+    b = Batch(systems, integs, use_graph=0 if args.no_graph else 1, **lopt)
     for k, x in enumerate(xs):
         b.set_positions(k, x)
     t = time.perf_counter()
@@ -316,16 +334,21 @@ def main():
         for k, (s, it) in enumerate(zip(systems, integs)):
             v = rng.normal(size=(s.n, 3)) * np.sqrt(kB * it.temperature / s.masses)[:, None]
             b.set_velocities(k, v)
+    b.set_timing(2)   # pair-kernel duration stamps on the device clock; graphs kept (captured in warmup)
     code = b.step(args.warmup)
     if code != abi.OCM_OK:
         raise SystemExit(f"warmup failed with status {code}")
     log(f"warmup {args.warmup} steps done")
 
     rebuilds0 = b.list_stats()[2]
+    ms0, n0 = b.timing(5)
     t0 = time.perf_counter()
     code = b.step(args.steps)
     t1 = time.perf_counter()
+    ms1, n1 = b.timing(5)
+    b.set_timing(0)
     rebuilds_timed = b.list_stats()[2] - rebuilds0
+    stamp_ms, stamp_n = ms1 - ms0, n1 - n0
     if code != abi.OCM_OK:
         raise SystemExit(f"timed run failed with status {code}")
     el = t1 - t0
@@ -335,27 +358,36 @@ def main():
     ns_day = sps * dt_ps * 86400.0 / 1000.0 * len(systems)
     log(f"timed {args.steps} steps in {el:.3f} s: {ms_per_step * 1000:.2f} us/step, {ns_day:.1f} ns/day")
 
-    # kernel timing phase (HIP events around each pair launch on the context stream)
+    # pair kernel duration over the timed region: first block start to last
+    # block end of every launch, on the device wall clock
+    entries, members, rebuilds = b.list_stats()
+    avg_us = stamp_ms / max(stamp_n, 1) * 1000.0
+    bytes_per_launch = 4 * entries + 64 * members
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    log(f"pair kernel {avg_us:.2f} us/launch over {stamp_n} launches of the timed region; list {entries} "
+        f"entries, {members} rows; {achieved:.0f} GB/s algorithmic")
+
+    # breakdown phase: HIP events around every launch (eager; adds launch latency to each figure)
     kt_steps = min(args.steps, 4000)
-    b.set_timing(True)
+    b.set_timing(1)
     code = b.step(kt_steps)
-    b.set_timing(False)
+    b.set_timing(0)
     pair_ms, pair_n = b.timing(0)
     bond_ms, bond_n = b.timing(1)
     int_ms, int_n = b.timing(2)
     nl_ms, nl_n = b.timing(3)
     pk_ms, pk_n = b.timing(4)
-    entries, members, rebuilds = b.list_stats()
-    avg_us = pair_ms / max(pair_n, 1) * 1000.0
-    bytes_per_launch = 4 * entries + 64 * members
-    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
-    log(f"pair kernel {avg_us:.2f} us/launch over {pair_n} launches; list {entries} entries, {members} rows; "
-        f"{achieved:.0f} GB/s algorithmic")
-    kernels = {"pair_us": avg_us, "bonded_us": bond_ms / max(bond_n, 1) * 1000, "integrator_us":
+    kernels = {"pair_us": avg_us, "pair_us_event_bracketed": pair_ms / max(pair_n, 1) * 1000,
+               "bonded_us": bond_ms / max(bond_n, 1) * 1000, "integrator_us":
                int_ms / max(int_n, 1) * 1000, "list_build_us": nl_ms / max(nl_n, 1) * 1000,
                "pack_us": pk_ms / max(pk_n, 1) * 1000, "list_entries": entries, "list_rows": members,
                "entries_per_s": entries / (avg_us * 1e-6), "list_rebuilds_timed": rebuilds_timed,
                "steps_per_rebuild": args.steps / max(rebuilds_timed, 1)}
+
+    traffic = None
+    tf = TRAFFIC_FILES.get(cfg["workload"])
+    if tf and os.path.exists(tf):
+        traffic = json.load(open(tf))["hbm_bytes_per_entry"] * entries
 
     cpu = None
     if not args.no_cpu and len(systems) == 1:
@@ -379,7 +411,7 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "kernels": kernels,
     }

@@ -161,7 +161,11 @@ typedef struct ocm_state {
 
 typedef struct ocm_context ocm_context;
 
-/* Fills `opt` with defaults (skin 0.3 nm, rebuild every 20 steps, ...). */
+/* Fills `opt` with defaults (skin 0.4 nm, rebuild every 20 steps, ...).
+ * Skin and interval change speed only: forces are fixed-point sums over the
+ * pairs inside the cutoff, so any valid list gives the same bits.  When a bead
+ * moves more than skin/2 between rebuilds the chunk is redone with half the
+ * interval; the interval doubles back after a run of clean chunks.          */
 void ocm_options_default(ocm_options* opt);
 
 /* OpenMM::Context(system, integrator, platform), src/Simulator.cpp:17.
@@ -201,9 +205,12 @@ int ocm_minimize(ocm_context* ctx, double tolerance, int32_t max_iterations);
 int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index,
                            int32_t* out_pairs, int64_t cap, int64_t* count);
 
-/* Per-kernel timing gathered with HIP events on the context's stream.
- * kernel: 0 = pair forces, 1 = bonded forces, 2 = integrator,
- *         3 = list build, 4 = record pack.                                 */
+/* Per-kernel timing.  enable = 1: HIP events on the context's stream around
+ * every launch (steps run eagerly), read with kernel 0 = pair forces,
+ * 1 = bonded forces, 2 = integrator, 3 = list build, 4 = record pack.
+ * enable = 2: steps keep running as captured graphs and every pair-force
+ * launch of a step records its first block start and last block end on the
+ * device wall clock; read with kernel 5.  enable = 0: off.                  */
 int ocm_set_timing(ocm_context* ctx, int32_t enable);
 int ocm_get_timing(ocm_context* ctx, int32_t kernel, double* total_ms,
                    int64_t* launches);

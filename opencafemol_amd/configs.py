@@ -247,11 +247,12 @@ def hps_types(rng, n_types=20):
 
 def hps_slab(n_chains=100, chain_len=140, box=(15.0, 15.0, 150.0), slab_z=20.0, seed=0, temperature=300.0,
              friction=0.5, dt=0.01, mass=110.0, eps=0.2, cutoff=2.0, dh_length=1.0, dh_cutoff=3.5,
-             charged_fraction=(0.15, 0.15), bond_k=110.4):
-    """HPS slab: chains packed on a lattice in a central slab of a periodic box."""
+             charged_fraction=(0.15, 0.15), bond_k=110.4, positions=None):
+    """HPS slab: chains packed on a lattice in a central slab of a periodic box
+    (or at `positions`, when given)."""
     rng = np.random.default_rng(seed)
     n = n_chains * chain_len
-    x = lattice_chains(n_chains, chain_len, box, slab_z, rng)
+    x = lattice_chains(n_chains, chain_len, box, slab_z, rng) if positions is None else np.asarray(positions)
     sig_t, lam_t = hps_types(rng)
     types = rng.integers(0, len(sig_t), n)
     sysm = SystemSpec(masses=np.full(n, mass), box=tuple(box))
@@ -308,6 +309,42 @@ def hps_slab_of_size(n_beads, seed=0, chain_len=140):
     n_chains = max(1, int(round(n_beads / chain_len)))
     xy = max(8.0, 15.0 * float(np.sqrt(n_chains * chain_len / 14000.0)))
     return hps_slab(n_chains=n_chains, chain_len=chain_len, box=(xy, xy, 150.0), slab_z=20.0, seed=seed)
+
+
+def serpentine_chains(n_chains, chain_len, box, spacing=(0.38, 0.95, 0.95)):
+    """Chains laid along one boustrophedon path through the sites of an
+    orthorhombic lattice (spacing nm along x, y, z) that fills the x-y extent of
+    `box` and as many z layers as the beads need, centred in z.  Steps along a
+    row are spacing[0] (the bond length, so chains start unstretched); row and
+    layer turns are one spacing[1] or spacing[2] step.  No two beads share a
+    site.  Vectorised: O(N) for a million beads.  Returns (N, 3) nm."""
+    sx, sy, sz = spacing
+    n = n_chains * chain_len
+    nx, ny = int(box[0] // sx), int(box[1] // sy)
+    per_layer = nx * ny
+    nz = -(-n // per_layer)
+    if nz * sz > box[2]:
+        raise ValueError("box too small for the requested beads")
+    k = np.arange(n)
+    layer = k // per_layer
+    r = k % per_layer
+    row = r // nx
+    col = r % nx
+    col = np.where((layer * ny + row) % 2 == 1, nx - 1 - col, col)   # snake along rows
+    row = np.where(layer % 2 == 1, ny - 1 - row, row)                # and back through the next layer
+    z0 = 0.5 * (box[2] - nz * sz)
+    return np.stack([(col + 0.5) * sx, (row + 0.5) * sy, z0 + (layer + 0.5) * sz], 1).astype(np.float64)
+
+
+def hps_condensate(n_chains=1000, chain_len=1000, box=(60.0, 60.0, 600.0), seed=0, spacing=(0.38, 0.95, 0.95)):
+    """Large HPS condensate: the slab's parameter distributions (20 types,
+    sigma ~ U[0.45, 0.80] nm, lambda ~ U[0, 1], 15%/15% charges, AH cutoff
+    2.0 nm, DH 1 nm / 3.5 nm) on serpentine-packed chains in a periodic box,
+    about 2.9 beads/nm^3 in the dense layer (the 14,000-bead slab's lattice
+    start holds 3.1)."""
+    x = serpentine_chains(n_chains, chain_len, box, spacing)
+    s, it, _ = hps_slab(n_chains=n_chains, chain_len=chain_len, box=box, seed=seed, positions=x)
+    return s, it, x
 
 
 def served_job_mix(n_jobs=24, seed=0, steps=(10000, 50000), frame_interval=10000, go_fraction=0.6):

@@ -92,8 +92,10 @@ class Batch:
     def minimize(self, tolerance=10.0, max_iterations=0):
         check(self.lib.ocm_minimize(self.h, float(tolerance), int(max_iterations)))
 
-    def set_timing(self, on):
-        check(self.lib.ocm_set_timing(self.h, int(bool(on))))
+    def set_timing(self, mode):
+        """0 off, 1 (or True) HIP events around every launch, 2 device-clock
+        stamps of every pair-kernel launch inside the step graphs."""
+        check(self.lib.ocm_set_timing(self.h, int(mode)))
 
     def timing(self, kernel):
         ms = C.c_double()

@@ -64,7 +64,7 @@ void ocm_options_default(ocm_options* o) {
     if (!o) return;
     o->device = 0;
     o->rebuild_interval = 20;
-    o->skin = 0.3;
+    o->skin = 0.4;
     o->team_size = 0;
     o->use_graph = 1;
     o->chunk_steps = 200;
@@ -183,14 +183,14 @@ int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index, int32_t* out_p
 
 int ocm_set_timing(ocm_context* ctx, int32_t enable) {
     return guarded([&] {
-        eng(ctx).set_timing(enable != 0);
+        eng(ctx).set_timing(enable);
         return OCM_OK;
     });
 }
 
 int ocm_get_timing(ocm_context* ctx, int32_t kernel, double* total_ms, int64_t* launches) {
     return guarded([&] {
-        if (kernel < 0 || kernel > 4) throw ocm::Error(OCM_ERR_INVALID, "kernel id out of range");
+        if (kernel < 0 || kernel > 5) throw ocm::Error(OCM_ERR_INVALID, "kernel id out of range");
         const auto t = eng(ctx).timing(kernel);
         if (total_ms) *total_ms = t.ms;
         if (launches) *launches = t.launches;

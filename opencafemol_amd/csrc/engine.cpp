@@ -622,6 +622,39 @@ template <typename F> void Engine::timed(int slot, F&& f) {
     pending_events_.push_back({slot, {a, b}});
 }
 
+void Engine::set_timing(int mode) {
+    if (mode < 0 || mode > 2) invalid("timing mode must be 0, 1 or 2");
+    sync();
+    timing_mode_ = mode;
+    timing_ = mode == 1;
+    if (mode == 2 && wall_khz_ == 0) {
+        kstamp_red_ = dalloc<unsigned long long>(2 * static_cast<size_t>(opt_.chunk_steps));
+        int khz = 0;
+        check_hip(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, opt_.device), "wall clock rate");
+        if (khz <= 0) throw Error(OCM_ERR_HIP, "device wall clock rate unknown");
+        wall_khz_ = khz;
+    }
+}
+
+// Stamp buffer for the current pair grid; a new buffer means new kernel
+// arguments, so captured graphs are dropped.
+void Engine::ensure_stamps() {
+    const int grid = pair_grid_size(d_);
+    if (grid != kstamp_grid_) {
+        kstamp_grid_ = grid;
+        drop_graphs();
+    }
+    const long long words = 2LL * opt_.chunk_steps * std::max(grid, 1);
+    if (words <= kstamp_words_) return;
+    if (kstamp_) {
+        auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(kstamp_));
+        if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+    }
+    kstamp_ = dalloc<unsigned long long>(static_cast<size_t>(words));
+    kstamp_words_ = words;
+    drop_graphs();
+}
+
 void Engine::collect_timing() {
     for (auto& pe : pending_events_) {
         float ms = 0;
@@ -656,7 +689,9 @@ void Engine::enqueue_chunk(int k, int interval) {
         timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0, rebuild ? 0 : 1, 1); });
         if (d_.nm > 0 && rebuild) timed(3, [&] { launch_rebuild(d_, stream_); });
         timed(1, [&] { launch_bonded(d_, stream_, false, d_.f, 0xFFFFFFFFu); });
-        timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu); });
+        unsigned long long* stamp =
+            timing_mode_ == 2 ? kstamp_ + 2 * static_cast<size_t>(kstamp_grid_) * s : nullptr;
+        timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu, stamp); });
     }
     timed(2, [&] { launch_half_kick(d_, stream_, k & 1); });
 }
@@ -787,10 +822,12 @@ int Engine::run_chunk(int k) {
         check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
         check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
         const int interval = std::min(interval_, k);
+        if (timing_mode_ == 2) ensure_stamps();
         // Stretch lengths set by callers vary; graphs are kept for at most
         // kMaxGraphs shapes besides the full chunk, other shapes run eagerly.
+        // Graphs with pair-kernel stamps are keyed apart (negative interval).
         constexpr std::size_t kMaxGraphs = 8;
-        const auto key = std::make_pair(k, interval);
+        const auto key = std::make_pair(k, timing_mode_ == 2 ? -interval : interval);
         const bool graph = opt_.use_graph && !timing_ &&
                            (k == opt_.chunk_steps || graphs_.count(key) || graphs_.size() < kMaxGraphs);
         if (graph) {
@@ -826,9 +863,31 @@ int Engine::run_chunk(int k) {
             else if ((err[s] & kErrStale) && sys_status_[s] == OCM_OK) stale = true;
         }
         if (stale && interval > 1) {
+            // a bead moved more than skin/2 since the list was built: redo the
+            // chunk with half the interval, and wait twice as long as last time
+            // before trying a longer one again
             restore();
             interval_ = std::max(1, interval / 2);
+            clean_chunks_ = 0;
+            recover_after_ = std::min(recover_after_ * 2, 64);
             continue;
+        }
+        if (timing_mode_ == 2) {
+            launch_reduce_stamps(stream_, kstamp_, k, kstamp_grid_, kstamp_red_);
+            sync();
+            std::vector<unsigned long long> st(2 * static_cast<size_t>(k));
+            check_hip(hipMemcpy(st.data(), kstamp_red_, st.size() * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost), "stamps");
+            for (int q = 0; q < k; ++q) {
+                if (st[2 * q] == ~0ull || st[2 * q + 1] < st[2 * q]) continue;  // no block ran
+                timing_slots_[5].ms += static_cast<double>(st[2 * q + 1] - st[2 * q]) / wall_khz_;
+                timing_slots_[5].launches += 1;
+            }
+        }
+        if (interval_ < opt_.rebuild_interval && ++clean_chunks_ >= recover_after_) {
+            interval_ = std::min(opt_.rebuild_interval, interval_ * 2);
+            clean_chunks_ = 0;
+            recover_after_ = std::max(4, recover_after_ / 2);
         }
         return numeric ? OCM_ERR_NUMERIC : OCM_OK;
     }

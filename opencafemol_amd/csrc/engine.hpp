@@ -87,7 +87,9 @@ public:
     // (device memory, 3*nb doubles: the x block, then y, then z).
     void enqueue_position_copy(double* dst);
 
-    void set_timing(bool on) { timing_ = on; }
+    // 0 off; 1 HIP events around every launch (eager); 2 device-clock stamps
+    // of the pair kernel inside the step graphs
+    void set_timing(int mode);
     TimingSlot timing(int k) const { return timing_slots_.at(k); }
     void list_stats(long long* entries, long long* members, long long* rebuilds);
 
@@ -129,9 +131,18 @@ private:
     std::vector<int64_t> sys_step_origin_;
     bool forces_dirty_ = true;
     int interval_ = 20;
+    int clean_chunks_ = 0;      // chunks since the interval was last shortened
+    int recover_after_ = 4;     // clean chunks before the interval doubles back
     long long rebuilds_ = 0;
     bool timing_ = false;
-    std::array<TimingSlot, 5> timing_slots_{};
+    std::array<TimingSlot, 6> timing_slots_{};
+    int timing_mode_ = 0;
+    unsigned long long* kstamp_ = nullptr;   // 2 words per pair workgroup per step of a chunk
+    unsigned long long* kstamp_red_ = nullptr;  // 2 words per step of a chunk
+    long long kstamp_words_ = 0;
+    int kstamp_grid_ = 0;
+    void ensure_stamps();
+    double wall_khz_ = 0;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_events_;
     std::vector<hipEvent_t> event_pool_;
     size_t event_next_ = 0;

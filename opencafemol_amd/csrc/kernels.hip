@@ -863,9 +863,13 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     const PairRec* __restrict__ rec, const int* __restrict__ row_start,
     const int* __restrict__ row_count, const int* __restrict__ col,
     const float* __restrict__ types_f, const double* __restrict__ types_d, ForceBuf out,
-    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks) {
+    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks,
+    unsigned long long* __restrict__ stamp) {
     extern __shared__ float lds_types[];
     int loaded = -1;
+    // optional duration stamps: each workgroup stores its start and end on the
+    // device's constant-rate wall clock (plain stores to its own two words)
+    if (stamp && threadIdx.x == 0) stamp[2 * blockIdx.x] = static_cast<unsigned long long>(wall_clock64());
     // fixed grid: each workgroup walks the block table with stride gridDim.x
     for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
     const int2 bt = blocks[bi];
@@ -977,6 +981,36 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     }
     if (bad) atomicOr(&err[G.sys], kErrNumeric);
     }  // block table
+    if (stamp) {
+        __syncthreads();
+        if (threadIdx.x == 0) stamp[2 * blockIdx.x + 1] = static_cast<unsigned long long>(wall_clock64());
+    }
+}
+
+// Launch q of a chunk left nblk {start, end} pairs at stamp + 2*nblk*q; one
+// workgroup per launch reduces them to {earliest start, latest end} in out.
+__global__ __launch_bounds__(256) void k_reduce_stamps(const unsigned long long* __restrict__ stamp, int nblk,
+                                                       unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long lo[4], hi[4];
+    const unsigned long long* s = stamp + 2 * static_cast<size_t>(nblk) * blockIdx.x;
+    unsigned long long a = ~0ull, b = 0ull;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        a = min(a, s[2 * i]);
+        b = max(b, s[2 * i + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a = min(a, static_cast<unsigned long long>(__shfl_xor(a, o)));
+        b = max(b, static_cast<unsigned long long>(__shfl_xor(b, o)));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { lo[w] = a; hi[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long A = lo[0], B = hi[0];
+        for (int k = 1; k < static_cast<int>(blockDim.x >> 6); ++k) { A = min(A, lo[k]); B = max(B, hi[k]); }
+        out[2 * blockIdx.x] = A;
+        out[2 * blockIdx.x + 1] = B;
+    }
 }
 
 // ------------------------------------------------------------- energies
@@ -1118,13 +1152,14 @@ void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf
 }
 
 template <bool E>
-static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, uint32_t groups) {
+static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, uint32_t groups,
+                          unsigned long long* stamp) {
     const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
     const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
 #define OCM_PAIR(T)                                                                                   \
     k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, \
                                            d.col, d.types_f, d.types_d, f, d.energy, groups, d.err,   \
-                                           d.n_pair_blocks)
+                                           d.n_pair_blocks, stamp)
     switch (d.team) {
     case 4: OCM_PAIR(4); break;
     case 8: OCM_PAIR(8); break;
@@ -1135,10 +1170,18 @@ static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, u
 #undef OCM_PAIR
 }
 
-void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {
+void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups,
+                 unsigned long long* stamp) {
     if (d.n_pair_blocks == 0) return;
-    if (energy) pair_dispatch<true>(d, st, f, groups);
-    else pair_dispatch<false>(d, st, f, groups);
+    if (energy) pair_dispatch<true>(d, st, f, groups, stamp);
+    else pair_dispatch<false>(d, st, f, groups, stamp);
+}
+
+int pair_grid_size(const DevPtrs& d) { return std::min(d.n_pair_blocks, d.pair_grid); }
+
+void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int nlaunch, int nblk,
+                          unsigned long long* out) {
+    if (nlaunch > 0) k_reduce_stamps<<<nlaunch, 256, 0, st>>>(stamp, nblk, out);
 }
 
 void launch_zero_energy(const DevPtrs& d, hipStream_t st) {

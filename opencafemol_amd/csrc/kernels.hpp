@@ -71,7 +71,13 @@ void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur);
 void launch_pack(const DevPtrs& d, hipStream_t st);
 void launch_rebuild(const DevPtrs& d, hipStream_t st);
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups);
-void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups);
+// stamp (optional): 2 * pair_grid_size words; workgroup w stores its start
+// and end on the device wall clock at stamp[2w], stamp[2w + 1].
+void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups,
+                 unsigned long long* stamp = nullptr);
+int pair_grid_size(const DevPtrs& d);
+void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int nlaunch, int nblk,
+                          unsigned long long* out);
 void launch_zero_energy(const DevPtrs& d, hipStream_t st);
 void launch_kinetic(const DevPtrs& d, hipStream_t st);
 void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members);
