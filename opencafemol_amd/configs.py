@@ -68,15 +68,24 @@ def _unit(rng, n):
     return v / np.linalg.norm(v, axis=1, keepdims=True)
 
 
-def saw_chain(n, rng, bond=0.38, min_dist=0.40, start=(0.0, 0.0, 0.0)):
+def saw_chain(n, rng, bond=0.38, min_dist=0.40, start=(0.0, 0.0, 0.0), angle_range=(75.0, 140.0)):
     """Self-avoiding random walk of n beads (nm): every new bead is `bond` from
-    the previous one and at least `min_dist` from all earlier beads."""
+    the previous one, at least `min_dist` from all earlier beads, and makes a
+    bond angle within `angle_range` (degrees) at the previous bead, the span of
+    C-alpha pseudo-bond angles.  Native angles far from 180 degrees keep the
+    chain away from collinear triples, where dihedral gradients diverge."""
+    cos_hi, cos_lo = np.cos(np.radians(angle_range[0])), np.cos(np.radians(angle_range[1]))
     x = np.zeros((n, 3))
     x[0] = start
     i = 1
     while i < n:
         for _ in range(200):
             cand = x[i - 1] + bond * _unit(rng, 1)[0]
+            if i >= 2:
+                u, w = x[i - 2] - x[i - 1], cand - x[i - 1]
+                ca = np.dot(u, w) / (np.linalg.norm(u) * np.linalg.norm(w))
+                if not (cos_lo <= ca <= cos_hi):
+                    continue
             if i < 2 or np.min(np.linalg.norm(x[: i - 1] - cand, axis=1)) >= min_dist:
                 x[i] = cand
                 i += 1
@@ -114,9 +123,16 @@ def chain_exclusions(chains, within):
 
 
 def go_chain(n=62, seed=0, temperature=300.0, friction=0.5, dt=0.01, charged_fraction=0.2,
-             mass=110.0, angle_k=10.0, contact_eps=0.3, exv_eps=0.6, exv_radius=2.0,
+             mass=110.0, angle_k=20.0, contact_eps=0.3, exv_eps=0.6, exv_radius=2.0,
              dh_length=1.0, dh_cutoff_ratio=3.0, exv_ratio=2.0):
-    """A Go-model Calpha chain grown as a seeded self-avoiding walk."""
+    """A Go-model Calpha chain grown as a seeded self-avoiding walk.
+
+    Native angles are held in [75, 140] degrees (saw_chain) and the angle
+    stiffness is the Clementi-type Go value, 20 kcal/mol/rad^2.  An
+    unconstrained walk has native angles near 180 degrees; replicas near 400 K
+    then reach collinear triples within ~10,000 steps, where the Gaussian
+    dihedral's gradient (which scales as 1/sin(angle)) ejects a bead and the
+    chain explodes."""
     rng = np.random.default_rng(seed)
     x = saw_chain(n, rng)
     sysm = SystemSpec(masses=np.full(n, mass))

@@ -146,6 +146,9 @@ def test_go_chain_config_properties():
     d = np.linalg.norm(x[:, None] - x[None], axis=2)
     iu = np.triu_indices(62, k=2)
     assert d[iu].min() >= 0.4 - 1e-12
+    a, c = x[:-2] - x[1:-1], x[2:] - x[1:-1]
+    ang = np.degrees(np.arccos(np.einsum("ij,ij->i", a, c) / np.linalg.norm(a, axis=1) / np.linalg.norm(c, axis=1)))
+    assert ang.min() >= 75.0 - 1e-9 and ang.max() <= 140.0 + 1e-9
     go = [f for f in s.forces if f.kind == abi.GO_CONTACT][0]
     assert np.all(go.indices[:, 1] - go.indices[:, 0] >= 4)
     assert np.all(go.params[:, 1] < 0.65)
