@@ -18,7 +18,7 @@
 //   fla_min_energy   FlexibleLocalAngleForceFieldGenerator.cpp:11-19,47-68
 //   fld_min_energy   FlexibleLocalDihedralForceFieldGenerator.cpp:7-26
 //   natural_spline   OpenMM Continuous1DFunction (natural cubic spline fit)
-//   baoab            the Langevin step of BASELINE's north star (BAOAB splitting
+//   baoab            the Langevin step this library implements (BAOAB splitting
 //                    with Philox4x32-10 noise); the reference's own integrator is
 //                    OpenMM's LangevinIntegrator (IntegratorGenerator.cpp:5-14),
 //                    whose RNG stream cannot be reproduced without OpenMM.
