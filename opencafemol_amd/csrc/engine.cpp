@@ -516,6 +516,7 @@ void Engine::pack() {
     bstep_ = dalloc<long long>(2);
     d_.sys = dalloc<DevSys>(nsys);
     upload(d_.sys, dsys);
+    d_.sys0 = dsys[0];
     d_.sys_of = dalloc<int>(nb); upload(d_.sys_of, sys_of);
     d_.local_id = dalloc<int>(nb); upload(d_.local_id, local);
     d_.inv_m = dalloc<double>(nb); upload(d_.inv_m, inv_m);
@@ -548,14 +549,27 @@ void Engine::pack() {
     d_.member_cell = dalloc<int>(nm);
     d_.member_rank = dalloc<int>(nm);
     {
-        std::vector<int> cnt(nb, 0), start(nb, 0), lst(nm);
+        // fixed-stride per-bead member tables, so the integrator reaches a
+        // bead's records with one dependent load
+        std::vector<int> cnt(nb, 0), kpos(std::max(nm, 1), 0);
         for (int m = 0; m < nm; ++m) ++cnt[member_bead[m]];
-        for (int b = 1; b < nb; ++b) start[b] = start[b - 1] + cnt[b - 1];
-        std::vector<int> fill(start);
-        for (int m = 0; m < nm; ++m) lst[fill[member_bead[m]]++] = m;
-        d_.bead_mem_start = dalloc<int>(nb); upload(d_.bead_mem_start, start);
+        int maxm = 1;
+        for (int b = 0; b < nb; ++b) maxm = std::max(maxm, cnt[b]);
+        d_.maxm = maxm;
+        std::vector<int> type(static_cast<size_t>(nb) * maxm, 0), fillc(nb, 0);
+        std::vector<double> hs2(static_cast<size_t>(nb) * maxm, 0.0);
+        for (int m = 0; m < nm; ++m) {
+            const int b = member_bead[m];
+            const int k = b * maxm + fillc[b]++;
+            kpos[m] = k;
+            type[k] = member_type[m];
+            hs2[k] = groups[member_group[m]].half_skin2;
+        }
         d_.bead_mem_count = dalloc<int>(nb); upload(d_.bead_mem_count, cnt);
-        d_.bead_mem = dalloc<int>(nm); upload(d_.bead_mem, lst);
+        d_.bead_slot = dalloc<int>(static_cast<size_t>(nb) * maxm);
+        d_.bead_type = dalloc<int>(type.size()); upload(d_.bead_type, type);
+        d_.bead_hs2 = dalloc<double>(hs2.size()); upload(d_.bead_hs2, hs2);
+        d_.member_kpos = dalloc<int>(kpos.size()); upload(d_.member_kpos, kpos);
     }
     d_.excl_start = dalloc<int>(nm); upload(d_.excl_start, excl_start);
     d_.excl_count = dalloc<int>(nm); upload(d_.excl_count, excl_count);
@@ -594,6 +608,8 @@ void Engine::build_pair_blocks(int team) {
     d_.team = team;
     const int rpb = kPairBlock / team;
     std::vector<int2> blocks;
+    // bonded-term chunks first: latency-bound, they start while the pair rows run
+    for (int t = 0; t < d_.nt; t += kPairBlock) blocks.push_back(make_int2(-1, t));
     for (int g = 0; g < d_.ngroups; ++g) {
         for (int r = 0; r < group_member_count_[g]; r += rpb) blocks.push_back(make_int2(g, group_member_begin_[g] + r));
     }
@@ -678,8 +694,7 @@ void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
         if (rebuild) { timed(3, [&] { launch_rebuild(d_, stream_); }); ++rebuilds_; }
         else timed(4, [&] { launch_pack(d_, stream_); });
     }
-    timed(1, [&] { launch_bonded(d_, stream_, false, f, groups); });
-    timed(0, [&] { launch_pair(d_, stream_, false, f, groups); });
+    timed(0, [&] { launch_pair(d_, stream_, false, f, groups); });   // bonded terms included
 }
 
 void Engine::enqueue_chunk(int k, int interval) {
@@ -688,9 +703,9 @@ void Engine::enqueue_chunk(int k, int interval) {
         // the integrator refreshes the pair records itself on non-rebuild steps
         timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0, rebuild ? 0 : 1, 1); });
         if (d_.nm > 0 && rebuild) timed(3, [&] { launch_rebuild(d_, stream_); });
-        timed(1, [&] { launch_bonded(d_, stream_, false, d_.f, 0xFFFFFFFFu); });
         unsigned long long* stamp =
             timing_mode_ == 2 ? kstamp_ + 2 * static_cast<size_t>(kstamp_grid_) * s : nullptr;
+        // bonded terms run inside the pair kernel's force pass
         timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu, stamp); });
     }
     timed(2, [&] { launch_half_kick(d_, stream_, k & 1); });
@@ -972,6 +987,8 @@ void Engine::set_steps_done(int s, long long steps_done) {
     check_hip(hipMemcpy(&D, d_.sys + s, sizeof(DevSys), hipMemcpyDeviceToHost), "sys");
     D.step_origin = sys_step_origin_[s];
     check_hip(hipMemcpy(d_.sys + s, &D, sizeof(DevSys), hipMemcpyHostToDevice), "sys");
+    if (s == 0) d_.sys0 = D;
+    drop_graphs();   // captured integrator launches carry system 0's record by value
 }
 
 // ------------------------------------------------------------ state I/O

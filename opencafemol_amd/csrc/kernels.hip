@@ -75,32 +75,32 @@ __global__ void k_fill_i64(long long* p, long long v, int n) {
 struct PackArgs {
     int enabled;
     int check;
-    const int* bead_mem_start;
+    int maxm;
     const int* bead_mem_count;
-    const int* bead_mem;
-    const DevPairGroup* groups;
-    const int* member_group;
-    const int* member_type;
-    const int* member_slot;
+    const int* bead_slot;
+    const int* bead_type;
+    const double* bead_hs2;
     const double* refx;
     const double* refy;
     const double* refz;
     PairRec* rec;
 };
 
-__device__ inline void pack_bead(const PackArgs& pk, int b, double x, double y, double z, int* err) {
-    const int k0 = pk.bead_mem_start[b], k1 = k0 + pk.bead_mem_count[b];
-    for (int k = k0; k < k1; ++k) {
-        const int m = pk.bead_mem[k];
+// Refreshes bead b's pair records and, when asked, checks that it has not
+// moved more than skin/2 since the list was built.
+__device__ inline void pack_bead(const PackArgs& pk, int b, int sys, double x, double y, double z, int* err) {
+    const int cnt = pk.bead_mem_count[b];
+    for (int w = 0; w < cnt; ++w) {
+        const int k = b * pk.maxm + w;
+        const int slot = pk.bead_slot[k];
         PairRec r;
         r.x = x; r.y = y; r.z = z;
-        r.type = pk.member_type[m];
+        r.type = pk.bead_type[k];
         r.bead = b;
-        pk.rec[pk.member_slot[m]] = r;
+        pk.rec[slot] = r;
         if (pk.check) {
-            const DevPairGroup& G = pk.groups[pk.member_group[m]];
-            const double dx = x - pk.refx[m], dy = y - pk.refy[m], dz = z - pk.refz[m];
-            if (!(dist2(dx, dy, dz) <= G.half_skin2)) atomicOr(&err[G.sys], kErrStale);
+            const double dx = x - pk.refx[slot], dy = y - pk.refy[slot], dz = z - pk.refz[slot];
+            if (!(dist2(dx, dy, dz) <= pk.bead_hs2[k])) atomicOr(&err[sys], kErrStale);
         }
     }
 }
@@ -123,18 +123,19 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
                             double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
                             long long* __restrict__ fx, long long* __restrict__ fy,
                             long long* __restrict__ fz, long long* gstep, int pin, int pout,
-                            int pending, int* __restrict__ err, PackArgs pk) {
+                            int pending, int* __restrict__ err, PackArgs pk, DevSys sys0, int single) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const long long step = gstep[pin];
     if (b == 0) gstep[pout] = step + 1;
     if (b >= nb) return;
-    const int s = sys_of[b];
-    const DevSys S = sys[s];
+    // a batch of one passes its system record by value: one dependent load less
+    const int s = single ? 0 : sys_of[b];
+    const DevSys S = single ? sys0 : sys[s];
     const double im = inv_m[b];
     const double f0 = fx[b] * kFixInv, f1 = fy[b] * kFixInv, f2 = fz[b] * kFixInv;
     fx[b] = 0; fy[b] = 0; fz[b] = 0;
     if (im == 0.0) {
-        if (pk.enabled) pack_bead(pk, b, px[b], py[b], pz[b], err);
+        if (pk.enabled) pack_bead(pk, b, s, px[b], py[b], pz[b], err);
         return;
     }
     double v0 = vx[b], v1 = vy[b], v2 = vz[b];
@@ -161,7 +162,7 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) atomicOr(&err[s], kErrNumeric);
     px[b] = x0; py[b] = x1; pz[b] = x2;
     vx[b] = v0; vy[b] = v1; vz[b] = v2;
-    if (pk.enabled) pack_bead(pk, b, x0, x1, x2, err);
+    if (pk.enabled) pack_bead(pk, b, s, x0, x1, x2, err);
 }
 
 // Closing half kick; also resets both global-step slots to the value reached.
@@ -196,13 +197,14 @@ __global__ void k_pack(int nm, const DevPairGroup* __restrict__ groups,
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nm) return;
     const int b = member_bead[m];
+    const int slot = member_slot[m];
     PairRec r;
     r.x = px[b]; r.y = py[b]; r.z = pz[b];
     r.type = member_type[m];
     r.bead = b;
-    rec[member_slot[m]] = r;
+    rec[slot] = r;
     const DevPairGroup& G = groups[member_group[m]];
-    const double dx = r.x - refx[m], dy = r.y - refy[m], dz = r.z - refz[m];
+    const double dx = r.x - refx[slot], dy = r.y - refy[slot], dz = r.z - refz[slot];
     if (!(dist2(dx, dy, dz) <= G.half_skin2)) atomicOr(&err[G.sys], kErrStale);
 }
 
@@ -250,6 +252,7 @@ __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* 
                         const int* __restrict__ cell_start, const double* __restrict__ px,
                         const double* __restrict__ py, const double* __restrict__ pz,
                         int* __restrict__ member_slot, int* __restrict__ slot_member,
+                        const int* __restrict__ member_kpos, int* __restrict__ bead_slot,
                         PairRec* __restrict__ rec, double* __restrict__ refx,
                         double* __restrict__ refy, double* __restrict__ refz) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,13 +260,14 @@ __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* 
     const int slot = cell_start[member_cell[m]] + member_rank[m];
     member_slot[m] = slot;
     slot_member[slot] = m;
+    bead_slot[member_kpos[m]] = slot;
     const int b = member_bead[m];
     PairRec r;
     r.x = px[b]; r.y = py[b]; r.z = pz[b];
     r.type = member_type[m];
     r.bead = b;
     rec[slot] = r;
-    refx[m] = r.x; refy[m] = r.y; refz[m] = r.z;
+    refx[slot] = r.x; refy[slot] = r.y; refz[slot] = r.z;
 }
 
 __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, int mj) {
@@ -857,6 +861,20 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
     *dedr_r = f;
 }
 
+// Bonded terms folded into the force pass of k_pair (block-table entries
+// {-1, first term}).
+struct BondedArgs {
+    int nt;
+    const BTerm* terms;
+    const DevSys* sys;
+    const int* sys_of;
+    const double* px;
+    const double* py;
+    const double* pz;
+    const FlaTable* fla;
+    const FldTable* fld;
+};
+
 template <bool ENERGY, int TEAM>
 __global__ __launch_bounds__(kPairBlock) void k_pair(
     const DevPairGroup* __restrict__ groups, const int2* __restrict__ blocks,
@@ -864,7 +882,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     const int* __restrict__ row_count, const int* __restrict__ col,
     const float* __restrict__ types_f, const double* __restrict__ types_d, ForceBuf out,
     long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks,
-    unsigned long long* __restrict__ stamp) {
+    unsigned long long* __restrict__ stamp, BondedArgs bd) {
     extern __shared__ float lds_types[];
     int loaded = -1;
     // optional duration stamps: each workgroup stores its start and end on the
@@ -873,6 +891,24 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     // fixed grid: each workgroup walks the block table with stride gridDim.x
     for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
     const int2 bt = blocks[bi];
+    if (bt.x < 0) {
+        // a chunk of bonded terms, one per thread (force pass only): their
+        // gathers and atomics overlap the pair rows of the other workgroups
+        if (!ENERGY) {
+            const int t = bt.y + static_cast<int>(threadIdx.x);
+            if (t < bd.nt) {
+                const BTerm T = bd.terms[t];
+                if ((gmask >> (T.meta & 31)) & 1u) {
+                    const int s = bd.sys_of[T.idx[0]];
+                    int bad = 0;
+                    bonded_term<false, float>(T, bd.sys[s], bd.px, bd.py, bd.pz, bd.fla, bd.fld,
+                                              ForceOut{out.fx, out.fy, out.fz}, energy, s, bad);
+                    if (bad) atomicOr(&err[s], kErrNumeric);
+                }
+            }
+        }
+        continue;
+    }
     const DevPairGroup& G = groups[bt.x];
     const bool periodic = G.periodic;
     const double bx = G.box[0], by = G.box[1], bz = G.box[2];
@@ -1098,11 +1134,11 @@ void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st) {
 }
 
 void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check) {
-    PackArgs pk{pack && d.nm > 0, check, d.bead_mem_start, d.bead_mem_count, d.bead_mem, d.groups,
-                d.member_group, d.member_type, d.member_slot, d.refx, d.refy, d.refz, d.rec};
+    PackArgs pk{pack && d.nm > 0, check, d.maxm, d.bead_mem_count, d.bead_slot, d.bead_type, d.bead_hs2,
+                d.refx, d.refy, d.refz, d.rec};
     k_integrate<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
                                                  d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
-                                                 pin, pout, pending, d.err, pk);
+                                                 pin, pout, pending, d.err, pk, d.sys0, d.nsys == 1 ? 1 : 0);
 }
 
 void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur) {
@@ -1132,7 +1168,7 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     launch_scan(d, st, d.cell_count, d.cell_start, d.ncell, d.scan_total);
     k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
                                              d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
-                                             d.rec, d.refx, d.refy, d.refz);
+                                             d.member_kpos, d.bead_slot, d.rec, d.refx, d.refy, d.refz);
     k_fill_i32<<<1, 64, 0, st>>>(d.tile_nitems, 0, 1);
     k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_count, d.tile_items, d.tile_nitems);
     k_nl_tiles<<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group, d.cell_start,
@@ -1155,11 +1191,12 @@ template <bool E>
 static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, uint32_t groups,
                           unsigned long long* stamp) {
     const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
+    const BondedArgs bd{d.nt, d.terms, d.sys, d.sys_of, d.px, d.py, d.pz, d.fla, d.fld};
     const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
 #define OCM_PAIR(T)                                                                                   \
     k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, \
                                            d.col, d.types_f, d.types_d, f, d.energy, groups, d.err,   \
-                                           d.n_pair_blocks, stamp)
+                                           d.n_pair_blocks, stamp, bd)
     switch (d.team) {
     case 4: OCM_PAIR(4); break;
     case 8: OCM_PAIR(8); break;

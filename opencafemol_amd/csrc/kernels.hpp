@@ -17,6 +17,7 @@ struct ForceBuf {
 // nt bonded terms, nm pair-group members (= slots), ncell cells.
 struct DevPtrs {
     int nb = 0, nsys = 0, nt = 0, nm = 0, ncell = 0, ngroups = 0;
+    DevSys sys0{};                 // host copy of system 0's record (kernel argument when nsys == 1)
     DevSys* sys = nullptr;
     int* sys_of = nullptr;
     int* local_id = nullptr;
@@ -36,9 +37,16 @@ struct DevPtrs {
     DevPairGroup* groups = nullptr;
     int *member_group = nullptr, *member_bead = nullptr, *member_type = nullptr, *member_cat = nullptr;
     int *member_slot = nullptr, *member_cell = nullptr, *member_rank = nullptr;
-    int *bead_mem_start = nullptr, *bead_mem_count = nullptr, *bead_mem = nullptr;  // bead -> members
+    // bead -> its pair-group members, `maxm` slots per bead: count, current
+    // record slot, type word and (skin/2)^2 of the member's group
+    int maxm = 1;
+    int* bead_mem_count = nullptr;
+    int* bead_slot = nullptr;
+    int* bead_type = nullptr;
+    double* bead_hs2 = nullptr;
+    int* member_kpos = nullptr;   // member -> its position in the bead tables
     int *excl_start = nullptr, *excl_count = nullptr, *excl = nullptr;
-    double *refx = nullptr, *refy = nullptr, *refz = nullptr;
+    double *refx = nullptr, *refy = nullptr, *refz = nullptr;   // positions at the last build, by slot
     PairRec* rec = nullptr;
     int* slot_member = nullptr;
     int *cell_count = nullptr, *cell_start = nullptr, *cell_group = nullptr;
