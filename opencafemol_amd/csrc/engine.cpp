@@ -255,6 +255,7 @@ void Engine::pack() {
     std::vector<double> inv_m(nb), mass(nb);
     std::vector<DevSys> dsys(nsys);
     sys_status_.assign(nsys, OCM_OK);
+    err_carry_.assign(nsys, 0);
     sys_step_origin_.assign(nsys, 0);
     sys_group_mask_.assign(nsys, 0);
     const double kB = 1.380649e-23 * 6.02214076e23 / 1000.0;  // kJ/mol/K
@@ -773,10 +774,22 @@ void Engine::grow_rows() {
     drop_graphs();
 }
 
+// Device error words are cleared after every read.  Flags raised by a force
+// evaluation outside a chunk (the one that starts stepping) are carried on the
+// host and reported with the next accepted chunk.
+void Engine::carry_errors() {
+    sync();
+    std::vector<int> err(d_.nsys);
+    check_hip(hipMemcpy(err.data(), d_.err, sizeof(int) * d_.nsys, hipMemcpyDeviceToHost), "err");
+    for (int s = 0; s < d_.nsys; ++s) err_carry_[s] |= err[s] & kErrNumeric;
+    check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
+}
+
 void Engine::ensure_forces() {
     if (!forces_dirty_) return;
     for (int attempt = 0; attempt < 8; ++attempt) {
         check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
+        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
         launch_zero_forces(d_, d_.f, stream_);
         enqueue_forces(d_.f, true, 0xFFFFFFFFu);
         sync();
@@ -787,7 +800,12 @@ void Engine::ensure_forces() {
             continue;
         }
         forces_dirty_ = false;
-        if (!team_tuned_) autotune_team();
+        if (!team_tuned_) {
+            autotune_team();   // clears the flags its scratch launches raised
+            launch_zero_forces(d_, d_.f, stream_);
+            enqueue_forces(d_.f, false, 0xFFFFFFFFu);   // again, with the tuned launch shape
+        }
+        carry_errors();
         return;
     }
     throw Error(OCM_ERR_OOM, "neighbor list did not fit after repeated growth");
@@ -835,7 +853,6 @@ int Engine::run_chunk(int k) {
     for (int attempt = 0; attempt < 16; ++attempt) {
         snapshot();
         check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
-        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
         const int interval = std::min(interval_, k);
         if (timing_mode_ == 2) ensure_stamps();
         // Stretch lengths set by callers vary; graphs are kept for at most
@@ -867,16 +884,15 @@ int Engine::run_chunk(int k) {
         std::vector<int> err(d_.nsys);
         check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
         check_hip(hipMemcpy(err.data(), d_.err, sizeof(int) * d_.nsys, hipMemcpyDeviceToHost), "err");
+        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
         if (flags & kFlagListOverflow) {
             restore();
             grow_rows();
             continue;
         }
         bool stale = false, numeric = false;
-        for (int s = 0; s < d_.nsys; ++s) {
-            if (err[s] & kErrNumeric) { sys_status_[s] = OCM_ERR_NUMERIC; numeric = true; }
-            else if ((err[s] & kErrStale) && sys_status_[s] == OCM_OK) stale = true;
-        }
+        for (int s = 0; s < d_.nsys; ++s)
+            if (!(err[s] & kErrNumeric) && (err[s] & kErrStale) && sys_status_[s] == OCM_OK) stale = true;
         if (stale && interval > 1) {
             // a bead moved more than skin/2 since the list was built: redo the
             // chunk with half the interval, and wait twice as long as last time
@@ -886,6 +902,11 @@ int Engine::run_chunk(int k) {
             clean_chunks_ = 0;
             recover_after_ = std::min(recover_after_ * 2, 64);
             continue;
+        }
+        // accepted: report this chunk's flags and those carried from before it
+        for (int s = 0; s < d_.nsys; ++s) {
+            if ((err[s] | err_carry_[s]) & kErrNumeric) { sys_status_[s] = OCM_ERR_NUMERIC; numeric = true; }
+            err_carry_[s] = 0;
         }
         if (timing_mode_ == 2) {
             launch_reduce_stamps(stream_, kstamp_, k, kstamp_grid_, kstamp_red_);
@@ -966,7 +987,11 @@ int Engine::minimize(double tol, int max_iter) {
             enqueue_forces(d_.f, true, 0xFFFFFFFFu);
         }
     }
+    // relaxation may pass through flagged overlaps; stepping starts from a
+    // fresh evaluation at the relaxed positions
     check_hip(hipMemset(d_.err, 0, sizeof(int) * d_.nsys), "memset");
+    std::fill(err_carry_.begin(), err_carry_.end(), 0);
+    forces_dirty_ = true;
     return it;
 }
 
@@ -1051,6 +1076,8 @@ void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
             src = d_.f2;
         }
         sync();
+        // a read of forces raises no step error (the full evaluation's flags are carried)
+        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
         std::vector<long long> fx(H.n), fy(H.n), fz(H.n);
         check_hip(hipMemcpy(fx.data(), src.fx + off, H.n * sizeof(long long), hipMemcpyDeviceToHost), "forces");
         check_hip(hipMemcpy(fy.data(), src.fy + off, H.n * sizeof(long long), hipMemcpyDeviceToHost), "forces");
@@ -1068,6 +1095,7 @@ void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
         launch_pair(d_, stream_, true, d_.f2, groups);
         launch_kinetic(d_, stream_);
         sync();
+        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");   // energies raise no step error
         int flags = 0;
         check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
         if (flags & kFlagListOverflow) {

@@ -98,6 +98,7 @@ private:
     void alloc_device();
     void build_pair_blocks(int team);
     void ensure_forces();
+    void carry_errors();
     void enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups);
     void enqueue_chunk(int k, int interval);
     int run_chunk(int k);
@@ -128,6 +129,7 @@ private:
     std::vector<long long> group_list_estimate_;
     std::vector<uint32_t> sys_group_mask_;
     std::vector<int> sys_status_;
+    std::vector<int> err_carry_;     // flags of evaluations outside a chunk, reported with the next one
     std::vector<int64_t> sys_step_origin_;
     bool forces_dirty_ = true;
     int interval_ = 20;

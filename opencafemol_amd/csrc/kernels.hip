@@ -861,6 +861,16 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
     *dedr_r = f;
 }
 
+// Pair contributions to fixed point.  fma(g * 2^32, d, 1.5 * 2^52) rounds the
+// exact product g*d*2^32 to the nearest integer (ties to even) as long as
+// |g d| < 2^19; that integer is the double's bit pattern minus the constant's.
+// Lanes add the raw bit patterns (64-bit, wrapping) and remove n constants at
+// the end.  Rounding to nearest even is odd-symmetric, so pair (j, i) adds
+// exactly the negation of what pair (i, j) adds.
+constexpr double kFixMagic = 6755399441055744.0;          // 1.5 * 2^52
+constexpr unsigned long long kFixMagicBits = 0x4338000000000000ull;
+constexpr float kPairLimit = 524288.0f;                   // 2^19 kJ/mol/nm per pair
+
 // Bonded terms folded into the force pass of k_pair (block-table entries
 // {-1, first term}).
 struct BondedArgs {
@@ -875,8 +885,9 @@ struct BondedArgs {
     const FldTable* fld;
 };
 
+// At least 5 waves per SIMD (<= 96 VGPRs): the record gathers need the latency hiding.
 template <bool ENERGY, int TEAM>
-__global__ __launch_bounds__(kPairBlock) void k_pair(
+__global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_pair(
     const DevPairGroup* __restrict__ groups, const int2* __restrict__ blocks,
     const PairRec* __restrict__ rec, const int* __restrict__ row_start,
     const int* __restrict__ row_count, const int* __restrict__ col,
@@ -930,7 +941,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
     const int lane = threadIdx.x % TEAM;
     const int row = bt.y + threadIdx.x / TEAM;
     const int row_end = G.member_begin + G.member_count;
-    long long ax = 0, ay = 0, az = 0;
+    unsigned long long ux = 0, uy = 0, uz = 0, nadd = 0;   // raw fma bit patterns, force pass
     long long ae0 = 0, ae1 = 0, ae2 = 0, ae3 = 0;
     int bad = 0;
     int bead_i = -1;
@@ -980,14 +991,20 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
                     const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
                     const float r2f = static_cast<float>(r2);
                     pair_eval<float>(kinds, Kf, tfi, tfj, nullptr, nullptr, r2f, &f, e);
-                    // |f d| <= |f| r: one range test covers the three components
-                    // (and fails for NaN); a failing pair contributes nothing
-                    const bool ok = f * f * r2f < kFixLimitF * kFixLimitF;
+                    // |f d| <= |f| r: one range test covers the three
+                    // components.  A pair beyond the limit is flagged and
+                    // clamped to it along its direction (so relaxation still
+                    // pushes the beads apart); NaN contributes nothing.
+                    const bool ok = f * f * r2f < kPairLimit * kPairLimit;
                     bad |= !ok;
-                    const float g = ok ? f : 0.0f;
-                    ax += fix_floor(g * static_cast<float>(dx));
-                    ay += fix_floor(g * static_cast<float>(dy));
-                    az += fix_floor(g * static_cast<float>(dz));
+                    const float g = ok ? f
+                                       : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f)
+                                                 : 0.0f);
+                    const double gs = static_cast<double>(g) * 4294967296.0;
+                    ux += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic)));
+                    uy += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic)));
+                    uz += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic)));
+                    ++nadd;
                 }
             }
         }
@@ -1006,6 +1023,10 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(
                               static_cast<unsigned long long>(v[k]));
         }
     } else {
+        const unsigned long long off = nadd * kFixMagicBits;
+        long long ax = static_cast<long long>(ux - off);
+        long long ay = static_cast<long long>(uy - off);
+        long long az = static_cast<long long>(uz - off);
         ax = warp_sum(ax, TEAM);
         ay = warp_sum(ay, TEAM);
         az = warp_sum(az, TEAM);
