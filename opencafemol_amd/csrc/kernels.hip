@@ -271,7 +271,7 @@ __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* 
 }
 
 __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, int mj) {
-    // exclusion rows are sorted by partner member index
+    // exclusion rows hold partner bead ids, sorted
     int lo = beg, hi = beg + cnt;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -311,6 +311,17 @@ __global__ void k_tile_items(int ncell, const int* __restrict__ cell_count, int2
 // slots writes only what fits, reports its length through group_need and the
 // overflow flag, and gets row_count 0; the host then widens the rows of that
 // group and builds again.
+//
+// The list only has to hold every pair within rc + skin (the pair kernel
+// applies the exact FP64 cutoff, and the neighbor-pair dump its own FP64
+// test), so candidates are tested in FP32 against (rc + skin)^2 (1 + 1e-3):
+// a superset.  Candidates are staged as FP32 positions relative to the item's
+// first row, already shifted to the image the stencil cell stands for; on a
+// periodic axis with a single cell the image is chosen per pair instead.
+constexpr float kListMargin = 1.001f;
+
+__device__ __forceinline__ double wrap_coord(double x, double L, double invL) { return x - L * floor(x * invL); }
+
 __global__ __launch_bounds__(256) void k_nl_tiles(
     const int2* __restrict__ items, const int* __restrict__ n_items, const DevPairGroup* __restrict__ groups,
     const int* __restrict__ cell_group,
@@ -318,10 +329,11 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
     int* __restrict__ group_need, int* __restrict__ col, int* __restrict__ flags) {
-    __shared__ PairRec cand[kTileCap];
-    __shared__ int cand_slot[kTileCap];
+    __shared__ float4 cpos[kTileCap];      // x, y, z relative to the item's first row; w = slot (bits)
+    __shared__ int2 cinfo[kTileCap];       // {type word, bead}
     __shared__ int row_run[kTileChunk];
     __shared__ int st_start[kMaxStencil], st_pre[kMaxStencil + 1];
+    __shared__ float st_shift[kMaxStencil][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
     const int nit = *n_items;
     for (int item = blockIdx.x; item < nit; item += gridDim.x) {
@@ -332,21 +344,40 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     const int gid = cell_group[home];
     const DevPairGroup& G = groups[gid];
     const int cap = G.stride;
-    const int cell = home - G.cell_begin;
-    const int cx = cell % G.ncell[0];
-    const int cy = (cell / G.ncell[0]) % G.ncell[1];
-    const int cz = cell / (G.ncell[0] * G.ncell[1]);
-    const int w0 = 2 * G.reach[0] + 1, w1 = 2 * G.reach[1] + 1, w2 = 2 * G.reach[2] + 1;
+    const bool periodic = G.periodic;
+    const double L[3] = {G.box[0], G.box[1], G.box[2]};
+    const double iL[3] = {G.inv_box[0], G.inv_box[1], G.inv_box[2]};
+    const int nc[3] = {G.ncell[0], G.ncell[1], G.ncell[2]};
+    const int rch[3] = {G.reach[0], G.reach[1], G.reach[2]};
+    // per-pair minimum image on periodic axes that have one cell
+    const bool mi0 = periodic && nc[0] == 1, mi1 = periodic && nc[1] == 1, mi2 = periodic && nc[2] == 1;
+    const bool any_mi = mi0 || mi1 || mi2;
+    const float Lf0 = static_cast<float>(L[0]), Lf1 = static_cast<float>(L[1]), Lf2 = static_cast<float>(L[2]);
+    const float iLf0 = static_cast<float>(iL[0]), iLf1 = static_cast<float>(iL[1]), iLf2 = static_cast<float>(iL[2]);
+    const float rl2f = static_cast<float>(G.rl2) * kListMargin;
+    const int cbeg = G.cell_begin;
+    // origin: the item's first row, wrapped into the box on periodic axes
+    const PairRec r0 = rec[h0];
+    const double o0 = periodic ? wrap_coord(r0.x, L[0], iL[0]) : r0.x;
+    const double o1 = periodic ? wrap_coord(r0.y, L[1], iL[1]) : r0.y;
+    const double o2 = periodic ? wrap_coord(r0.z, L[2], iL[2]) : r0.z;
+    const int cell = home - cbeg;
+    const int cx = cell % nc[0];
+    const int cy = (cell / nc[0]) % nc[1];
+    const int cz = cell / (nc[0] * nc[1]);
+    const int w0 = 2 * rch[0] + 1, w1 = 2 * rch[1] + 1, w2 = 2 * rch[2] + 1;
     const int nst = w0 * w1 * w2;
     if (tid < nst) {
-        const int dx = tid % w0 - G.reach[0], dy = (tid / w0) % w1 - G.reach[1], dz = tid / (w0 * w1) - G.reach[2];
+        const int dx = tid % w0 - rch[0], dy = (tid / w0) % w1 - rch[1], dz = tid / (w0 * w1) - rch[2];
         int x = cx + dx, y = cy + dy, z = cz + dz;
-        if (x < 0) x += G.ncell[0]; else if (x >= G.ncell[0]) x -= G.ncell[0];
-        if (y < 0) y += G.ncell[1]; else if (y >= G.ncell[1]) y -= G.ncell[1];
-        if (z < 0) z += G.ncell[2]; else if (z >= G.ncell[2]) z -= G.ncell[2];
-        const int c = G.cell_begin + (z * G.ncell[1] + y) * G.ncell[0] + x;
+        float sx = 0.f, sy = 0.f, sz = 0.f;   // image shift of the cell's beads
+        if (x < 0) { x += nc[0]; sx = -Lf0; } else if (x >= nc[0]) { x -= nc[0]; sx = Lf0; }
+        if (y < 0) { y += nc[1]; sy = -Lf1; } else if (y >= nc[1]) { y -= nc[1]; sy = Lf1; }
+        if (z < 0) { z += nc[2]; sz = -Lf2; } else if (z >= nc[2]) { z -= nc[2]; sz = Lf2; }
+        const int c = cbeg + (z * nc[1] + y) * nc[0] + x;
         st_start[tid] = cell_start[c];
         st_pre[tid + 1] = cell_count[c];
+        st_shift[tid][0] = sx; st_shift[tid][1] = sy; st_shift[tid][2] = sz;
     }
     __syncthreads();
     if (tid == 0) {
@@ -368,22 +399,31 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                 if (st_pre[mid] <= flat) lo = mid; else hi = mid - 1;
             }
             const int sl = st_start[lo] + flat - st_pre[lo];
-            cand[k] = rec[sl];
-            cand_slot[k] = sl;
+            const PairRec r = rec[sl];
+            double x = r.x, y = r.y, z = r.z;
+            if (periodic) {
+                x = wrap_coord(x, L[0], iL[0]);
+                y = wrap_coord(y, L[1], iL[1]);
+                z = wrap_coord(z, L[2], iL[2]);
+            }
+            cpos[k] = make_float4(static_cast<float>(x - o0) + st_shift[lo][0],
+                                  static_cast<float>(y - o1) + st_shift[lo][1],
+                                  static_cast<float>(z - o2) + st_shift[lo][2], __int_as_float(sl));
+            cinfo[k] = make_int2(r.type, r.bead);
         }
         __syncthreads();
         for (int r = wave; r < nr; r += nwave) {
             const int i = h0 + r;
             const PairRec ri = rec[i];
+            const float xi = static_cast<float>((periodic ? wrap_coord(ri.x, L[0], iL[0]) : ri.x) - o0);
+            const float yi = static_cast<float>((periodic ? wrap_coord(ri.y, L[1], iL[1]) : ri.y) - o1);
+            const float zi = static_cast<float>((periodic ? wrap_coord(ri.z, L[2], iL[2]) : ri.z) - o2);
             const uint32_t amask = G.allow[ri.type >> 16];
             const int m = slot_member[i];
             const int eb = excl_start[m], ec = excl_count[m];
-            // the row's excluded partners (bead ids) held wave-uniformly when <= 16
+            // the row's excluded partners (bead ids), one per lane when <= 64
             const int mine = lane < ec ? excl[eb + lane] : -1;
-            int ex[16];
-#pragma unroll
-            for (int e = 0; e < 16; ++e) ex[e] = __builtin_amdgcn_readlane(mine, e);
-            const bool few = ec <= 16;
+            const bool few = ec <= 64;
             const int base = row_start[i];
             int run = row_run[r];
             for (int kb = 0; kb < nw; kb += 64) {
@@ -391,25 +431,24 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                 bool pass = false;
                 int sl = -1;
                 if (k < nw) {
-                    sl = cand_slot[k];
-                    const PairRec rj = cand[k];
-                    if (sl != i && ((amask >> (rj.type >> 16)) & 1u)) {
-                        double ddx = rj.x - ri.x, ddy = rj.y - ri.y, ddz = rj.z - ri.z;
-                        if (G.periodic) {
-                            ddx = min_image(ddx, G.box[0], G.inv_box[0]);
-                            ddy = min_image(ddy, G.box[1], G.inv_box[1]);
-                            ddz = min_image(ddz, G.box[2], G.inv_box[2]);
-                        }
-                        pass = dist2(ddx, ddy, ddz) < G.rl2;
-                        if (pass && ec) {
-                            if (few) {
-                                bool hit = false;
-#pragma unroll
-                                for (int e = 0; e < 16; ++e) hit |= (e < ec) & (ex[e] == rj.bead);
-                                pass = !hit;
-                            } else {
-                                pass = !excluded(excl, eb, ec, rj.bead);
-                            }
+                    const float4 c = cpos[k];
+                    const int2 inf = cinfo[k];
+                    sl = __float_as_int(c.w);
+                    float dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+                    if (any_mi) {
+                        if (mi0) dx -= Lf0 * rintf(dx * iLf0);
+                        if (mi1) dy -= Lf1 * rintf(dy * iLf1);
+                        if (mi2) dz -= Lf2 * rintf(dz * iLf2);
+                    }
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    pass = sl != i && ((amask >> (inf.x >> 16)) & 1u) && d2 < rl2f;
+                    if (pass && ec) {
+                        if (few) {
+                            bool hit = false;
+                            for (int e = 0; e < ec; ++e) hit |= __builtin_amdgcn_readlane(mine, e) == inf.y;
+                            pass = !hit;
+                        } else {
+                            pass = !excluded(excl, eb, ec, inf.y);
                         }
                     }
                 }
