@@ -579,7 +579,8 @@ void Engine::pack() {
     d_.rec = dalloc<PairRec>(nm);
     d_.slot_member = dalloc<int>(nm);
     d_.cell_count = dalloc<int>(d_.ncell);
-    d_.cell_start = dalloc<int>(d_.ncell);
+    check_hip(hipMemset(d_.cell_count, 0, sizeof(int) * d_.ncell), "memset");   // each build leaves it zero
+    d_.cell_start = dalloc<int>(d_.ncell + 1);   // exclusive scan; [ncell] holds the total
     d_.cell_group = dalloc<int>(d_.ncell);
     d_.tile_items = dalloc<int2>(static_cast<size_t>(nm) / 16 + std::min(d_.ncell, std::max(nm, 1)) + 16);
     d_.tile_nitems = dalloc<int>(1);

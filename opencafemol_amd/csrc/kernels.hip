@@ -231,8 +231,10 @@ __global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
                               const int* __restrict__ member_group, const int* __restrict__ member_bead,
                               const double* __restrict__ px, const double* __restrict__ py,
                               const double* __restrict__ pz, int* __restrict__ member_cell,
-                              int* __restrict__ member_rank, int* __restrict__ cell_count) {
+                              int* __restrict__ member_rank, int* __restrict__ cell_count,
+                              int* __restrict__ n_items) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) *n_items = 0;   // the tile work list of this build starts empty
     if (m >= nm) return;
     const DevPairGroup& G = groups[member_group[m]];
     const int b = member_bead[m];
@@ -272,8 +274,11 @@ __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* 
                         int* __restrict__ member_slot, int* __restrict__ slot_member,
                         const int* __restrict__ member_kpos, int* __restrict__ bead_slot,
                         PairRec* __restrict__ rec, double* __restrict__ refx,
-                        double* __restrict__ refy, double* __restrict__ refz) {
+                        double* __restrict__ refy, double* __restrict__ refz,
+                        int* __restrict__ cell_count, int ncell) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    // the counts were consumed by the scan; leave them zero for the next build
+    for (int c = m; c < ncell; c += gridDim.x * blockDim.x) cell_count[c] = 0;
     if (m >= nm) return;
     const int slot = cell_start[member_cell[m]] + member_rank[m];
     member_slot[m] = slot;
@@ -314,11 +319,11 @@ constexpr int kTileChunk = 16;  // rows per work item
 // Work items of the tile build: one per (non-empty cell, 16-row chunk),
 // appended by k_tile_items after the cell scan; the tile kernels run on a
 // fixed grid and loop over them.
-__global__ void k_tile_items(int ncell, const int* __restrict__ cell_count, int2* __restrict__ items,
+__global__ void k_tile_items(int ncell, const int* __restrict__ cell_start, int2* __restrict__ items,
                              int* __restrict__ n_items) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    const int n = (cell_count[c] + kTileChunk - 1) / kTileChunk;
+    const int n = (cell_start[c + 1] - cell_start[c] + kTileChunk - 1) / kTileChunk;
     if (n == 0) return;
     const int at = atomicAdd(n_items, n);
     for (int q = 0; q < n; ++q) items[at + q] = make_int2(c, q);
@@ -343,7 +348,7 @@ __device__ __forceinline__ double wrap_coord(double x, double L, double invL) { 
 __global__ __launch_bounds__(256) void k_nl_tiles(
     const int2* __restrict__ items, const int* __restrict__ n_items, const DevPairGroup* __restrict__ groups,
     const int* __restrict__ cell_group,
-    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const PairRec* __restrict__ rec,
+    const int* __restrict__ cell_start, const PairRec* __restrict__ rec,
     const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
     int* __restrict__ group_need, int* __restrict__ col, int* __restrict__ flags) {
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     __syncthreads();  // LDS of the previous item is no longer read
     const int home = items[item].x;
     const int h0 = cell_start[home] + items[item].y * kTileChunk;
-    const int nr = min(kTileChunk, cell_count[home] - items[item].y * kTileChunk);
+    const int nr = min(kTileChunk, cell_start[home + 1] - h0);
     const int gid = cell_group[home];
     const DevPairGroup& G = groups[gid];
     const int cap = G.stride;
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
         if (z < 0) { z += nc[2]; sz = -Lf2; } else if (z >= nc[2]) { z -= nc[2]; sz = Lf2; }
         const int c = cbeg + (z * nc[1] + y) * nc[0] + x;
         st_start[tid] = cell_start[c];
-        st_pre[tid + 1] = cell_count[c];
+        st_pre[tid + 1] = cell_start[c + 1] - cell_start[c];
         st_shift[tid][0] = sx; st_shift[tid][1] = sy; st_shift[tid][2] = sz;
     }
     __syncthreads();
@@ -1238,19 +1243,64 @@ static void launch_scan(const DevPtrs& d, hipStream_t st, const int* in, int* ou
     k_scan_add<<<nb, kScanThreads, 0, st>>>(out, n, d.scan_partial);
 }
 
+// Exclusive scan of the cell counts in one workgroup (grids up to
+// kSmallCells cells): each thread takes a run of consecutive cells, the block
+// scans the run sums, then every thread writes its cells' starts.
+// start[ncell] receives the total.
+constexpr int kSmallCells = 16384;
+constexpr int kSmallScanThreads = 1024;
+
+__global__ __launch_bounds__(kSmallScanThreads) void k_scan_cells_small(int ncell, const int* __restrict__ count,
+                                                                          int* __restrict__ start) {
+    __shared__ int wb[kSmallScanThreads / 64];
+    const int per = (ncell + kSmallScanThreads - 1) / kSmallScanThreads;
+    const int c0 = min(ncell, static_cast<int>(threadIdx.x) * per), c1 = min(ncell, c0 + per);
+    int sb = 0;
+    for (int c = c0; c < c1; ++c) sb += count[c];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int ib = sb;   // inclusive scan within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(ib, o, 64);
+        if (lane >= o) ib += t;
+    }
+    if (lane == 63) wb[w] = ib;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int rb = 0;
+        for (int k = 0; k < kSmallScanThreads / 64; ++k) {
+            const int t = wb[k];
+            wb[k] = rb;
+            rb += t;
+        }
+        start[ncell] = rb;
+    }
+    __syncthreads();
+    int rb = ib - sb + wb[w];
+    for (int c = c0; c < c1; ++c) {
+        start[c] = rb;
+        rb += count[c];
+    }
+}
+
 void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     if (d.nm == 0) return;
-    k_fill_i32<<<nblk(d.ncell, 256), 256, 0, st>>>(d.cell_count, 0, d.ncell);
+    // cell_count is zero here: the previous build's k_place cleared it
     k_cell_assign<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.groups, d.member_group, d.member_bead, d.px, d.py,
-                                                   d.pz, d.member_cell, d.member_rank, d.cell_count);
-    launch_scan(d, st, d.cell_count, d.cell_start, d.ncell, d.scan_total);
-    k_place<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell, d.member_rank,
-                                             d.cell_start, d.px, d.py, d.pz, d.member_slot, d.slot_member,
-                                             d.member_kpos, d.bead_slot, d.rec, d.refx, d.refy, d.refz);
-    k_fill_i32<<<1, 64, 0, st>>>(d.tile_nitems, 0, 1);
-    k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_count, d.tile_items, d.tile_nitems);
+                                                   d.pz, d.member_cell, d.member_rank, d.cell_count,
+                                                   d.tile_nitems);
+    if (d.ncell <= kSmallCells)
+        k_scan_cells_small<<<1, kSmallScanThreads, 0, st>>>(d.ncell, d.cell_count, d.cell_start);
+    else
+        launch_scan(d, st, d.cell_count, d.cell_start, d.ncell, d.cell_start + d.ncell);
+    k_place<<<nblk(std::max(d.nm, 1), 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell,
+                                                          d.member_rank, d.cell_start, d.px, d.py, d.pz,
+                                                          d.member_slot, d.slot_member, d.member_kpos, d.bead_slot,
+                                                          d.rec, d.refx, d.refy, d.refz, d.cell_count, d.ncell);
+    // work items appended with atomics: the order interleaves the groups, which
+    // measured faster than cell order
+    k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_start, d.tile_items, d.tile_nitems);
     k_nl_tiles<<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group, d.cell_start,
-                                            d.cell_count, d.rec, d.slot_member, d.excl_start, d.excl_count, d.excl,
+                                            d.rec, d.slot_member, d.excl_start, d.excl_count, d.excl,
                                             d.row_start, d.row_count, d.group_need, d.col, d.flags);
 }
 
