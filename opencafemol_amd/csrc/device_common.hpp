@@ -75,6 +75,8 @@ struct DevPairGroup {
     int egroup[4];         // energy group of the EXV, DH, LJR, LJA member forces
     int n_cat;
     int stride;            // list entries reserved per row (multiple of 4)
+    int lj_shared;         // LJ repulsive and attractive members share sigma for every type
+    int pad1;
     double rc2;            // cutoff^2 (nm^2)
     double rl2;            // (cutoff + skin)^2
     double half_skin2;     // (skin/2)^2
@@ -89,8 +91,10 @@ struct DevPairGroup {
     uint32_t allow[32];    // category allow masks
 };
 
-// Per-type parameters: {exv_radius, dh_charge, ljr_eps, ljr_sigma, lja_eps, lja_sigma, 0, 0}
+// Per-type parameters: {exv_radius, dh_charge, 0, 0, ljr_eps, ljr_sigma, lja_eps, lja_sigma};
+// the four LJ values sit 16-byte aligned for one vector read.
 constexpr int kTypeStride = 8;
+constexpr int kTyExv = 0, kTyQ = 1, kTyLjrEps = 4, kTyLjrSig = 5, kTyLjaEps = 6, kTyLjaSig = 7;
 
 // ------------------------------------------------------------------ Philox
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

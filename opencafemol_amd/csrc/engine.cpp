@@ -422,10 +422,10 @@ void Engine::pack() {
                 if (it == tmap.end()) {
                     ty = static_cast<int>(tmap.size());
                     tmap.emplace(tup, ty);
+                    const double row[kTypeStride] = {tup[0], tup[1], 0.0, 0.0, tup[2], tup[3], tup[4], tup[5]};
                     for (int k = 0; k < kTypeStride; ++k) {
-                        const double v = k < 6 ? tup[k] : 0.0;
-                        types_d.push_back(v);
-                        types_f.push_back(static_cast<float>(v));
+                        types_d.push_back(row[k]);
+                        types_f.push_back(static_cast<float>(row[k]));
                     }
                 } else {
                     ty = it->second;
@@ -438,6 +438,9 @@ void Engine::pack() {
             }
             G.member_count = static_cast<int>(member_bead.size()) - G.member_begin;
             G.n_types = static_cast<int>(tmap.size());
+            G.lj_shared = byk[2] && byk[3] ? 1 : 0;
+            for (const auto& kv : tmap)
+                if (kv.first[3] != kv.first[5]) G.lj_shared = 0;
             if (G.n_types * kTypeStride * sizeof(float) > 64 * 1024)
                 invalid("more than 2048 distinct per-particle parameter sets in one nonbonded force");
             max_types = std::max(max_types, G.n_types);
@@ -825,7 +828,7 @@ void Engine::autotune_team() {
     float best = 1e30f;
     int best_team = d_.team, best_grid = d_.pair_grid;
     const int teams[4] = {8, 16, 32, 64};
-    const int grids[4] = {512, 1024, 2048, 1 << 30};
+    const int grids[4] = {1024, 2048, 4096, 1 << 30};
     for (int t : teams) {
         build_pair_blocks(t);
         for (int g : grids) {

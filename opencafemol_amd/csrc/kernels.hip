@@ -818,6 +818,7 @@ template <> __device__ inline double r_exp_fast<double>(double v) { return exp(v
 template <typename R> struct PairK {
     R exv_eps, exv_corr, dh_K, dh_il, dh_corr, lja_rsw, lja_inv_sw;
     bool lja_switch;
+    bool lj_shared;
 };
 template <typename R> __device__ inline PairK<R> pair_constants(const DevPairGroup& G) {
     PairK<R> k;
@@ -829,6 +830,7 @@ template <typename R> __device__ inline PairK<R> pair_constants(const DevPairGro
     k.lja_rsw = static_cast<R>(G.lja_rsw);
     k.lja_inv_sw = static_cast<R>(G.lja_inv_sw);
     k.lja_switch = G.lja_rsw > 0;
+    k.lj_shared = G.lj_shared != 0;
     return k;
 }
 
@@ -857,7 +859,7 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
         r = r2 * ir;
     }
     if (kinds & kExv) {
-        const R s = P(true, 0) + P(false, 0);
+        const R s = P(true, kTyExv) + P(false, kTyExv);
         const R s2 = s * s;
         const R q2 = s2 * ir2;
         const R q6 = q2 * q2 * q2, q12 = q6 * q6;
@@ -867,7 +869,7 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
         e[0] = eps * (q12 - s6 * s6 * G.exv_corr);
     }
     if (kinds & kDh) {
-        const R qq = P(true, 1) * P(false, 1);
+        const R qq = P(true, kTyQ) * P(false, kTyQ);
         const R il = G.dh_il;
         const R ex = r_exp_fast<R>(-r * il);
         const R K = G.dh_K;
@@ -877,8 +879,8 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
     const R c13 = static_cast<R>(1.2599210498948731648);  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
     R s6 = 0, s12 = 0, sb_r = -1;
     if (kinds & kLjr) {
-        const R eb = R(0.5) * (P(true, 2) + P(false, 2));
-        const R sb = R(0.5) * (P(true, 3) + P(false, 3));
+        const R eb = R(0.5) * (P(true, kTyLjrEps) + P(false, kTyLjrEps));
+        const R sb = R(0.5) * (P(true, kTyLjrSig) + P(false, kTyLjrSig));
         const R s2 = sb * sb * ir2;
         s6 = s2 * s2 * s2;
         s12 = s6 * s6;
@@ -891,8 +893,8 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
         }
     }
     if (kinds & kLja) {
-        const R eb = R(0.5) * (P(true, 4) + P(false, 4));
-        const R sb = R(0.5) * (P(true, 5) + P(false, 5));
+        const R eb = R(0.5) * (P(true, kTyLjaEps) + P(false, kTyLjaEps));
+        const R sb = R(0.5) * (P(true, kTyLjaSig) + P(false, kTyLjaSig));
         if (sb != sb_r) {  // Ashbaugh-Hatch pairs share sigma with the repulsive term
             const R s2 = sb * sb * ir2;
             s6 = s2 * s2 * s2;
@@ -921,6 +923,102 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
         e[3] = u;
     }
     *dedr_r = f;
+}
+
+// The row's own type parameters, held in registers for the force pass.
+struct RowType {
+    float exv, q, er, sr, ea, sa;
+};
+__device__ inline RowType row_type(const float* __restrict__ t) {
+    return {t[kTyExv], t[kTyQ], t[kTyLjrEps], t[kTyLjrSig], t[kTyLjaEps], t[kTyLjaSig]};
+}
+
+// (dE/dr)/r of the force pass.  Same terms as pair_eval<float>; when the
+// repulsive and attractive LJ members of the group share sigma (Ashbaugh-
+// Hatch) they are evaluated together: one set of sigma powers, one
+// (24 s^6 - 48 s^12)/r^2 factor scaled by the repulsive epsilon inside r_min
+// and by the attractive one outside, the switch on the attractive part.
+__device__ inline float pair_force(int kinds, const PairK<float>& G, const RowType& ti,
+                                   const float* __restrict__ tj, float r2) {
+    const float ir2 = __builtin_amdgcn_rcpf(r2);
+    float f = 0.f, r = 0.f, ir = 0.f;
+    if ((kinds & kDh) || ((kinds & kLja) && G.lja_switch)) {
+        ir = __builtin_amdgcn_rsqf(r2);
+        r = r2 * ir;
+    }
+    if (kinds & kExv) {
+        const float s = ti.exv + tj[kTyExv];
+        const float q2 = s * s * ir2;
+        const float q6 = q2 * q2 * q2, q12 = q6 * q6;
+        f += -12.f * G.exv_eps * q12 * ir2;
+    }
+    if (kinds & kDh) {
+        const float qq = ti.q * tj[kTyQ];
+        const float ex = r_exp_fast<float>(-r * G.dh_il);
+        f += -qq * G.dh_K * ex * (ir + G.dh_il) * ir2;
+    }
+    const float c13 = 1.2599210498948731648f;  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
+    if ((kinds & (kLjr | kLja)) == (kLjr | kLja) && G.lj_shared) {
+        const float4 lj = *reinterpret_cast<const float4*>(tj + kTyLjrEps);   // er, sr, ea, sa
+        const float sb = 0.5f * (ti.sr + lj.y);
+        const float sb2 = sb * sb;
+        const float s2 = sb2 * ir2;
+        const float s6 = s2 * s2 * s2, s12 = s6 * s6;
+        const float th2 = c13 * sb2;
+        const float X = (24.f * s6 - 48.f * s12) * ir2;
+        const float er = 0.5f * (ti.er + lj.x), ea = 0.5f * (ti.ea + lj.z);
+        if (th2 - r2 >= 0.f) f += er * X;
+        float u = 0.f, du = 0.f;
+        if (r2 - th2 >= 0.f) {
+            u = 4.f * ea * (s12 - s6);
+            du = ea * X;
+        }
+        if (th2 - r2 >= 0.f) u -= ea;
+        if (G.lja_switch && r > G.lja_rsw) {
+            const float x = (r - G.lja_rsw) * G.lja_inv_sw;
+            const float x2 = x * x, x3 = x2 * x;
+            const float S = 1.f - x3 * (10.f - 15.f * x + 6.f * x2);
+            const float dS = -30.f * x2 * (1.f - x) * (1.f - x) * G.lja_inv_sw;
+            du = du * S + u * dS * ir;
+        }
+        f += du;
+        return f;
+    }
+    float s6 = 0.f, s12 = 0.f, sb_r = -1.f;
+    if (kinds & kLjr) {
+        const float eb = 0.5f * (ti.er + tj[kTyLjrEps]);
+        const float sb = 0.5f * (ti.sr + tj[kTyLjrSig]);
+        const float s2 = sb * sb * ir2;
+        s6 = s2 * s2 * s2;
+        s12 = s6 * s6;
+        sb_r = sb;
+        if (c13 * sb * sb - r2 >= 0.f) f += eb * (-48.f * s12 + 24.f * s6) * ir2;
+    }
+    if (kinds & kLja) {
+        const float eb = 0.5f * (ti.ea + tj[kTyLjaEps]);
+        const float sb = 0.5f * (ti.sa + tj[kTyLjaSig]);
+        if (sb != sb_r) {
+            const float s2 = sb * sb * ir2;
+            s6 = s2 * s2 * s2;
+            s12 = s6 * s6;
+        }
+        const float th2 = c13 * sb * sb;
+        float u = 0.f, du = 0.f;
+        if (r2 - th2 >= 0.f) {
+            u += 4.f * eb * (s12 - s6);
+            du += eb * (-48.f * s12 + 24.f * s6) * ir2;
+        }
+        if (th2 - r2 >= 0.f) u -= eb;
+        if (G.lja_switch && r > G.lja_rsw) {
+            const float x = (r - G.lja_rsw) * G.lja_inv_sw;
+            const float x2 = x * x, x3 = x2 * x;
+            const float S = 1.f - x3 * (10.f - 15.f * x + 6.f * x2);
+            const float dS = -30.f * x2 * (1.f - x) * (1.f - x) * G.lja_inv_sw;
+            du = du * S + u * dS * ir;
+        }
+        f += du;
+    }
+    return f;
 }
 
 // Pair contributions to fixed point.  fma(g * 2^32, d, 1.5 * 2^52) rounds the
@@ -956,7 +1054,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
     const float* __restrict__ types_f, const double* __restrict__ types_d, ForceBuf out,
     long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks,
     unsigned long long* __restrict__ stamp, BondedArgs bd) {
-    extern __shared__ float lds_types[];
+    extern __shared__ __attribute__((aligned(16))) float lds_types[];
     int loaded = -1;
     // optional duration stamps: each workgroup stores its start and end on the
     // device's constant-rate wall clock (plain stores to its own two words)
@@ -1014,6 +1112,8 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
         const int ti = ri.type & 0xFFFF;
         const float* tfi = lds_types + ti * kTypeStride;
         const double* tdi = types_d + static_cast<size_t>(G.type_begin + ti) * kTypeStride;
+        RowType rt{};
+        if (!ENERGY) rt = row_type(tfi);
         // rows are padded to multiples of 4 entries (-1 = empty); a lane takes
         // 4 consecutive entries per round so 4 record gathers are in flight
         const int4* col4 = reinterpret_cast<const int4*>(col + beg);
@@ -1023,10 +1123,13 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
             PairRec rjs[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) rjs[u] = rec[cs[u] >= 0 ? cs[u] : row];
+            if (!ENERGY) nadd += 4;
+            // the force pass evaluates the four entries without branches (an
+            // entry that is padding or beyond the cutoff is computed at r2 = 1
+            // and masked to zero) so their dependency chains interleave
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int s = cs[u];
-                if (s < 0) continue;
                 const PairRec rj = rjs[u];
                 double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
                 if (periodic) {
@@ -1035,8 +1138,9 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
                     dz = min_image(dz, bz, ibz);
                 }
                 const double r2 = dist2(dx, dy, dz);
-                if (!(r2 < rc2)) continue;
+                const bool in = s >= 0 && r2 < rc2;
                 if (ENERGY) {
+                    if (!in) continue;
                     if (row < s) {
                         double f;
                         double e[4] = {0, 0, 0, 0};
@@ -1048,25 +1152,22 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
                         ae3 += to_fix_d(e[3], bad);
                     }
                 } else {
-                    float f;
-                    float e[4];
                     const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
-                    const float r2f = static_cast<float>(r2);
-                    pair_eval<float>(kinds, Kf, tfi, tfj, nullptr, nullptr, r2f, &f, e);
+                    const float r2f = in ? static_cast<float>(r2) : 1.0f;
+                    const float f = pair_force(kinds, Kf, rt, tfj, r2f);
                     // |f d| <= |f| r: one range test covers the three
                     // components.  A pair beyond the limit is flagged and
                     // clamped to it along its direction (so relaxation still
                     // pushes the beads apart); NaN contributes nothing.
                     const bool ok = f * f * r2f < kPairLimit * kPairLimit;
-                    bad |= !ok;
-                    const float g = ok ? f
-                                       : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f)
-                                                 : 0.0f);
+                    bad |= in && !ok;
+                    float g = ok ? f
+                                 : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
+                    g = in ? g : 0.0f;   // a masked entry adds exactly the bias constant
                     const double gs = static_cast<double>(g) * 4294967296.0;
                     ux += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic)));
                     uy += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic)));
                     uz += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic)));
-                    ++nadd;
                 }
             }
         }

@@ -70,7 +70,7 @@ struct DevPtrs {
     int2* dump_pairs = nullptr;
 };
 
-constexpr int kPairBlock = 256;
+constexpr int kPairBlock = 128;
 
 void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st);
 // pack != 0: also refresh the pair records (and check list validity when check != 0)
