@@ -6,6 +6,12 @@
  * OpenMM::System (src/input/ReadTOMLInput.cpp:964-1098): parse, expand
  * includes, build the generators and the system, read the integrator and the
  * initial coordinates.  The returned handle owns the descriptors.
+ *
+ * ocmh_load_genesis does the same for a GENESIS `.inp` file and the `.top`
+ * (with its `#include`s) and `.gro` files it names, as read_genesis_input
+ * does (src/input/ReadGenesisInput.cpp:217-520).  Such inputs carry no
+ * velocities: the reference starts them at zero, and ocmh_velocities returns
+ * NULL.
  */
 #ifndef OCM_HOST_H
 #define OCM_HOST_H
@@ -19,6 +25,7 @@ extern "C" {
 typedef struct ocmh_input ocmh_input;
 
 int ocmh_load_toml(const char* path, ocmh_input** out);
+int ocmh_load_genesis(const char* inp_path, ocmh_input** out);
 void ocmh_free(ocmh_input* in);
 const ocm_system_desc* ocmh_system(ocmh_input* in);
 const ocm_integrator_desc* ocmh_integrator(ocmh_input* in);

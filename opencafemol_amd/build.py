@@ -52,7 +52,7 @@ def build_hip(verbose=False):
         print("built", os.path.join(LIB_DIR, "libocm.so"))
 
 
-HOST_SOURCES = ["toml.cpp", "forcefield.cpp", "aicg2_tables.cpp", "read_toml.cpp", "simulator.cpp", "host_capi.cpp"]
+HOST_SOURCES = ["toml.cpp", "forcefield.cpp", "aicg2_tables.cpp", "read_toml.cpp", "read_genesis.cpp", "simulator.cpp", "host_capi.cpp"]
 HOST = os.path.join(HERE, "host")
 BIN_DIR = os.path.join(HERE, "bin")
 CXX = os.environ.get("CXX", "g++")

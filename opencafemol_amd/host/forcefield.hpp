@@ -90,6 +90,7 @@ public:
                                     const std::vector<double>& ks, bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "HarmonicBond"; }
+    std::size_t size() const { return indices_.size(); }
     const index_pairs_type& indices() const { return indices_; }
 private:
     index_pairs_type indices_;
@@ -104,6 +105,7 @@ public:
                                     bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "GaussianBond"; }
+    std::size_t size() const { return indices_.size(); }
 private:
     index_pairs_type indices_;
     std::vector<double> ks_, v0s_, sigmas_;
@@ -116,6 +118,7 @@ public:
                                  const std::vector<double>& r0s, bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "GoContact"; }
+    std::size_t size() const { return indices_.size(); }
 private:
     index_pairs_type indices_;
     std::vector<double> ks_, r0s_;
@@ -129,6 +132,7 @@ public:
                                      const std::vector<double>& ks, bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "HarmonicAngle"; }
+    std::size_t size() const { return indices_.size(); }
 private:
     std::vector<indices_type> indices_;
     std::vector<double> v0s_, ks_;
@@ -143,6 +147,7 @@ public:
                                           bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "FlexibleLocalAngle (" + aa_name_ + ")"; }
+    std::size_t size() const { return indices_.size(); }
     double min_energy() const;  // the reference's 1e-4 sweep of its own spline evaluator
 private:
     std::vector<indices_type> indices_;
@@ -160,6 +165,7 @@ public:
                                         bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "GaussianDihedral"; }
+    std::size_t size() const { return indices_.size(); }
 private:
     std::vector<indices_type> indices_;
     std::vector<double> ks_, theta0s_, sigmas_;
@@ -174,6 +180,7 @@ public:
                                              bool use_periodic);
     ForceDescriptor generate() const override;
     std::string name() const override { return "FlexibleLocalDihedral (" + aa_pair_name_ + ")"; }
+    std::size_t size() const { return indices_.size(); }
     double min_energy() const;
 private:
     std::vector<indices_type> indices_;

@@ -1,5 +1,6 @@
 // host_capi.cpp -- C entry points of include/ocm_host.h.
 #include "../../include/ocm_host.h"
+#include "read_genesis.hpp"
 #include "read_toml.hpp"
 
 #include <string>
@@ -32,6 +33,30 @@ int ocmh_load_toml(const char* path, ocmh_input** out) {
             in->integ = ocmhost::read_toml_integrator_gen(data)->generate();
             for (const auto& p : ocmhost::read_toml_initial_conf(data)) in->positions.insert(in->positions.end(), p.begin(), p.end());
             for (const auto& v : ocmhost::read_toml_initial_vel(data)) in->velocities.insert(in->velocities.end(), v.begin(), v.end());
+        } catch (...) {
+            delete in;
+            throw;
+        }
+        *out = in;
+        return OCM_OK;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return OCM_ERR_INVALID;
+    }
+}
+
+int ocmh_load_genesis(const char* inp_path, ocmh_input** out) {
+    try {
+        g_err.clear();
+        if (!inp_path || !out) throw std::runtime_error("[error] NULL argument");
+        ocmhost::GenesisInput g = ocmhost::read_genesis_input(inp_path);
+        auto* in = new ocmh_input();
+        try {
+            in->system = g.system.generate();
+            in->groups = g.system.ffname_groupid_map();
+            in->view = in->system.view();
+            in->integ = g.integrator->generate();
+            for (const auto& p : g.positions) in->positions.insert(in->positions.end(), p.begin(), p.end());
         } catch (...) {
             delete in;
             throw;

@@ -1,6 +1,7 @@
-// main.cpp -- command-line driver: ocm_run <input.toml>
+// main.cpp -- command-line driver: ocm_run <input.toml> or <input.inp>
 // (the reference's src/OpenAICG2Plus.cpp:16-94 with OpenMM plugin loading
 // replaced by the gfx950 device step).
+#include "read_genesis.hpp"
 #include "read_toml.hpp"
 #include "simulator.hpp"
 
@@ -9,6 +10,29 @@
 #include <iostream>
 
 using namespace ocmhost;
+
+static void note_platform(const std::string& name) {
+    std::cerr << "    platform \"" << name << "\" requested; running on the gfx950 device step" << std::endl;
+}
+
+// ReadGenesisInput.cpp:217-503: no minimization and no velocity setting, so
+// the run starts from zero velocities; DCD and .ene outputs share the
+// dcdfile prefix, and the DCD header carries the timestep in ps.
+static Simulator make_genesis_simulator(const std::string& path) {
+    GenesisInput in = read_genesis_input(path);
+    if (!in.pdbfile.empty()) write_genesis_initial_pdb(in);
+    const SystemDescriptor probe = in.system.generate();
+    std::vector<std::unique_ptr<ObserverBase>> obs;
+    obs.push_back(std::make_unique<DCDObserver>(in.file_prefix, in.nsteps, in.crdout_period,
+                                                static_cast<float>(in.timestep), in.periodic, probe.box));
+    obs.push_back(std::make_unique<EnergyObserver>(in.file_prefix, in.system.ffname_groupid_map()));
+    if (!in.platform.empty()) note_platform(in.platform);
+    ocm_options opt;
+    ocm_options_default(&opt);
+    Simulator sim(in.system, std::move(in.integrator), opt, in.nsteps, in.crdout_period, obs);
+    sim.initialize(in.positions);
+    return sim;
+}
 
 static Simulator make_simulator(const std::string& path) {
     std::cerr << "parsing " << path << "..." << std::endl;
@@ -26,8 +50,7 @@ static Simulator make_simulator(const std::string& path) {
     obs.push_back(std::make_unique<EnergyObserver>(prefix, sys.ffname_groupid_map()));
     if (const Value* plat = data.find("platform"))
         if (const Value* name = plat->find("name"))
-            std::cerr << "    platform \"" << name->as_string() << "\" requested; running on the gfx950 device step"
-                      << std::endl;
+            note_platform(name->as_string());
     ocm_options opt;
     ocm_options_default(&opt);
     Simulator sim(sys, std::move(integ), opt, rs.total_step, rs.save_step, obs, rs.energy_minimization, rs.progress_bar);
@@ -47,12 +70,15 @@ static Simulator make_simulator(const std::string& path) {
 
 int main(int argc, char** argv) {
     std::cerr << "opencafemol_amd " << ocm_version() << std::endl;
-    if (argc != 2) {
-        std::fprintf(stderr, "Usage: %s <input.toml>\n", argv[0]);
+    const std::string arg = argc == 2 ? argv[1] : "";
+    const auto dot = arg.rfind('.');
+    const std::string suffix = dot == std::string::npos ? "" : arg.substr(dot);
+    if (suffix != ".toml" && suffix != ".inp") {
+        std::fprintf(stderr, "Usage: %s <input.toml> or <input.inp>\n", argv[0]);
         return 1;
     }
     try {
-        Simulator sim = make_simulator(argv[1]);
+        Simulator sim = suffix == ".inp" ? make_genesis_simulator(arg) : make_simulator(arg);
         const auto start = std::chrono::system_clock::now();
         std::cerr << "calculation start!" << std::endl;
         sim.run();

@@ -1,8 +1,9 @@
-"""ctypes face of include/ocm_host.h (libocm_host.so): the C++ TOML front end.
+"""ctypes face of include/ocm_host.h (libocm_host.so): the C++ TOML and
+GENESIS front ends.
 
-`load_toml(path)` returns a model.SystemSpec, an IntegratorSpec, the initial
-positions (nm) and velocities (nm/ps or None) exactly as the C++ readers and
-generators produced them.
+`load_toml(path)` and `load_genesis(inp_path)` return a model.SystemSpec, an
+IntegratorSpec, the initial positions (nm) and velocities (nm/ps or None)
+exactly as the C++ readers and generators produced them.
 """
 import ctypes as C
 import os
@@ -25,6 +26,8 @@ def load_library():
         L = C.CDLL(LIB_PATH)
         L.ocmh_load_toml.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
         L.ocmh_load_toml.restype = C.c_int
+        L.ocmh_load_genesis.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.ocmh_load_genesis.restype = C.c_int
         L.ocmh_free.argtypes = [C.c_void_p]
         L.ocmh_system.argtypes = [C.c_void_p]
         L.ocmh_system.restype = C.POINTER(abi.SystemDesc)
@@ -48,9 +51,17 @@ def _arr(ptr, n, dtype):
 
 
 def load_toml(path):
+    return _load(load_library().ocmh_load_toml, path)
+
+
+def load_genesis(inp_path):
+    return _load(load_library().ocmh_load_genesis, inp_path)
+
+
+def _load(loader, path):
     L = load_library()
     h = C.c_void_p()
-    code = L.ocmh_load_toml(path.encode(), C.byref(h))
+    code = loader(path.encode(), C.byref(h))
     if code != abi.OCM_OK:
         raise abi.OcmError(code, L.ocmh_last_error().decode())
     try:

@@ -68,3 +68,34 @@ def test_cli_writes_dcd_for_periodic_input(tmp_path):
     frame = (4 + 48 + 4) + 3 * (4 + 4 * natom + 4)
     assert len(data) == header + 3 * frame
     assert (tmp_path / "hps_velocity.dcd").stat().st_size == header + 3 * (3 * (4 + 4 * natom + 4))
+
+
+def test_cli_runs_genesis_input(tmp_path, oracle):
+    """ocm_run on a GENESIS .inp: the initial-state PDB, DCD frames with the
+    timestep in ps, and an .ene whose step-0 row is the oracle's energies at
+    the .gro positions with zero kinetic energy (this input mode sets no
+    velocities, ReadGenesisInput.cpp:499-503)."""
+    from tests.golden import sh3_genesis
+    inp = sh3_genesis.write(str(tmp_path), nsteps=300, crdout_period=100, platform="CUDA")
+    spec, integ, x, v = host_abi.load_genesis(inp)
+    r = subprocess.run([EXE, inp], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pdb = (tmp_path / "sh3_init.pdb").read_text().splitlines()
+    assert pdb[0] == "MODEL     0" and pdb[-1] == "ENDMDL" and len(pdb) == 58
+    for i, line in enumerate(pdb[1:-1]):
+        assert line.startswith("ATOM  %5d   CA " % (i + 1)) and line.endswith("  1.00  0.00")
+        np.testing.assert_allclose([float(line[30:38]), float(line[38:46]), float(line[46:54])], x[i] * 10,
+                                   atol=5e-4)
+    data = (tmp_path / "sh3_md_position.dcd").read_bytes()
+    assert struct.unpack("<i4s", data[:8]) == (84, b"CORD")
+    assert struct.unpack("<4i", data[8:24]) == (3, 0, 100, 300)
+    assert struct.unpack("<f", data[44:48])[0] == pytest.approx(0.01)
+    ene = (tmp_path / "sh3_md.ene").read_text().splitlines()
+    names = ene[1].split()[2:-1]
+    rows = [list(map(float, l.split())) for l in ene[2:]]
+    assert [r[0] for r in rows] == [0, 100, 200]
+    _, E = oracle.evaluate(spec, x)
+    got = dict(zip(names, rows[0][1:-1]))
+    for name, g in spec.group_ids.items():
+        assert got[name] == pytest.approx(E[g] / 4.184, rel=1e-6, abs=2e-6), name
+    assert rows[0][-1] == 0.0 and rows[-1][-1] > 0.0
