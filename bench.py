@@ -76,6 +76,18 @@ def cpu_baseline(system, integ, x, v, dt_ps, budget_s=10.0):
                       f"(oracle/oracle.cpp oracle_baoab_cells), OpenMP {threads} threads"}
 
 
+def list_options(args, base=None):
+    """Neighbor-list options given on the command line, over `base`."""
+    lopt = dict(base or {})
+    if args.skin is not None:
+        lopt["skin"] = args.skin
+    if args.rebuild_interval is not None:
+        lopt["rebuild_interval"] = args.rebuild_interval
+    if args.prune_interval is not None:
+        lopt["prune_interval"] = args.prune_interval
+    return lopt
+
+
 def replicas_bench(args):
     """256 Go chains (50-500 beads, 280-400 K ladder) packed in one batch vs the
     same replicas stepped one at a time through single-system contexts."""
@@ -85,7 +97,8 @@ def replicas_bench(args):
     kB = 1.380649e-23 * 6.02214076e23 / 1000.0
     rng = np.random.default_rng(5)
     vs = [rng.normal(size=(s.n, 3)) * np.sqrt(kB * it.temperature / s.masses)[:, None] for s, it in zip(systems, integs)]
-    b = Batch(systems, integs, use_graph=0 if args.no_graph else 1)
+    lopt = list_options(args)
+    b = Batch(systems, integs, use_graph=0 if args.no_graph else 1, **lopt)
     for k in range(len(systems)):
         b.set_positions(k, xs[k])
         b.set_velocities(k, vs[k])
@@ -104,7 +117,7 @@ def replicas_bench(args):
     serial_steps = max(200, args.steps // 10)
     tot = 0.0
     for k in range(len(systems)):
-        c = Context(systems[k], integs[k], use_graph=0 if args.no_graph else 1)
+        c = Context(systems[k], integs[k], use_graph=0 if args.no_graph else 1, **lopt)
         c.set_positions(xs[k])
         c.set_velocities(vs[k])
         c.step(200)
@@ -287,6 +300,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for profilers)")
     ap.add_argument("--skin", type=float, default=None, help="neighbor-list skin (nm); default: library's")
     ap.add_argument("--rebuild-interval", type=int, default=None, help="steps between list rebuilds")
+    ap.add_argument("--prune-interval", type=int, default=None,
+                    help="steps between inner-list prunes (<= 1: off); default: library's")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-steps", default="10000,50000", help="per-job step range (multiples of 10000)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="arrival rate / serial service rate")
@@ -313,13 +328,8 @@ def main():
 
     systems, integs, xs, cfg = build_workload(args.workload)
     log(f"workload {cfg['workload']}: {sum(s.n for s in systems)} beads in {len(systems)} system(s)")
-    lopt = {}
-    if args.workload == "condensate":   # the configuration states its list parameters
-        lopt = {"skin": 0.3, "rebuild_interval": 20}
-    if args.skin is not None:
-        lopt["skin"] = args.skin
-    if args.rebuild_interval is not None:
-        lopt["rebuild_interval"] = args.rebuild_interval
+    # the condensate configuration states its list parameters
+    lopt = list_options(args, {"skin": 0.3, "rebuild_interval": 20} if args.workload == "condensate" else None)
     b = Batch(systems, integs, use_graph=0 if args.no_graph else 1, **lopt)
     for k, x in enumerate(xs):
         b.set_positions(k, x)
@@ -361,11 +371,17 @@ def main():
     # pair kernel duration over the timed region: first block start to last
     # block end of every launch, on the device wall clock
     entries, members, rebuilds = b.list_stats()
+    inner_entries, prune = b.inner_list_stats()
     avg_us = stamp_ms / max(stamp_n, 1) * 1000.0
-    bytes_per_launch = 4 * entries + 64 * members
+    # entries one force-pass launch reads: the inner rows when the inner list
+    # is on (a separate prune kernel rewrites them from the outer rows every
+    # few steps), the outer rows otherwise
+    steps_per_rebuild = args.steps / max(rebuilds_timed, 1)
+    read = float(inner_entries if prune > 1 else entries)
+    bytes_per_launch = 4 * read + 64 * members
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
     log(f"pair kernel {avg_us:.2f} us/launch over {stamp_n} launches of the timed region; list {entries} "
-        f"entries, {members} rows; {achieved:.0f} GB/s algorithmic")
+        f"entries (inner {inner_entries}, prune interval {prune}), {members} rows; {achieved:.0f} GB/s algorithmic")
 
     # breakdown phase: HIP events around every launch (eager; adds launch latency to each figure)
     kt_steps = min(args.steps, 4000)
@@ -377,17 +393,20 @@ def main():
     int_ms, int_n = b.timing(2)
     nl_ms, nl_n = b.timing(3)
     pk_ms, pk_n = b.timing(4)
+    pr_ms, pr_n = b.timing(6)
     kernels = {"pair_us": avg_us, "pair_us_event_bracketed": pair_ms / max(pair_n, 1) * 1000,
                "bonded_us": bond_ms / max(bond_n, 1) * 1000, "integrator_us":
                int_ms / max(int_n, 1) * 1000, "list_build_us": nl_ms / max(nl_n, 1) * 1000,
-               "pack_us": pk_ms / max(pk_n, 1) * 1000, "list_entries": entries, "list_rows": members,
-               "entries_per_s": entries / (avg_us * 1e-6), "list_rebuilds_timed": rebuilds_timed,
-               "steps_per_rebuild": args.steps / max(rebuilds_timed, 1)}
+               "pack_us": pk_ms / max(pk_n, 1) * 1000, "prune_us_event_bracketed": pr_ms / max(pr_n, 1) * 1000,
+               "prunes_per_step": pr_n / max(kt_steps, 1), "list_entries": entries, "list_rows": members,
+               "inner_list_entries": inner_entries, "prune_interval": prune,
+               "entries_read_per_launch": read, "entries_per_s": read / (avg_us * 1e-6),
+               "list_rebuilds_timed": rebuilds_timed, "steps_per_rebuild": steps_per_rebuild}
 
     traffic = None
     tf = TRAFFIC_FILES.get(cfg["workload"])
     if tf and os.path.exists(tf):
-        traffic = json.load(open(tf))["hbm_bytes_per_entry"] * entries
+        traffic = json.load(open(tf))["hbm_bytes_per_entry"] * read
 
     cpu = None
     if not args.no_cpu and len(systems) == 1:

@@ -140,7 +140,9 @@ typedef struct ocm_options {
     int32_t team_size;         /* lanes per pair-list row: 4,8,16,32,64; 0 = auto */
     int32_t use_graph;         /* capture step chunks as hipGraphs           */
     int32_t chunk_steps;       /* steps between host checks of device flags  */
-    int32_t reserved;
+    int32_t prune_interval;    /* steps between prunes of the inner list
+                                  (cutoff + 0.1 nm, capped by the skin);
+                                  <= 1: every pair pass reads the full list  */
 } ocm_options;
 
 /* getState selection bits */
@@ -161,7 +163,8 @@ typedef struct ocm_state {
 
 typedef struct ocm_context ocm_context;
 
-/* Fills `opt` with defaults (skin 0.4 nm, rebuild every 20 steps, ...).
+/* Fills `opt` with defaults (skin 0.6 nm, rebuild every 40 steps, inner
+ * list pruned every 5 steps, ...).
  * Skin and interval change speed only: forces are fixed-point sums over the
  * pairs inside the cutoff, so any valid list gives the same bits.  When a bead
  * moves more than skin/2 between rebuilds the chunk is redone with half the
@@ -207,7 +210,8 @@ int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index,
 
 /* Per-kernel timing.  enable = 1: HIP events on the context's stream around
  * every launch (steps run eagerly), read with kernel 0 = pair forces,
- * 1 = bonded forces, 2 = integrator, 3 = list build, 4 = record pack.
+ * 1 = bonded forces, 2 = integrator, 3 = list build, 4 = record pack,
+ * 6 = inner-list prune.
  * enable = 2: steps keep running as captured graphs and every pair-force
  * launch of a step records its first block start and last block end on the
  * device wall clock; read with kernel 5.  enable = 0: off.                  */
@@ -219,6 +223,12 @@ int ocm_get_timing(ocm_context* ctx, int32_t kernel, double* total_ms,
  * member beads summed over all pair groups.                                */
 int ocm_get_list_stats(ocm_context* ctx, int64_t* list_entries,
                        int64_t* members, int64_t* rebuilds);
+/* Inner (pruned) list: padded entries now held, and the prune interval in
+ * use (lowered after a bead moved more than margin/2 between prunes, raised
+ * back after clean chunks; 0 when the inner list is off).  No counterpart in
+ * the reference, whose list lives inside OpenMM; a diagnostic.            */
+int ocm_get_inner_list_stats(ocm_context* ctx, int64_t* inner_entries,
+                             int32_t* prune_interval);
 
 /* ------------------------------------------------------------------ batch */
 /* Many independent systems packed into one set of device arrays; every

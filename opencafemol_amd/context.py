@@ -108,6 +108,12 @@ class Batch:
         check(self.lib.ocm_get_list_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def inner_list_stats(self):
+        """(padded inner-list entries, prune interval in use; 0 = inner list off)."""
+        a, p = C.c_int64(), C.c_int32()
+        check(self.lib.ocm_get_inner_list_stats(self.h, C.byref(a), C.byref(p)))
+        return a.value, p.value
+
 
 class Context(Batch):
     """OpenMM::Context equivalent for one system (src/Simulator.cpp:17)."""

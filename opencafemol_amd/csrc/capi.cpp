@@ -63,12 +63,12 @@ extern "C" {
 void ocm_options_default(ocm_options* o) {
     if (!o) return;
     o->device = 0;
-    o->rebuild_interval = 20;
-    o->skin = 0.4;
+    o->rebuild_interval = 40;
+    o->skin = 0.6;
     o->team_size = 0;
     o->use_graph = 1;
     o->chunk_steps = 200;
-    o->reserved = 0;
+    o->prune_interval = 5;
 }
 
 int ocm_batch_create(const ocm_system_desc* systems, const ocm_integrator_desc* integrators, int32_t n,
@@ -190,7 +190,7 @@ int ocm_set_timing(ocm_context* ctx, int32_t enable) {
 
 int ocm_get_timing(ocm_context* ctx, int32_t kernel, double* total_ms, int64_t* launches) {
     return guarded([&] {
-        if (kernel < 0 || kernel > 5) throw ocm::Error(OCM_ERR_INVALID, "kernel id out of range");
+        if (kernel < 0 || kernel > 6) throw ocm::Error(OCM_ERR_INVALID, "kernel id out of range");
         const auto t = eng(ctx).timing(kernel);
         if (total_ms) *total_ms = t.ms;
         if (launches) *launches = t.launches;
@@ -205,6 +205,17 @@ int ocm_get_list_stats(ocm_context* ctx, int64_t* entries, int64_t* members, int
         if (entries) *entries = a;
         if (members) *members = b;
         if (rebuilds) *rebuilds = c;
+        return OCM_OK;
+    });
+}
+
+int ocm_get_inner_list_stats(ocm_context* ctx, int64_t* entries, int32_t* prune_interval) {
+    return guarded([&] {
+        long long a = 0;
+        int p = 0;
+        eng(ctx).inner_stats(&a, &p);
+        if (entries) *entries = a;
+        if (prune_interval) *prune_interval = p;
         return OCM_OK;
     });
 }

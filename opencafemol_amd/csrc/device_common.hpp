@@ -15,6 +15,7 @@ constexpr float kFixLimitF = 1073741824.0f;          // |contribution| must stay
 // Per-system error bits (DevSys::err).
 constexpr int kErrNumeric = 1;   // NaN or fixed-point overflow in a force/energy term
 constexpr int kErrStale = 2;     // a bead moved more than skin/2 since the list build
+constexpr int kErrPrune = 4;     // a bead moved more than margin/2 since the inner list was pruned
 // Batch-wide flag word bits.
 constexpr int kFlagListOverflow = 1;
 
@@ -80,6 +81,7 @@ struct DevPairGroup {
     double rc2;            // cutoff^2 (nm^2)
     double rl2;            // (cutoff + skin)^2
     double half_skin2;     // (skin/2)^2
+    double rin2;           // (cutoff + margin)^2: radius of the pruned inner list
     double rc;
     double box[3], inv_box[3];
     double cell_scale[3];  // ncell / box

@@ -206,6 +206,7 @@ Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(s
     if (opt_.chunk_steps < 1) opt_.chunk_steps = 200;
     if (!(opt_.skin >= 0.0)) invalid("skin must be >= 0");
     interval_ = opt_.rebuild_interval;
+    prune_ = opt_.prune_interval > 1 ? opt_.prune_interval : 0;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(OCM_ERR_NO_DEVICE, "no HIP device found");
     if (opt_.device < 0 || opt_.device >= ndev) throw Error(OCM_ERR_NO_DEVICE, "device ordinal out of range");
@@ -368,6 +369,9 @@ void Engine::pack() {
             const double rl = f0.cutoff + skin;
             G.rl2 = rl * rl;
             G.half_skin2 = 0.25 * skin * skin;
+            const double margin = std::min(kPruneMargin, skin);
+            G.rin2 = (f0.cutoff + margin) * (f0.cutoff + margin);
+            prune_hm2_ = std::min(prune_hm2_, 0.25 * margin * margin * (1.0 - 1e-6));
             for (int c = 0; c < 3; ++c) {
                 G.box[c] = H.box[c];
                 G.inv_box[c] = G.periodic ? 1.0 / H.box[c] : 0.0;
@@ -560,25 +564,24 @@ void Engine::pack() {
         int maxm = 1;
         for (int b = 0; b < nb; ++b) maxm = std::max(maxm, cnt[b]);
         d_.maxm = maxm;
-        std::vector<int> type(static_cast<size_t>(nb) * maxm, 0), fillc(nb, 0);
-        std::vector<double> hs2(static_cast<size_t>(nb) * maxm, 0.0);
+        std::vector<int> fillc(nb, 0);
+        std::vector<double> hs2(nb, std::numeric_limits<double>::max());   // per bead: smallest (skin/2)^2
         for (int m = 0; m < nm; ++m) {
             const int b = member_bead[m];
             const int k = b * maxm + fillc[b]++;
             kpos[m] = k;
-            type[k] = member_type[m];
-            hs2[k] = groups[member_group[m]].half_skin2;
+            hs2[b] = std::min(hs2[b], groups[member_group[m]].half_skin2);
         }
-        d_.bead_mem_count = dalloc<int>(nb); upload(d_.bead_mem_count, cnt);
         d_.bead_slot = dalloc<int>(static_cast<size_t>(nb) * maxm);
-        d_.bead_type = dalloc<int>(type.size()); upload(d_.bead_type, type);
+        // slots of absent memberships stay -1; the build fills the others
+        check_hip(hipMemset(d_.bead_slot, 0xFF, sizeof(int) * static_cast<size_t>(nb) * maxm), "memset");
         d_.bead_hs2 = dalloc<double>(hs2.size()); upload(d_.bead_hs2, hs2);
         d_.member_kpos = dalloc<int>(kpos.size()); upload(d_.member_kpos, kpos);
     }
     d_.excl_start = dalloc<int>(nm); upload(d_.excl_start, excl_start);
     d_.excl_count = dalloc<int>(nm); upload(d_.excl_count, excl_count);
     d_.excl = dalloc<int>(excl.size()); upload(d_.excl, excl);
-    d_.refx = dalloc<double>(nm); d_.refy = dalloc<double>(nm); d_.refz = dalloc<double>(nm);
+    d_.refx = dalloc<double>(nb); d_.refy = dalloc<double>(nb); d_.refz = dalloc<double>(nb);
     d_.rec = dalloc<PairRec>(nm);
     d_.slot_member = dalloc<int>(nm);
     d_.cell_count = dalloc<int>(d_.ncell);
@@ -592,6 +595,14 @@ void Engine::pack() {
     d_.row_count = dalloc<int>(nm);
     d_.row_start = dalloc<int>(nm);
     check_hip(hipMemset(d_.row_count, 0, sizeof(int) * std::max(nm, 1)), "memset");
+    if (prune_ && nm > 0) {
+        d_.row_count_in = dalloc<int>(nm);
+        check_hip(hipMemset(d_.row_count_in, 0, sizeof(int) * nm), "memset");
+        d_.prx = dalloc<double>(nb); d_.pry = dalloc<double>(nb); d_.prz = dalloc<double>(nb);
+        d_.prune_hm2 = prune_hm2_;
+    } else {
+        prune_ = 0;
+    }
     layout_rows();
     const int scan_n = std::max(d_.ncell, nm);
     d_.scan_partial = dalloc<int>(scan_n / 2048 + 2);
@@ -699,19 +710,27 @@ void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
         if (rebuild) { timed(3, [&] { launch_rebuild(d_, stream_); }); ++rebuilds_; }
         else timed(4, [&] { launch_pack(d_, stream_); });
     }
-    timed(0, [&] { launch_pair(d_, stream_, false, f, groups); });   // bonded terms included
+    // bonded terms included; a full evaluation also prunes the inner list
+    timed(0, [&] { launch_pair(d_, stream_, false, f, groups, kListPrune); });
 }
 
-void Engine::enqueue_chunk(int k, int interval) {
+// Step s of the chunk rebuilds the outer list when s % interval == 0 and
+// prunes the inner list when (s % interval) % prune == 0 (so at every
+// rebuild, and at the chunk's first step); the other steps read the inner list.
+void Engine::enqueue_chunk(int k, int interval, int prune) {
     for (int s = 0; s < k; ++s) {
         const bool rebuild = s % interval == 0;
+        const bool pruned = prune > 1 && (s % interval) % prune == 0;
+        const int pr = prune > 1 ? (pruned ? kPruneWrite : kPruneCheck) : kPruneNone;
         // the integrator refreshes the pair records itself on non-rebuild steps
-        timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0, rebuild ? 0 : 1, 1); });
+        timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0, rebuild ? 0 : 1, 1, pr); });
         if (d_.nm > 0 && rebuild) timed(3, [&] { launch_rebuild(d_, stream_); });
         unsigned long long* stamp =
             timing_mode_ == 2 ? kstamp_ + 2 * static_cast<size_t>(kstamp_grid_) * s : nullptr;
         // bonded terms run inside the pair kernel's force pass
-        timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu, stamp); });
+        if (pruned) timed(6, [&] { launch_prune(d_, stream_); });
+        const int list = prune > 1 ? kListInner : kListOuter;
+        timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu, list, stamp); });
     }
     timed(2, [&] { launch_half_kick(d_, stream_, k & 1); });
 }
@@ -754,6 +773,17 @@ void Engine::layout_rows() {
     d_.list_capacity = std::max<long long>(total, 4);
     d_.col = dalloc<int>(d_.list_capacity);
     col_bytes_ = d_.list_capacity * static_cast<long long>(sizeof(int));
+    if (d_.row_count_in) {
+        if (d_.col_in) {
+            auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(d_.col_in));
+            if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+            d_.col_in = nullptr;
+        }
+        d_.col_in = dalloc<int>(d_.list_capacity);
+        col_bytes_ *= 2;
+        // a widened layout leaves no valid inner rows until the next prune
+        check_hip(hipMemset(d_.row_count_in, 0, sizeof(int) * std::max(d_.nm, 1)), "memset");
+    }
 }
 
 // After a build reported rows longer than their slots: widen those groups to
@@ -833,9 +863,11 @@ void Engine::autotune_team() {
         build_pair_blocks(t);
         for (int g : grids) {
             d_.pair_grid = g;
-            launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);  // warm-up into the scratch buffer
+            // the inner list (pruned by the evaluation before) is what most steps read
+            const int list = prune_ ? kListInner : kListOuter;
+            launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu, list);  // warm-up into the scratch buffer
             check_hip(hipEventRecord(a, stream_), "event");
-            for (int r = 0; r < 3; ++r) launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu);
+            for (int r = 0; r < 3; ++r) launch_pair(d_, stream_, false, d_.f2, 0xFFFFFFFFu, list);
             check_hip(hipEventRecord(b, stream_), "event");
             check_hip(hipEventSynchronize(b), "event");
             float ms = 0;
@@ -863,7 +895,8 @@ int Engine::run_chunk(int k) {
         // kMaxGraphs shapes besides the full chunk, other shapes run eagerly.
         // Graphs with pair-kernel stamps are keyed apart (negative interval).
         constexpr std::size_t kMaxGraphs = 8;
-        const auto key = std::make_pair(k, timing_mode_ == 2 ? -interval : interval);
+        const int prune = prune_ > 1 ? prune_ : 0;
+        const auto key = std::make_tuple(k, timing_mode_ == 2 ? -interval : interval, prune);
         const bool graph = opt_.use_graph && !timing_ &&
                            (k == opt_.chunk_steps || graphs_.count(key) || graphs_.size() < kMaxGraphs);
         if (graph) {
@@ -871,7 +904,7 @@ int Engine::run_chunk(int k) {
             if (it == graphs_.end()) {
                 hipGraph_t g;
                 check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
-                enqueue_chunk(k, interval);
+                enqueue_chunk(k, interval, prune);
                 check_hip(hipStreamEndCapture(stream_, &g), "capture");
                 hipGraphExec_t ge;
                 check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
@@ -880,7 +913,7 @@ int Engine::run_chunk(int k) {
             }
             check_hip(hipGraphLaunch(it->second, stream_), "hipGraphLaunch");
         } else {
-            enqueue_chunk(k, interval);
+            enqueue_chunk(k, interval, prune);
         }
         if (d_.nm > 0) rebuilds_ += (k + interval - 1) / interval;
         sync();
@@ -894,9 +927,12 @@ int Engine::run_chunk(int k) {
             grow_rows();
             continue;
         }
-        bool stale = false, numeric = false;
-        for (int s = 0; s < d_.nsys; ++s)
-            if (!(err[s] & kErrNumeric) && (err[s] & kErrStale) && sys_status_[s] == OCM_OK) stale = true;
+        bool stale = false, numeric = false, pruned_stale = false;
+        for (int s = 0; s < d_.nsys; ++s) {
+            if ((err[s] & kErrNumeric) || sys_status_[s] != OCM_OK) continue;
+            if (err[s] & kErrStale) stale = true;
+            if (err[s] & kErrPrune) pruned_stale = true;
+        }
         if (stale && interval > 1) {
             // a bead moved more than skin/2 since the list was built: redo the
             // chunk with half the interval, and wait twice as long as last time
@@ -905,6 +941,15 @@ int Engine::run_chunk(int k) {
             interval_ = std::max(1, interval / 2);
             clean_chunks_ = 0;
             recover_after_ = std::min(recover_after_ * 2, 64);
+            continue;
+        }
+        if (pruned_stale && prune > 1) {
+            // the same for the inner list: a bead moved more than margin/2
+            // since the prune; below 2 the inner list is not used
+            restore();
+            prune_ = prune / 2 > 1 ? prune / 2 : 1;
+            prune_clean_ = 0;
+            prune_recover_after_ = std::min(prune_recover_after_ * 2, 64);
             continue;
         }
         // accepted: report this chunk's flags and those carried from before it
@@ -928,6 +973,11 @@ int Engine::run_chunk(int k) {
             interval_ = std::min(opt_.rebuild_interval, interval_ * 2);
             clean_chunks_ = 0;
             recover_after_ = std::max(4, recover_after_ / 2);
+        }
+        if (d_.col_in && prune_ < opt_.prune_interval && ++prune_clean_ >= prune_recover_after_) {
+            prune_ = std::min(opt_.prune_interval, prune_ * 2);
+            prune_clean_ = 0;
+            prune_recover_after_ = std::max(4, prune_recover_after_ / 2);
         }
         return numeric ? OCM_ERR_NUMERIC : OCM_OK;
     }
@@ -1148,6 +1198,18 @@ long long Engine::neighbor_pairs(int s, int force_index, int32_t* out, long long
     std::sort(v.begin(), v.end());
     for (long long k = 0; k < ncopy; ++k) { out[2 * k] = v[k].first; out[2 * k + 1] = v[k].second; }
     return cnt;
+}
+
+void Engine::inner_stats(long long* entries, int* prune) {
+    sync();
+    long long total = 0;
+    if (d_.row_count_in) {
+        std::vector<int> cnt(d_.nm, 0);
+        check_hip(hipMemcpy(cnt.data(), d_.row_count_in, sizeof(int) * d_.nm, hipMemcpyDeviceToHost), "rows");
+        for (int c : cnt) total += c;
+    }
+    *entries = total;
+    *prune = d_.col_in ? prune_ : 0;
 }
 
 void Engine::list_stats(long long* entries, long long* members, long long* rebuilds) {

@@ -10,6 +10,7 @@
 #include <array>
 #include <cstdint>
 #include <map>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -92,6 +93,7 @@ public:
     void set_timing(int mode);
     TimingSlot timing(int k) const { return timing_slots_.at(k); }
     void list_stats(long long* entries, long long* members, long long* rebuilds);
+    void inner_stats(long long* entries, int* prune);
 
 private:
     void pack();
@@ -100,7 +102,7 @@ private:
     void ensure_forces();
     void carry_errors();
     void enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups);
-    void enqueue_chunk(int k, int interval);
+    void enqueue_chunk(int k, int interval, int prune);
     int run_chunk(int k);
     void snapshot();
     void restore();
@@ -135,9 +137,13 @@ private:
     int interval_ = 20;
     int clean_chunks_ = 0;      // chunks since the interval was last shortened
     int recover_after_ = 4;     // clean chunks before the interval doubles back
+    int prune_ = 0;             // steps between inner-list prunes (0: no inner list; 1: suspended)
+    double prune_hm2_ = 1e300;  // (margin/2)^2 over all groups, the smallest
+    int prune_clean_ = 0;       // the same backoff for the prune interval
+    int prune_recover_after_ = 4;
     long long rebuilds_ = 0;
     bool timing_ = false;
-    std::array<TimingSlot, 6> timing_slots_{};
+    std::array<TimingSlot, 7> timing_slots_{};
     int timing_mode_ = 0;
     unsigned long long* kstamp_ = nullptr;   // 2 words per pair workgroup per step of a chunk
     unsigned long long* kstamp_red_ = nullptr;  // 2 words per step of a chunk
@@ -155,7 +161,7 @@ private:
     long long* bf_ = nullptr;
     long long* bstep_ = nullptr;
 
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs_;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_;   // (steps, interval, prune)
     long long col_bytes_ = 0;
     std::size_t device_bytes_ = 0;
 };

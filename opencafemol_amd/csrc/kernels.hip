@@ -69,38 +69,71 @@ __global__ void k_fill_i64(long long* p, long long v, int n) {
 }
 
 // ---------------------------------------------------- record refresh
-// Writes the pair record of every pair-group membership of bead b and, when
-// requested, flags the system if the bead moved more than skin/2 since the
-// last list build (the list is then rebuilt by the host's rollback).
+// The integrator refreshes the coordinates of every pair record of bead b
+// (type and bead id were written by the build) and, when requested, flags
+// the system if the bead moved more than skin/2 since the last list build
+// (the list is then rebuilt by the host's rollback).  Reference positions and
+// the (skin/2)^2 bound are per bead, so every input is loaded at once.
 struct PackArgs {
     int enabled;
     int check;
-    int maxm;
-    const int* bead_mem_count;
+    int maxm;              // member slots per bead in bead_slot (unused ones hold -1)
     const int* bead_slot;
-    const int* bead_type;
-    const double* bead_hs2;
-    const double* refx;
+    const double* bead_hs2;   // per bead: smallest (skin/2)^2 of its groups
+    const double* refx;       // per bead: position at the last build
     const double* refy;
     const double* refz;
     PairRec* rec;
+    double* prx;           // positions at the last inner-list prune, by bead
+    double* pry;
+    double* prz;
+    double prune_hm2;
+    int prune;             // kPruneNone / kPruneWrite / kPruneCheck
 };
 
-// Refreshes bead b's pair records and, when asked, checks that it has not
-// moved more than skin/2 since the list was built.
-__device__ inline void pack_bead(const PackArgs& pk, int b, int sys, double x, double y, double z, int* err) {
-    const int cnt = pk.bead_mem_count[b];
-    for (int w = 0; w < cnt; ++w) {
-        const int k = b * pk.maxm + w;
-        const int slot = pk.bead_slot[k];
-        PairRec r;
-        r.x = x; r.y = y; r.z = z;
-        r.type = pk.bead_type[k];
-        r.bead = b;
-        pk.rec[slot] = r;
+// Inputs of one bead's record refresh and checks, loaded before its step is
+// computed so the loads overlap the arithmetic.
+struct PackIn {
+    int slot0, slot1;
+    double rx, ry, rz, hs2;
+    double qx, qy, qz;
+};
+
+__device__ __forceinline__ PackIn pack_load(const PackArgs& pk, int b) {
+    PackIn in{-1, -1, 0, 0, 0, 0, 0, 0, 0};
+    if (!pk.enabled && pk.prune != kPruneCheck) return in;
+    in.slot0 = pk.bead_slot[static_cast<size_t>(b) * pk.maxm];
+    in.slot1 = pk.maxm > 1 ? pk.bead_slot[static_cast<size_t>(b) * pk.maxm + 1] : -1;
+    if (pk.enabled && pk.check) {
+        in.rx = pk.refx[b]; in.ry = pk.refy[b]; in.rz = pk.refz[b];
+        in.hs2 = pk.bead_hs2[b];
+    }
+    if (pk.prune == kPruneCheck) {
+        in.qx = pk.prx[b]; in.qy = pk.pry[b]; in.qz = pk.prz[b];
+    }
+    return in;
+}
+
+__device__ __forceinline__ void put_xyz(PairRec* rec, int slot, double x, double y, double z) {
+    double* p = reinterpret_cast<double*>(rec + slot);
+    p[0] = x; p[1] = y; p[2] = z;
+}
+
+__device__ inline void pack_bead(const PackArgs& pk, const PackIn& in, int b, int sys, double x, double y,
+                                 double z, int* err) {
+    if (pk.enabled && in.slot0 >= 0) {
+        put_xyz(pk.rec, in.slot0, x, y, z);
+        if (in.slot1 >= 0) {
+            put_xyz(pk.rec, in.slot1, x, y, z);
+            for (int w = 2; w < pk.maxm; ++w) {
+                const int slot = pk.bead_slot[static_cast<size_t>(b) * pk.maxm + w];
+                if (slot < 0) break;
+                put_xyz(pk.rec, slot, x, y, z);
+            }
+        }
         if (pk.check) {
-            const double dx = x - pk.refx[slot], dy = y - pk.refy[slot], dz = z - pk.refz[slot];
-            if (!(dist2(dx, dy, dz) <= pk.bead_hs2[k])) atomicOr(&err[sys], kErrStale);
+            const double dx = x - in.rx, dy = y - in.ry, dz = z - in.rz;
+            if (!(dist2(dx, dy, dz) <= in.hs2)) atomicOr(&err[sys], kErrStale);
         }
     }
 }
@@ -150,10 +183,14 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     const int s = single ? 0 : sys_of[b];
     const DevSys S = single ? sys0 : sys[s];
     const double im = inv_m[b];
+    const PackIn pin_ = pack_load(pk, b);
     const double f0 = fx[b] * kFixInv, f1 = fy[b] * kFixInv, f2 = fz[b] * kFixInv;
     fx[b] = 0; fy[b] = 0; fz[b] = 0;
     if (im == 0.0) {
-        if (pk.enabled) pack_bead(pk, b, s, px[b], py[b], pz[b], err);
+        if (pk.prune == kPruneWrite) {
+            pk.prx[b] = px[b]; pk.pry[b] = py[b]; pk.prz[b] = pz[b];
+        }
+        pack_bead(pk, pin_, b, s, px[b], py[b], pz[b], err);
         return;
     }
     double v0 = vx[b], v1 = vy[b], v2 = vz[b];
@@ -180,7 +217,15 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) atomicOr(&err[s], kErrNumeric);
     px[b] = x0; py[b] = x1; pz[b] = x2;
     vx[b] = v0; vy[b] = v1; vz[b] = v2;
-    if (pk.enabled) pack_bead(pk, b, s, x0, x1, x2, err);
+    // inner list: a pair dropped at the prune (r >= cutoff + margin) cannot be
+    // inside the cutoff while both beads stay within margin/2 of where they were
+    if (pk.prune == kPruneWrite) {
+        pk.prx[b] = x0; pk.pry[b] = x1; pk.prz[b] = x2;
+    } else if (pk.prune == kPruneCheck && pin_.slot0 >= 0) {
+        const double dx = x0 - pin_.qx, dy = x1 - pin_.qy, dz = x2 - pin_.qz;
+        if (!(dist2(dx, dy, dz) <= pk.prune_hm2)) atomicOr(&err[s], kErrPrune);
+    }
+    pack_bead(pk, pin_, b, s, x0, x1, x2, err);
 }
 
 // Closing half kick; also resets both global-step slots to the value reached.
@@ -222,7 +267,7 @@ __global__ void k_pack(int nm, const DevPairGroup* __restrict__ groups,
     r.bead = b;
     rec[slot] = r;
     const DevPairGroup& G = groups[member_group[m]];
-    const double dx = r.x - refx[slot], dy = r.y - refy[slot], dz = r.z - refz[slot];
+    const double dx = r.x - refx[b], dy = r.y - refy[b], dz = r.z - refz[b];
     if (!(dist2(dx, dy, dz) <= G.half_skin2)) atomicOr(&err[G.sys], kErrStale);
 }
 
@@ -290,7 +335,7 @@ __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* 
     r.type = member_type[m];
     r.bead = b;
     rec[slot] = r;
-    refx[slot] = r.x; refy[slot] = r.y; refz[slot] = r.z;
+    refx[b] = r.x; refy[b] = r.y; refz[b] = r.z;   // every member of b writes the same values
 }
 
 __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, int mj) {
@@ -938,11 +983,15 @@ __device__ inline RowType row_type(const float* __restrict__ t) {
 // Hatch) they are evaluated together: one set of sigma powers, one
 // (24 s^6 - 48 s^12)/r^2 factor scaled by the repulsive epsilon inside r_min
 // and by the attractive one outside, the switch on the attractive part.
-__device__ inline float pair_force(int kinds, const PairK<float>& G, const RowType& ti,
-                                   const float* __restrict__ tj, float r2) {
+// `kinds`, `shared` and `lsw` (the attractive switch is on) are compile-time
+// constants in the specialized row loops, which then hold no branches; the
+// switch polynomial is evaluated at x = max(0, (r - rsw)/w), where x = 0 gives
+// S = 1 and dS = 0, the unswitched value exactly.
+__device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, const PairK<float>& G,
+                                            const RowType& ti, const float* __restrict__ tj, float r2) {
     const float ir2 = __builtin_amdgcn_rcpf(r2);
     float f = 0.f, r = 0.f, ir = 0.f;
-    if ((kinds & kDh) || ((kinds & kLja) && G.lja_switch)) {
+    if ((kinds & kDh) || ((kinds & kLja) && lsw)) {
         ir = __builtin_amdgcn_rsqf(r2);
         r = r2 * ir;
     }
@@ -958,7 +1007,7 @@ __device__ inline float pair_force(int kinds, const PairK<float>& G, const RowTy
         f += -qq * G.dh_K * ex * (ir + G.dh_il) * ir2;
     }
     const float c13 = 1.2599210498948731648f;  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
-    if ((kinds & (kLjr | kLja)) == (kLjr | kLja) && G.lj_shared) {
+    if ((kinds & (kLjr | kLja)) == (kLjr | kLja) && shared) {
         const float4 lj = *reinterpret_cast<const float4*>(tj + kTyLjrEps);   // er, sr, ea, sa
         const float sb = 0.5f * (ti.sr + lj.y);
         const float sb2 = sb * sb;
@@ -967,15 +1016,13 @@ __device__ inline float pair_force(int kinds, const PairK<float>& G, const RowTy
         const float th2 = c13 * sb2;
         const float X = (24.f * s6 - 48.f * s12) * ir2;
         const float er = 0.5f * (ti.er + lj.x), ea = 0.5f * (ti.ea + lj.z);
-        if (th2 - r2 >= 0.f) f += er * X;
-        float u = 0.f, du = 0.f;
-        if (r2 - th2 >= 0.f) {
-            u = 4.f * ea * (s12 - s6);
-            du = ea * X;
-        }
-        if (th2 - r2 >= 0.f) u -= ea;
-        if (G.lja_switch && r > G.lja_rsw) {
-            const float x = (r - G.lja_rsw) * G.lja_inv_sw;
+        const bool inner = th2 - r2 >= 0.f, outer = r2 - th2 >= 0.f;
+        f += inner ? er * X : 0.f;
+        float u = outer ? 4.f * ea * (s12 - s6) : 0.f;
+        float du = outer ? ea * X : 0.f;
+        u = inner ? u - ea : u;
+        if (lsw) {
+            const float x = fmaxf((r - G.lja_rsw) * G.lja_inv_sw, 0.f);
             const float x2 = x * x, x3 = x2 * x;
             const float S = 1.f - x3 * (10.f - 15.f * x + 6.f * x2);
             const float dS = -30.f * x2 * (1.f - x) * (1.f - x) * G.lja_inv_sw;
@@ -1009,8 +1056,8 @@ __device__ inline float pair_force(int kinds, const PairK<float>& G, const RowTy
             du += eb * (-48.f * s12 + 24.f * s6) * ir2;
         }
         if (th2 - r2 >= 0.f) u -= eb;
-        if (G.lja_switch && r > G.lja_rsw) {
-            const float x = (r - G.lja_rsw) * G.lja_inv_sw;
+        if (lsw) {
+            const float x = fmaxf((r - G.lja_rsw) * G.lja_inv_sw, 0.f);
             const float x2 = x * x, x3 = x2 * x;
             const float S = 1.f - x3 * (10.f - 15.f * x + 6.f * x2);
             const float dS = -30.f * x2 * (1.f - x) * (1.f - x) * G.lja_inv_sw;
@@ -1044,6 +1091,165 @@ struct BondedArgs {
     const FlaTable* fla;
     const FldTable* fld;
 };
+
+// Force-pass row loops, specialized by pair-group kind so the per-entry body
+// has no branches and the four entries of a round interleave.
+constexpr int kSpecGeneric = 0, kSpecExvPeriodic = 1, kSpecExvOpen = 2, kSpecDh = 3, kSpecLjShared = 4;
+
+struct RowGeom {
+    double bx, by, bz, ibx, iby, ibz, rc2, rin2;
+    bool periodic;
+};
+
+struct RowAcc {
+    unsigned long long ux = 0, uy = 0, uz = 0, nadd = 0;   // raw fma bit patterns
+    int bad = 0;
+};
+
+// One member row for lanes `lane` of a team of TEAM: forces of its entries
+// in fixed point (see kFixMagic).
+template <int TEAM, int SPEC>
+__device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const PairK<float>& Kf, const RowType& rt,
+                                          const PairRec& ri, int row, int beg, int cnt, int lane,
+                                          const PairRec* __restrict__ rec, const int* __restrict__ col,
+                                          const float* __restrict__ lds_types, RowAcc& acc) {
+    // A round covers 4*TEAM entries of the row; lane l takes entries l, l+T,
+    // l+2T, l+3T of it, so the lanes of one gather read neighbouring entries,
+    // which point at neighbouring (cell-ordered) records: a few cache lines
+    // per gather instead of one per lane.  Entries past the row end read as
+    // -1 (padding).  The round's 4 record gathers are all issued before any
+    // of them is used (the scheduling barrier keeps the arithmetic below
+    // them), and the next round's column indices are fetched while this
+    // round computes.
+    const int* crow = col + beg;
+    int cn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
+    for (int base = 0; base < cnt; base += 4 * TEAM) {
+        const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
+        PairRec rjs[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rjs[u] = rec[cs[u] >= 0 ? cs[u] : row];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = base + 4 * TEAM + lane + u * TEAM;
+            cn[u] = e < cnt ? crow[e] : -1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        acc.nadd += 4;
+        // the four entries are evaluated without branches (an entry that is
+        // padding or beyond the cutoff is computed at r2 = 1 and masked to
+        // zero) so their dependency chains interleave
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int s = cs[u];
+            const PairRec rj = rjs[u];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+            // a group without a box has inverse box 0, for which the minimum
+            // image is the identity
+            if (SPEC == kSpecGeneric ? g.periodic : SPEC != kSpecExvOpen) {
+                dx = min_image(dx, g.bx, g.ibx);
+                dy = min_image(dy, g.by, g.iby);
+                dz = min_image(dz, g.bz, g.ibz);
+            }
+            const double r2 = dist2(dx, dy, dz);
+            const bool in = s >= 0 && r2 < g.rc2;
+            const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
+            const float r2f = in ? static_cast<float>(r2) : 1.0f;
+            float f;
+            if (SPEC == kSpecExvPeriodic || SPEC == kSpecExvOpen) f = pair_force(kExv, false, false, Kf, rt, tfj, r2f);
+            else if (SPEC == kSpecDh) f = pair_force(kDh, false, false, Kf, rt, tfj, r2f);
+            else if (SPEC == kSpecLjShared) f = pair_force(kLjr | kLja, true, true, Kf, rt, tfj, r2f);
+            else f = pair_force(kinds, Kf.lj_shared, Kf.lja_switch, Kf, rt, tfj, r2f);
+            // |f d| <= |f| r: one range test covers the three components.  A
+            // pair beyond the limit is flagged and clamped to it along its
+            // direction (so relaxation still pushes the beads apart); NaN
+            // contributes nothing.
+            const bool ok = f * f * r2f < kPairLimit * kPairLimit;
+            acc.bad |= in && !ok;
+            float gg = ok ? f : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
+            gg = in ? gg : 0.0f;   // a masked entry adds exactly the bias constant
+            const double gs = static_cast<double>(gg) * 4294967296.0;
+            acc.ux += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic)));
+            acc.uy += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic)));
+            acc.uz += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic)));
+        }
+    }
+}
+
+// Inner-list prune: each member row's outer entries within cutoff + margin,
+// written in order to the same row slots of the inner list (col_w) and padded
+// to a multiple of 4; a team of TEAM lanes per row, as in the force pass.
+template <int TEAM>
+__global__ __launch_bounds__(kPairBlock) void k_prune(const DevPairGroup* __restrict__ groups,
+                                                      const int2* __restrict__ blocks,
+                                                      const PairRec* __restrict__ rec,
+                                                      const int* __restrict__ row_start,
+                                                      const int* __restrict__ row_count,
+                                                      const int* __restrict__ col, int* __restrict__ row_count_w,
+                                                      int* __restrict__ col_w, int nblocks) {
+    const int wl = static_cast<int>(threadIdx.x & 63u);
+    // the team's lanes within the wave, and those below this lane
+    const unsigned long long team_lanes = TEAM == 64 ? ~0ull : (((1ull << TEAM) - 1ull) << (wl / TEAM * TEAM));
+    const unsigned long long below = (1ull << wl) - 1ull;
+    const int lane = threadIdx.x % TEAM;
+    for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+        const int2 bt = blocks[bi];
+        if (bt.x < 0) continue;   // bonded chunk
+        const DevPairGroup& G = groups[bt.x];
+        const double bx = G.box[0], by = G.box[1], bz = G.box[2];
+        const double ibx = G.inv_box[0], iby = G.inv_box[1], ibz = G.inv_box[2];
+        const double rin2 = G.rin2;
+        const int row = bt.y + threadIdx.x / TEAM;
+        const bool live = row < G.member_begin + G.member_count;
+        // every lane of a team runs the same rounds (the loop bound is the
+        // row's), so the ballots below see the whole team
+        int n = 0;   // entries the team has written to the inner row
+        int beg = 0;
+        if (live) {
+            const double xi = rec[row].x, yi = rec[row].y, zi = rec[row].z;
+            beg = row_start[row];
+            const int cnt = row_count[row];
+            // lane l takes entries l, l+T, l+2T, l+3T of each round of 4T (as
+            // in the force pass); kept entries are written in list order
+            const int* crow = col + beg;
+            int cn[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
+            for (int base = 0; base < cnt; base += 4 * TEAM) {
+                const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
+                double px[4], py[4], pz[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const PairRec* r = rec + (cs[u] >= 0 ? cs[u] : row);
+                    px[u] = r->x; py[u] = r->y; pz[u] = r->z;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = base + 4 * TEAM + lane + u * TEAM;
+                    cn[u] = e < cnt ? crow[e] : -1;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double dx = min_image(px[u] - xi, bx, ibx);
+                    const double dy = min_image(py[u] - yi, by, iby);
+                    const double dz = min_image(pz[u] - zi, bz, ibz);
+                    const bool keep = cs[u] >= 0 && dist2(dx, dy, dz) < rin2;
+                    // entry base + u*T + l precedes base + u*T + l' for l < l'
+                    const unsigned long long m = __ballot(keep) & team_lanes;
+                    if (keep) col_w[beg + n + __popcll(m & below)] = cs[u];
+                    n += __popcll(m);
+                }
+            }
+        }
+        if (lane == 0 && live) {
+            int p = n;
+            for (; p & 3; ++p) col_w[beg + p] = -1;
+            row_count_w[row] = p;
+        }
+    }
+}
 
 // At least 5 waves per SIMD (<= 96 VGPRs): the record gathers need the latency hiding.
 template <bool ENERGY, int TEAM>
@@ -1081,10 +1287,8 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
         continue;
     }
     const DevPairGroup& G = groups[bt.x];
-    const bool periodic = G.periodic;
-    const double bx = G.box[0], by = G.box[1], bz = G.box[2];
-    const double ibx = G.inv_box[0], iby = G.inv_box[1], ibz = G.inv_box[2];
-    const double rc2 = G.rc2;
+    const RowGeom geo{G.box[0], G.box[1], G.box[2], G.inv_box[0], G.inv_box[1], G.inv_box[2], G.rc2, G.rin2,
+                      G.periodic != 0};
     const PairK<float> Kf = pair_constants<float>(G);
     const PairK<double> Kd = pair_constants<double>(G);
     if (!ENERGY && bt.x != loaded) {  // bt.x is uniform over the workgroup
@@ -1101,73 +1305,71 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
     const int lane = threadIdx.x % TEAM;
     const int row = bt.y + threadIdx.x / TEAM;
     const int row_end = G.member_begin + G.member_count;
-    unsigned long long ux = 0, uy = 0, uz = 0, nadd = 0;   // raw fma bit patterns, force pass
+    RowAcc acc;
     long long ae0 = 0, ae1 = 0, ae2 = 0, ae3 = 0;
-    int bad = 0;
     int bead_i = -1;
     if (row < row_end && kinds) {
         const PairRec ri = rec[row];
         bead_i = ri.bead;
         const int beg = row_start[row], cnt = row_count[row];
         const int ti = ri.type & 0xFFFF;
-        const float* tfi = lds_types + ti * kTypeStride;
-        const double* tdi = types_d + static_cast<size_t>(G.type_begin + ti) * kTypeStride;
-        RowType rt{};
-        if (!ENERGY) rt = row_type(tfi);
-        // rows are padded to multiples of 4 entries (-1 = empty); a lane takes
-        // 4 consecutive entries per round so 4 record gathers are in flight
-        const int4* col4 = reinterpret_cast<const int4*>(col + beg);
-        for (int k = lane; 4 * k < cnt; k += TEAM) {
-            const int4 c4 = col4[k];
-            const int cs[4] = {c4.x, c4.y, c4.z, c4.w};
-            PairRec rjs[4];
+        if (!ENERGY) {
+            const RowType rt = row_type(lds_types + ti * kTypeStride);
+            // the group's kind selects the loop (uniform over the workgroup)
+            int spec = kSpecGeneric;
+            if (kinds == kExv) spec = geo.periodic ? kSpecExvPeriodic : kSpecExvOpen;
+            else if (kinds == kDh && geo.periodic) spec = kSpecDh;
+            else if (kinds == (kLjr | kLja) && Kf.lj_shared && Kf.lja_switch && geo.periodic) spec = kSpecLjShared;
+            switch (spec) {
+            case kSpecExvPeriodic:
+                force_row<TEAM, kSpecExvPeriodic>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                                                         lds_types, acc);
+                break;
+            case kSpecExvOpen:
+                force_row<TEAM, kSpecExvOpen>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                                                     lds_types, acc);
+                break;
+            case kSpecDh:
+                force_row<TEAM, kSpecDh>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                                                lds_types, acc);
+                break;
+            case kSpecLjShared:
+                force_row<TEAM, kSpecLjShared>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                                                      lds_types, acc);
+                break;
+            default:
+                force_row<TEAM, kSpecGeneric>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                                                     lds_types, acc);
+                break;
+            }
+        } else {
+            // energy pass: FP64 terms of each pair once (row < column)
+            const double* tdi = types_d + static_cast<size_t>(G.type_begin + ti) * kTypeStride;
+            const int4* col4 = reinterpret_cast<const int4*>(col + beg);
+            for (int k = lane; 4 * k < cnt; k += TEAM) {
+                const int4 c4 = col4[k];
+                const int cs[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) rjs[u] = rec[cs[u] >= 0 ? cs[u] : row];
-            if (!ENERGY) nadd += 4;
-            // the force pass evaluates the four entries without branches (an
-            // entry that is padding or beyond the cutoff is computed at r2 = 1
-            // and masked to zero) so their dependency chains interleave
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int s = cs[u];
-                const PairRec rj = rjs[u];
-                double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-                if (periodic) {
-                    dx = min_image(dx, bx, ibx);
-                    dy = min_image(dy, by, iby);
-                    dz = min_image(dz, bz, ibz);
-                }
-                const double r2 = dist2(dx, dy, dz);
-                const bool in = s >= 0 && r2 < rc2;
-                if (ENERGY) {
-                    if (!in) continue;
-                    if (row < s) {
-                        double f;
-                        double e[4] = {0, 0, 0, 0};
-                        const double* tdj = types_d + static_cast<size_t>(G.type_begin + (rj.type & 0xFFFF)) * kTypeStride;
-                        pair_eval<double>(kinds, Kd, nullptr, nullptr, tdi, tdj, r2, &f, e);
-                        ae0 += to_fix_d(e[0], bad);
-                        ae1 += to_fix_d(e[1], bad);
-                        ae2 += to_fix_d(e[2], bad);
-                        ae3 += to_fix_d(e[3], bad);
+                for (int u = 0; u < 4; ++u) {
+                    const int s = cs[u];
+                    if (s < 0 || row >= s) continue;
+                    const PairRec rj = rec[s];
+                    double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+                    if (geo.periodic) {
+                        dx = min_image(dx, geo.bx, geo.ibx);
+                        dy = min_image(dy, geo.by, geo.iby);
+                        dz = min_image(dz, geo.bz, geo.ibz);
                     }
-                } else {
-                    const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
-                    const float r2f = in ? static_cast<float>(r2) : 1.0f;
-                    const float f = pair_force(kinds, Kf, rt, tfj, r2f);
-                    // |f d| <= |f| r: one range test covers the three
-                    // components.  A pair beyond the limit is flagged and
-                    // clamped to it along its direction (so relaxation still
-                    // pushes the beads apart); NaN contributes nothing.
-                    const bool ok = f * f * r2f < kPairLimit * kPairLimit;
-                    bad |= in && !ok;
-                    float g = ok ? f
-                                 : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
-                    g = in ? g : 0.0f;   // a masked entry adds exactly the bias constant
-                    const double gs = static_cast<double>(g) * 4294967296.0;
-                    ux += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic)));
-                    uy += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic)));
-                    uz += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic)));
+                    const double r2 = dist2(dx, dy, dz);
+                    if (!(r2 < geo.rc2)) continue;
+                    double f;
+                    double e[4] = {0, 0, 0, 0};
+                    const double* tdj = types_d + static_cast<size_t>(G.type_begin + (rj.type & 0xFFFF)) * kTypeStride;
+                    pair_eval<double>(kinds, Kd, nullptr, nullptr, tdi, tdj, r2, &f, e);
+                    ae0 += to_fix_d(e[0], acc.bad);
+                    ae1 += to_fix_d(e[1], acc.bad);
+                    ae2 += to_fix_d(e[2], acc.bad);
+                    ae3 += to_fix_d(e[3], acc.bad);
                 }
             }
         }
@@ -1186,10 +1388,10 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
                               static_cast<unsigned long long>(v[k]));
         }
     } else {
-        const unsigned long long off = nadd * kFixMagicBits;
-        long long ax = static_cast<long long>(ux - off);
-        long long ay = static_cast<long long>(uy - off);
-        long long az = static_cast<long long>(uz - off);
+        const unsigned long long off = acc.nadd * kFixMagicBits;
+        long long ax = static_cast<long long>(acc.ux - off);
+        long long ay = static_cast<long long>(acc.uy - off);
+        long long az = static_cast<long long>(acc.uz - off);
         ax = warp_sum(ax, TEAM);
         ay = warp_sum(ay, TEAM);
         az = warp_sum(az, TEAM);
@@ -1199,7 +1401,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
             atomicAdd(reinterpret_cast<unsigned long long*>(&out.fz[bead_i]), static_cast<unsigned long long>(az));
         }
     }
-    if (bad) atomicOr(&err[G.sys], kErrNumeric);
+    if (acc.bad) atomicOr(&err[G.sys], kErrNumeric);
     }  // block table
     if (stamp) {
         __syncthreads();
@@ -1317,9 +1519,10 @@ void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st) {
     k_fill_i64<<<nblk(d.nb, 256), 256, 0, st>>>(f.fz, 0, d.nb);
 }
 
-void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check) {
-    PackArgs pk{pack && d.nm > 0, check, d.maxm, d.bead_mem_count, d.bead_slot, d.bead_type, d.bead_hs2,
-                d.refx, d.refy, d.refz, d.rec};
+void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check,
+                      int prune) {
+    PackArgs pk{pack && d.nm > 0, check, d.maxm, d.bead_slot, d.bead_hs2, d.refx, d.refy, d.refz, d.rec,
+                d.prx, d.pry, d.prz, d.prune_hm2, d.nm > 0 && d.col_in ? prune : kPruneNone};
     k_integrate<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
                                                  d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
                                                  pin, pout, pending, d.err, pk, d.sys0, d.nsys == 1 ? 1 : 0);
@@ -1418,13 +1621,13 @@ void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf
 
 template <bool E>
 static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, uint32_t groups,
-                          unsigned long long* stamp) {
+                          const int* count, const int* col, unsigned long long* stamp) {
     const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
     const BondedArgs bd{d.nt, d.terms, d.sys, d.sys_of, d.px, d.py, d.pz, d.fla, d.fld};
     const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
-#define OCM_PAIR(T)                                                                                   \
-    k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, \
-                                           d.col, d.types_f, d.types_d, f, d.energy, groups, d.err,   \
+#define OCM_PAIR(T)                                                                                     \
+    k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col,     \
+                                           d.types_f, d.types_d, f, d.energy, groups, d.err,            \
                                            d.n_pair_blocks, stamp, bd)
     switch (d.team) {
     case 4: OCM_PAIR(4); break;
@@ -1436,11 +1639,35 @@ static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, u
 #undef OCM_PAIR
 }
 
+void launch_prune(const DevPtrs& d, hipStream_t st) {
+    if (d.n_pair_blocks == 0 || !d.col_in) return;
+    const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
+#define OCM_PRUNE(T)                                                                                      \
+    k_prune<T><<<grid, block, 0, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, d.col,   \
+                                       d.row_count_in, d.col_in, d.n_pair_blocks)
+    switch (d.team) {
+    case 4: OCM_PRUNE(4); break;
+    case 8: OCM_PRUNE(8); break;
+    case 32: OCM_PRUNE(32); break;
+    case 64: OCM_PRUNE(64); break;
+    default: OCM_PRUNE(16); break;
+    }
+#undef OCM_PRUNE
+}
+
 void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups,
-                 unsigned long long* stamp) {
+                 int list, unsigned long long* stamp) {
     if (d.n_pair_blocks == 0) return;
-    if (energy) pair_dispatch<true>(d, st, f, groups, stamp);
-    else pair_dispatch<false>(d, st, f, groups, stamp);
+    // without an inner list, or for a subset of groups or the energy pass, the
+    // outer list is read and nothing is written
+    if (!d.col_in || energy || (list == kListPrune && groups != 0xFFFFFFFFu)) list = kListOuter;
+    if (energy) {
+        pair_dispatch<true>(d, st, f, groups, d.row_count, d.col, stamp);
+        return;
+    }
+    if (list == kListPrune) launch_prune(d, st);   // then the force pass reads the fresh inner list
+    if (list == kListOuter) pair_dispatch<false>(d, st, f, groups, d.row_count, d.col, stamp);
+    else pair_dispatch<false>(d, st, f, groups, d.row_count_in, d.col_in, stamp);
 }
 
 int pair_grid_size(const DevPtrs& d) { return std::min(d.n_pair_blocks, d.pair_grid); }

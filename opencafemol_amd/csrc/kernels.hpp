@@ -37,20 +37,24 @@ struct DevPtrs {
     DevPairGroup* groups = nullptr;
     int *member_group = nullptr, *member_bead = nullptr, *member_type = nullptr, *member_cat = nullptr;
     int *member_slot = nullptr, *member_cell = nullptr, *member_rank = nullptr;
-    // bead -> its pair-group members, `maxm` slots per bead: count, current
-    // record slot, type word and (skin/2)^2 of the member's group
+    // bead -> its pair-group members, `maxm` slots per bead: the current record
+    // slot of each (-1 past the bead's last membership) and the smallest
+    // (skin/2)^2 of its groups
     int maxm = 1;
-    int* bead_mem_count = nullptr;
     int* bead_slot = nullptr;
-    int* bead_type = nullptr;
     double* bead_hs2 = nullptr;
     int* member_kpos = nullptr;   // member -> its position in the bead tables
     int *excl_start = nullptr, *excl_count = nullptr, *excl = nullptr;
-    double *refx = nullptr, *refy = nullptr, *refz = nullptr;   // positions at the last build, by slot
+    double *refx = nullptr, *refy = nullptr, *refz = nullptr;   // positions at the last build, by bead
     PairRec* rec = nullptr;
     int* slot_member = nullptr;
     int *cell_count = nullptr, *cell_start = nullptr, *cell_group = nullptr;
     int *row_count = nullptr, *row_start = nullptr, *col = nullptr;
+    // inner list: the outer rows pruned to cutoff + margin, same row layout
+    int *row_count_in = nullptr, *col_in = nullptr;
+    double *prx = nullptr, *pry = nullptr, *prz = nullptr;   // positions at the last prune, by bead
+    double prune_hm2 = 0.0;        // (margin/2)^2, slightly reduced
+
     long long list_capacity = 0;
     int2* tile_items = nullptr;    // (cell, chunk) work items of the list build
     int* tile_nitems = nullptr;
@@ -71,18 +75,30 @@ struct DevPtrs {
 };
 
 constexpr int kPairBlock = 128;
+// Inner-list margin beyond the cutoff (nm), capped by the group's skin.
+constexpr double kPruneMargin = 0.1;
 
 void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st);
-// pack != 0: also refresh the pair records (and check list validity when check != 0)
-void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check);
+// pack != 0: also refresh the pair records (and check list validity when check != 0).
+// prune: kPruneNone, kPruneWrite (this step's pair pass prunes: record the
+// positions) or kPruneCheck (flag beads that moved more than margin/2 since).
+constexpr int kPruneNone = 0, kPruneWrite = 1, kPruneCheck = 2;
+void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check,
+                      int prune = kPruneNone);
 void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur);
 void launch_pack(const DevPtrs& d, hipStream_t st);
 void launch_rebuild(const DevPtrs& d, hipStream_t st);
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups);
 // stamp (optional): 2 * pair_grid_size words; workgroup w stores its start
 // and end on the device wall clock at stamp[2w], stamp[2w + 1].
+// list: kListOuter reads the outer list; kListPrune reads it and writes the
+// inner list (full group mask only; otherwise it reads the outer list);
+// kListInner reads the inner list.
+constexpr int kListOuter = 0, kListPrune = 1, kListInner = 2;
 void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups,
-                 unsigned long long* stamp = nullptr);
+                 int list = kListOuter, unsigned long long* stamp = nullptr);
+// Rewrites the inner list from the outer one at the current positions.
+void launch_prune(const DevPtrs& d, hipStream_t st);
 int pair_grid_size(const DevPtrs& d);
 void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int nlaunch, int nblk,
                           unsigned long long* out);

@@ -87,7 +87,8 @@ std::size_t Server::estimate_bytes(const HostSystem& s) const {
         } else {
             per_bead = std::min(s.n, 256);
         }
-        bytes += s.n * (4.0 * per_bead + 48.0) + 8.0 * f.exclusions.size();
+        const double list_bytes = opt_.engine.prune_interval > 1 ? 8.0 : 4.0;   // outer (+ inner) list
+        bytes += s.n * (list_bytes * per_bead + 48.0) + 8.0 * f.exclusions.size();
     }
     return static_cast<std::size_t>(bytes) + (1u << 20);
 }

@@ -56,7 +56,7 @@ def test_options_default_and_version():
     lib = abi.load_library()
     o = abi.Options()
     lib.ocm_options_default(C.byref(o))
-    assert (o.rebuild_interval, o.skin, o.use_graph, o.chunk_steps) == (20, 0.4, 1, 200)
+    assert (o.rebuild_interval, o.skin, o.use_graph, o.chunk_steps, o.prune_interval) == (40, 0.6, 1, 200, 5)
     assert b"gfx950" in lib.ocm_version()
 
 
@@ -132,7 +132,7 @@ def test_server_options_default():
     o = abi.ServerOptions()
     lib.ocm_server_options_default(C.byref(o))
     assert (o.hbm_budget_bytes, o.max_active_jobs, o.max_queued_frames, o.quantum_steps) == (0, 256, 4096, 2000)
-    assert (o.engine.rebuild_interval, o.engine.chunk_steps) == (20, 200)
+    assert (o.engine.rebuild_interval, o.engine.chunk_steps) == (40, 200)
 
 
 def test_server_reports_no_device_on_a_cpu_host():
