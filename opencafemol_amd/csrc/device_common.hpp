@@ -82,6 +82,8 @@ struct DevPairGroup {
     double rl2;            // (cutoff + skin)^2
     double half_skin2;     // (skin/2)^2
     double rin2;           // (cutoff + margin)^2: radius of the pruned inner list
+    double rin2_wide;      // periodic groups: the FP32 prune's threshold, (cutoff + margin + 2 delta)^2
+                           // with delta a bound on its per-axis coordinate error
     double rc;
     double box[3], inv_box[3];
     double cell_scale[3];  // ncell / box

@@ -371,6 +371,15 @@ void Engine::pack() {
             G.half_skin2 = 0.25 * skin * skin;
             const double margin = std::min(kPruneMargin, skin);
             G.rin2 = (f0.cutoff + margin) * (f0.cutoff + margin);
+            G.rin2_wide = G.rin2;
+            if (G.periodic) {
+                // the FP32 prune's wrapped coordinates lie in [0, L): each axis of
+                // a difference is off by at most 2.5 ulp(L); 4 ulp per axis bounds it
+                const double L = std::max({H.box[0], H.box[1], H.box[2]});
+                const double delta = 4.0 * std::ldexp(1.0, std::ilogb(L) - 23);
+                const double rw = f0.cutoff + margin + 2.0 * delta;
+                G.rin2_wide = rw * rw;
+            }
             prune_hm2_ = std::min(prune_hm2_, 0.25 * margin * margin * (1.0 - 1e-6));
             for (int c = 0; c < 3; ++c) {
                 G.box[c] = H.box[c];
@@ -429,7 +438,10 @@ void Engine::pack() {
                     const double row[kTypeStride] = {tup[0], tup[1], 0.0, 0.0, tup[2], tup[3], tup[4], tup[5]};
                     for (int k = 0; k < kTypeStride; ++k) {
                         types_d.push_back(row[k]);
-                        types_f.push_back(static_cast<float>(row[k]));
+                        // the force pass's FP32 rows hold half of each LJ epsilon and
+                        // sigma (its mixing rules are then single adds)
+                        const float v = static_cast<float>(row[k]);
+                        types_f.push_back(k >= kTyLjrEps ? 0.5f * v : v);
                     }
                 } else {
                     ty = it->second;
@@ -499,6 +511,8 @@ void Engine::pack() {
             groups.push_back(G);
         }
     }
+    if (member_bead.size() >= (1u << 26))   // record offsets are 32-bit byte offsets
+        throw Error(OCM_ERR_OOM, "more than 2^26 pair-group members");
     d_.nm = static_cast<int>(member_bead.size());
     d_.ncell = std::max(ncell_total, 1);
     d_.ngroups = static_cast<int>(groups.size());
@@ -599,6 +613,7 @@ void Engine::pack() {
         d_.row_count_in = dalloc<int>(nm);
         check_hip(hipMemset(d_.row_count_in, 0, sizeof(int) * nm), "memset");
         d_.prx = dalloc<double>(nb); d_.pry = dalloc<double>(nb); d_.prz = dalloc<double>(nb);
+        d_.rec32 = dalloc<float4>(nm);
         d_.prune_hm2 = prune_hm2_;
     } else {
         prune_ = 0;
@@ -875,6 +890,24 @@ void Engine::autotune_team() {
             if (ms < best) { best = ms; best_team = t; best_grid = g; }
             if (g >= d_.n_pair_blocks) break;
         }
+    }
+    // the prune's team, on the same list (it rewrites the inner rows with the
+    // same content each time)
+    if (d_.col_in) {
+        float best_p = 1e30f;
+        int best_pt = d_.prune_team;
+        for (int t : {16, 32, 64}) {
+            d_.prune_team = t;
+            launch_prune(d_, stream_);
+            check_hip(hipEventRecord(a, stream_), "event");
+            for (int r = 0; r < 3; ++r) launch_prune(d_, stream_);
+            check_hip(hipEventRecord(b, stream_), "event");
+            check_hip(hipEventSynchronize(b), "event");
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, a, b), "event");
+            if (ms < best_p) { best_p = ms; best_pt = t; }
+        }
+        d_.prune_team = best_pt;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

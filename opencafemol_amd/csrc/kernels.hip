@@ -84,6 +84,7 @@ struct PackArgs {
     const double* refy;
     const double* refz;
     PairRec* rec;
+    float4* rec32;         // written on prune steps for periodic systems
     double* prx;           // positions at the last inner-list prune, by bead
     double* pry;
     double* prz;
@@ -101,7 +102,7 @@ struct PackIn {
 
 __device__ __forceinline__ PackIn pack_load(const PackArgs& pk, int b) {
     PackIn in{-1, -1, 0, 0, 0, 0, 0, 0, 0};
-    if (!pk.enabled && pk.prune != kPruneCheck) return in;
+    if (!pk.enabled && pk.prune == kPruneNone) return in;
     in.slot0 = pk.bead_slot[static_cast<size_t>(b) * pk.maxm];
     in.slot1 = pk.maxm > 1 ? pk.bead_slot[static_cast<size_t>(b) * pk.maxm + 1] : -1;
     if (pk.enabled && pk.check) {
@@ -119,8 +120,28 @@ __device__ __forceinline__ void put_xyz(PairRec* rec, int slot, double x, double
     p[0] = x; p[1] = y; p[2] = z;
 }
 
-__device__ inline void pack_bead(const PackArgs& pk, const PackIn& in, int b, int sys, double x, double y,
-                                 double z, int* err) {
+// Position wrapped into [0, L) on every axis, rounded to FP32: the prune's
+// copy for periodic groups (one 16-byte gather instead of 24 bytes in two).
+__device__ __forceinline__ float4 wrap32(double x, double y, double z, const double* box, const double* ib) {
+    return make_float4(static_cast<float>(x - box[0] * floor(x * ib[0])),
+                       static_cast<float>(y - box[1] * floor(y * ib[1])),
+                       static_cast<float>(z - box[2] * floor(z * ib[2])), 0.f);
+}
+
+__device__ inline void pack_bead(const PackArgs& pk, const PackIn& in, int b, int sys, const DevSys& S, double x,
+                                 double y, double z, int* err) {
+    // the prune reads FP32 copies of periodic systems' positions
+    const bool w32 = pk.prune == kPruneWrite && S.periodic && in.slot0 >= 0;
+    const float4 q = w32 ? wrap32(x, y, z, S.box, S.inv_box) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (w32) {
+        pk.rec32[in.slot0] = q;
+        if (in.slot1 >= 0) pk.rec32[in.slot1] = q;
+        for (int w = 2; in.slot1 >= 0 && w < pk.maxm; ++w) {
+            const int slot = pk.bead_slot[static_cast<size_t>(b) * pk.maxm + w];
+            if (slot < 0) break;
+            pk.rec32[slot] = q;
+        }
+    }
     if (pk.enabled && in.slot0 >= 0) {
         put_xyz(pk.rec, in.slot0, x, y, z);
         if (in.slot1 >= 0) {
@@ -190,7 +211,7 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
         if (pk.prune == kPruneWrite) {
             pk.prx[b] = px[b]; pk.pry[b] = py[b]; pk.prz[b] = pz[b];
         }
-        pack_bead(pk, pin_, b, s, px[b], py[b], pz[b], err);
+        pack_bead(pk, pin_, b, s, S, px[b], py[b], pz[b], err);
         return;
     }
     double v0 = vx[b], v1 = vy[b], v2 = vz[b];
@@ -225,7 +246,7 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
         const double dx = x0 - pin_.qx, dy = x1 - pin_.qy, dz = x2 - pin_.qz;
         if (!(dist2(dx, dy, dz) <= pk.prune_hm2)) atomicOr(&err[s], kErrPrune);
     }
-    pack_bead(pk, pin_, b, s, x0, x1, x2, err);
+    pack_bead(pk, pin_, b, s, S, x0, x1, x2, err);
 }
 
 // Closing half kick; also resets both global-step slots to the value reached.
@@ -256,7 +277,7 @@ __global__ void k_pack(int nm, const DevPairGroup* __restrict__ groups,
                        const double* __restrict__ px, const double* __restrict__ py,
                        const double* __restrict__ pz, const double* __restrict__ refx,
                        const double* __restrict__ refy, const double* __restrict__ refz,
-                       PairRec* __restrict__ rec, int* __restrict__ err) {
+                       PairRec* __restrict__ rec, float4* __restrict__ rec32, int* __restrict__ err) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nm) return;
     const int b = member_bead[m];
@@ -267,6 +288,7 @@ __global__ void k_pack(int nm, const DevPairGroup* __restrict__ groups,
     r.bead = b;
     rec[slot] = r;
     const DevPairGroup& G = groups[member_group[m]];
+    if (rec32 && G.periodic) rec32[slot] = wrap32(r.x, r.y, r.z, G.box, G.inv_box);
     const double dx = r.x - refx[b], dy = r.y - refy[b], dz = r.z - refz[b];
     if (!(dist2(dx, dy, dz) <= G.half_skin2)) atomicOr(&err[G.sys], kErrStale);
 }
@@ -312,13 +334,14 @@ __global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
     }
 }
 
-__global__ void k_place(int nm, const int* __restrict__ member_bead, const int* __restrict__ member_type,
+__global__ void k_place(int nm, const DevPairGroup* __restrict__ groups, const int* __restrict__ member_group,
+                        const int* __restrict__ member_bead, const int* __restrict__ member_type,
                         const int* __restrict__ member_cell, const int* __restrict__ member_rank,
                         const int* __restrict__ cell_start, const double* __restrict__ px,
                         const double* __restrict__ py, const double* __restrict__ pz,
                         int* __restrict__ member_slot, int* __restrict__ slot_member,
                         const int* __restrict__ member_kpos, int* __restrict__ bead_slot,
-                        PairRec* __restrict__ rec, double* __restrict__ refx,
+                        PairRec* __restrict__ rec, float4* __restrict__ rec32, double* __restrict__ refx,
                         double* __restrict__ refy, double* __restrict__ refz,
                         int* __restrict__ cell_count, int ncell) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,6 +359,10 @@ __global__ void k_place(int nm, const int* __restrict__ member_bead, const int* 
     r.bead = b;
     rec[slot] = r;
     refx[b] = r.x; refy[b] = r.y; refz[b] = r.z;   // every member of b writes the same values
+    if (rec32) {
+        const DevPairGroup& G = groups[member_group[m]];
+        if (G.periodic) rec32[slot] = wrap32(r.x, r.y, r.z, G.box, G.inv_box);
+    }
 }
 
 __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, int mj) {
@@ -983,6 +1010,9 @@ __device__ inline RowType row_type(const float* __restrict__ t) {
 // Hatch) they are evaluated together: one set of sigma powers, one
 // (24 s^6 - 48 s^12)/r^2 factor scaled by the repulsive epsilon inside r_min
 // and by the attractive one outside, the switch on the attractive part.
+// The FP32 type rows hold half of each LJ epsilon and sigma, so the
+// Lorentz-Berthelot means are single adds (halving is exact, so the sums are
+// the same bits as 0.5 * (a + b)).
 // `kinds`, `shared` and `lsw` (the attractive switch is on) are compile-time
 // constants in the specialized row loops, which then hold no branches; the
 // switch polynomial is evaluated at x = max(0, (r - rsw)/w), where x = 0 gives
@@ -1009,13 +1039,13 @@ __device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, co
     const float c13 = 1.2599210498948731648f;  // 2^(1/3): (2^(1/6) sigma)^2 / sigma^2
     if ((kinds & (kLjr | kLja)) == (kLjr | kLja) && shared) {
         const float4 lj = *reinterpret_cast<const float4*>(tj + kTyLjrEps);   // er, sr, ea, sa
-        const float sb = 0.5f * (ti.sr + lj.y);
+        const float sb = ti.sr + lj.y;
         const float sb2 = sb * sb;
         const float s2 = sb2 * ir2;
         const float s6 = s2 * s2 * s2, s12 = s6 * s6;
         const float th2 = c13 * sb2;
         const float X = (24.f * s6 - 48.f * s12) * ir2;
-        const float er = 0.5f * (ti.er + lj.x), ea = 0.5f * (ti.ea + lj.z);
+        const float er = ti.er + lj.x, ea = ti.ea + lj.z;
         const bool inner = th2 - r2 >= 0.f, outer = r2 - th2 >= 0.f;
         f += inner ? er * X : 0.f;
         float u = outer ? 4.f * ea * (s12 - s6) : 0.f;
@@ -1033,8 +1063,8 @@ __device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, co
     }
     float s6 = 0.f, s12 = 0.f, sb_r = -1.f;
     if (kinds & kLjr) {
-        const float eb = 0.5f * (ti.er + tj[kTyLjrEps]);
-        const float sb = 0.5f * (ti.sr + tj[kTyLjrSig]);
+        const float eb = ti.er + tj[kTyLjrEps];
+        const float sb = ti.sr + tj[kTyLjrSig];
         const float s2 = sb * sb * ir2;
         s6 = s2 * s2 * s2;
         s12 = s6 * s6;
@@ -1042,8 +1072,8 @@ __device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, co
         if (c13 * sb * sb - r2 >= 0.f) f += eb * (-48.f * s12 + 24.f * s6) * ir2;
     }
     if (kinds & kLja) {
-        const float eb = 0.5f * (ti.ea + tj[kTyLjaEps]);
-        const float sb = 0.5f * (ti.sa + tj[kTyLjaSig]);
+        const float eb = ti.ea + tj[kTyLjaEps];
+        const float sb = ti.sa + tj[kTyLjaSig];
         if (sb != sb_r) {
             const float s2 = sb * sb * ir2;
             s6 = s2 * s2 * s2;
@@ -1096,6 +1126,28 @@ struct BondedArgs {
 // has no branches and the four entries of a round interleave.
 constexpr int kSpecGeneric = 0, kSpecExvPeriodic = 1, kSpecExvOpen = 2, kSpecDh = 3, kSpecLjShared = 4;
 
+// Raw buffer over the pair records: 32-bit offsets instead of 64-bit
+// addresses, and a read past the end (the -1 padding entries) returns zeros
+// without touching memory (gfx9 buffer resource word 3 = 0x00020000, stride 0,
+// range in bytes).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const PairRec* rec, int nrec) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<PairRec*>(rec), 0, nrec * static_cast<int>(sizeof(PairRec)),
+                                             0x00020000);
+}
+
+__device__ __forceinline__ PairRec load_rec(__amdgpu_buffer_rsrc_t rs, int slot) {
+    const unsigned off = static_cast<unsigned>(slot) * static_cast<unsigned>(sizeof(PairRec));   // -1: past the end
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>(off), 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>(off + 16u), 0, 0);
+    PairRec r;
+    r.x = __hiloint2double(static_cast<int>(a[1]), static_cast<int>(a[0]));
+    r.y = __hiloint2double(static_cast<int>(a[3]), static_cast<int>(a[2]));
+    r.z = __hiloint2double(static_cast<int>(b[1]), static_cast<int>(b[0]));
+    r.type = static_cast<int>(b[2]);
+    r.bead = static_cast<int>(b[3]);
+    return r;
+}
+
 struct RowGeom {
     double bx, by, bz, ibx, iby, ibz, rc2, rin2;
     bool periodic;
@@ -1110,8 +1162,8 @@ struct RowAcc {
 // in fixed point (see kFixMagic).
 template <int TEAM, int SPEC>
 __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const PairK<float>& Kf, const RowType& rt,
-                                          const PairRec& ri, int row, int beg, int cnt, int lane,
-                                          const PairRec* __restrict__ rec, const int* __restrict__ col,
+                                          const PairRec& ri, int beg, int cnt, int lane,
+                                          const PairRec* __restrict__ rec, int nrec, const int* __restrict__ col,
                                           const float* __restrict__ lds_types, RowAcc& acc) {
     // A round covers 4*TEAM entries of the row; lane l takes entries l, l+T,
     // l+2T, l+3T of it, so the lanes of one gather read neighbouring entries,
@@ -1122,6 +1174,7 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
     // them), and the next round's column indices are fetched while this
     // round computes.
     const int* crow = col + beg;
+    const __amdgpu_buffer_rsrc_t rs = rec_rsrc(rec, nrec);
     int cn[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
@@ -1129,7 +1182,7 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
         const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
         PairRec rjs[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) rjs[u] = rec[cs[u] >= 0 ? cs[u] : row];
+        for (int u = 0; u < 4; ++u) rjs[u] = load_rec(rs, cs[u]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int e = base + 4 * TEAM + lane + u * TEAM;
@@ -1177,78 +1230,106 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
     }
 }
 
-// Inner-list prune: each member row's outer entries within cutoff + margin,
-// written in order to the same row slots of the inner list (col_w) and padded
-// to a multiple of 4; a team of TEAM lanes per row, as in the force pass.
+// Inner-list prune of one member row for lanes `lane` of a team of TEAM: its
+// outer entries within cutoff + margin, written in list order to the same row
+// slots of the inner list (col_w) and padded to a multiple of 4.  Lane l takes
+// entries l, l+T, l+2T, l+3T of each round of 4T (as in the force pass).
+// F32 (periodic groups): box-wrapped FP32 positions, one 16-byte gather per
+// entry, kept while r^2 < rin2_wide, which exceeds (cutoff + margin)^2 by more
+// than the FP32 coordinates' error, so the inner row still holds every entry
+// an exact FP64 test would keep.  Otherwise the FP64 records and rin2.
+template <int TEAM, bool F32>
+__device__ __forceinline__ void prune_row(const DevPairGroup& G, int row, int lane, unsigned long long team_lanes,
+                                          unsigned long long below, const PairRec* __restrict__ rec,
+                                          const float4* __restrict__ rec32, const int* __restrict__ row_start,
+                                          const int* __restrict__ row_count, const int* __restrict__ col,
+                                          int* __restrict__ row_count_w, int* __restrict__ col_w) {
+    const int beg = row_start[row];
+    const int cnt = row_count[row];
+    double xi = 0, yi = 0, zi = 0;
+    float4 qi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (F32) qi = rec32[row];
+    else { xi = rec[row].x; yi = rec[row].y; zi = rec[row].z; }
+    const float fbx = static_cast<float>(G.box[0]), fby = static_cast<float>(G.box[1]);
+    const float fbz = static_cast<float>(G.box[2]);
+    const float fibx = static_cast<float>(G.inv_box[0]), fiby = static_cast<float>(G.inv_box[1]);
+    const float fibz = static_cast<float>(G.inv_box[2]);
+    const float rin2f = static_cast<float>(G.rin2_wide);
+    // every lane of a team runs the same rounds (the loop bound is the
+    // row's), so the ballots below see the whole team
+    int n = 0;   // entries the team has written to the inner row
+    const int* crow = col + beg;
+    int cn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
+    for (int base = 0; base < cnt; base += 4 * TEAM) {
+        const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
+        double px[4], py[4], pz[4];
+        float4 qj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = cs[u] >= 0 ? cs[u] : row;
+            if (F32) qj[u] = rec32[j];
+            else { px[u] = rec[j].x; py[u] = rec[j].y; pz[u] = rec[j].z; }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = base + 4 * TEAM + lane + u * TEAM;
+            cn[u] = e < cnt ? crow[e] : -1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            bool keep;
+            if (F32) {
+                float dx = qj[u].x - qi.x, dy = qj[u].y - qi.y, dz = qj[u].z - qi.z;
+                dx = fmaf(-fbx, rintf(dx * fibx), dx);
+                dy = fmaf(-fby, rintf(dy * fiby), dy);
+                dz = fmaf(-fbz, rintf(dz * fibz), dz);
+                keep = cs[u] >= 0 && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rin2f;
+            } else {
+                const double dx = min_image(px[u] - xi, G.box[0], G.inv_box[0]);
+                const double dy = min_image(py[u] - yi, G.box[1], G.inv_box[1]);
+                const double dz = min_image(pz[u] - zi, G.box[2], G.inv_box[2]);
+                keep = cs[u] >= 0 && dist2(dx, dy, dz) < G.rin2;
+            }
+            // entry base + u*T + l precedes base + u*T + l' for l < l'
+            const unsigned long long m = __ballot(keep) & team_lanes;
+            if (keep) col_w[beg + n + __popcll(m & below)] = cs[u];
+            n += __popcll(m);
+        }
+    }
+    if (lane == 0) {
+        int p = n;
+        for (; p & 3; ++p) col_w[beg + p] = -1;
+        row_count_w[row] = p;
+    }
+}
+
+// Inner-list prune: a team of TEAM lanes per member row, rows taken in order
+// (row = global member index), independent of the force pass's block table
+// so its team can differ.
 template <int TEAM>
-__global__ __launch_bounds__(kPairBlock) void k_prune(const DevPairGroup* __restrict__ groups,
-                                                      const int2* __restrict__ blocks,
+__global__ __launch_bounds__(kPairBlock) void k_prune(int nm, const DevPairGroup* __restrict__ groups,
+                                                      const int* __restrict__ member_group,
                                                       const PairRec* __restrict__ rec,
+                                                      const float4* __restrict__ rec32,
                                                       const int* __restrict__ row_start,
                                                       const int* __restrict__ row_count,
                                                       const int* __restrict__ col, int* __restrict__ row_count_w,
-                                                      int* __restrict__ col_w, int nblocks) {
+                                                      int* __restrict__ col_w) {
     const int wl = static_cast<int>(threadIdx.x & 63u);
     // the team's lanes within the wave, and those below this lane
     const unsigned long long team_lanes = TEAM == 64 ? ~0ull : (((1ull << TEAM) - 1ull) << (wl / TEAM * TEAM));
     const unsigned long long below = (1ull << wl) - 1ull;
     const int lane = threadIdx.x % TEAM;
-    for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
-        const int2 bt = blocks[bi];
-        if (bt.x < 0) continue;   // bonded chunk
-        const DevPairGroup& G = groups[bt.x];
-        const double bx = G.box[0], by = G.box[1], bz = G.box[2];
-        const double ibx = G.inv_box[0], iby = G.inv_box[1], ibz = G.inv_box[2];
-        const double rin2 = G.rin2;
-        const int row = bt.y + threadIdx.x / TEAM;
-        const bool live = row < G.member_begin + G.member_count;
-        // every lane of a team runs the same rounds (the loop bound is the
-        // row's), so the ballots below see the whole team
-        int n = 0;   // entries the team has written to the inner row
-        int beg = 0;
-        if (live) {
-            const double xi = rec[row].x, yi = rec[row].y, zi = rec[row].z;
-            beg = row_start[row];
-            const int cnt = row_count[row];
-            // lane l takes entries l, l+T, l+2T, l+3T of each round of 4T (as
-            // in the force pass); kept entries are written in list order
-            const int* crow = col + beg;
-            int cn[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
-            for (int base = 0; base < cnt; base += 4 * TEAM) {
-                const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
-                double px[4], py[4], pz[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const PairRec* r = rec + (cs[u] >= 0 ? cs[u] : row);
-                    px[u] = r->x; py[u] = r->y; pz[u] = r->z;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = base + 4 * TEAM + lane + u * TEAM;
-                    cn[u] = e < cnt ? crow[e] : -1;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double dx = min_image(px[u] - xi, bx, ibx);
-                    const double dy = min_image(py[u] - yi, by, iby);
-                    const double dz = min_image(pz[u] - zi, bz, ibz);
-                    const bool keep = cs[u] >= 0 && dist2(dx, dy, dz) < rin2;
-                    // entry base + u*T + l precedes base + u*T + l' for l < l'
-                    const unsigned long long m = __ballot(keep) & team_lanes;
-                    if (keep) col_w[beg + n + __popcll(m & below)] = cs[u];
-                    n += __popcll(m);
-                }
-            }
-        }
-        if (lane == 0 && live) {
-            int p = n;
-            for (; p & 3; ++p) col_w[beg + p] = -1;
-            row_count_w[row] = p;
-        }
-    }
+    const int row = static_cast<int>((static_cast<long long>(blockIdx.x) * kPairBlock + threadIdx.x) / TEAM);
+    if (row >= nm) return;   // whole teams leave together
+    const DevPairGroup& G = groups[member_group[row]];
+    if (G.periodic) prune_row<TEAM, true>(G, row, lane, team_lanes, below, rec, rec32, row_start, row_count, col,
+                                          row_count_w, col_w);
+    else prune_row<TEAM, false>(G, row, lane, team_lanes, below, rec, rec32, row_start, row_count, col,
+                                row_count_w, col_w);
 }
 
 // At least 5 waves per SIMD (<= 96 VGPRs): the record gathers need the latency hiding.
@@ -1258,7 +1339,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
     const PairRec* __restrict__ rec, const int* __restrict__ row_start,
     const int* __restrict__ row_count, const int* __restrict__ col,
     const float* __restrict__ types_f, const double* __restrict__ types_d, ForceBuf out,
-    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks,
+    long long* __restrict__ energy, uint32_t gmask, int* __restrict__ err, int nblocks, int nrec,
     unsigned long long* __restrict__ stamp, BondedArgs bd) {
     extern __shared__ __attribute__((aligned(16))) float lds_types[];
     int loaded = -1;
@@ -1322,23 +1403,23 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
             else if (kinds == (kLjr | kLja) && Kf.lj_shared && Kf.lja_switch && geo.periodic) spec = kSpecLjShared;
             switch (spec) {
             case kSpecExvPeriodic:
-                force_row<TEAM, kSpecExvPeriodic>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                force_row<TEAM, kSpecExvPeriodic>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                          lds_types, acc);
                 break;
             case kSpecExvOpen:
-                force_row<TEAM, kSpecExvOpen>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                force_row<TEAM, kSpecExvOpen>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                      lds_types, acc);
                 break;
             case kSpecDh:
-                force_row<TEAM, kSpecDh>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                force_row<TEAM, kSpecDh>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                 lds_types, acc);
                 break;
             case kSpecLjShared:
-                force_row<TEAM, kSpecLjShared>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                force_row<TEAM, kSpecLjShared>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                       lds_types, acc);
                 break;
             default:
-                force_row<TEAM, kSpecGeneric>(geo, kinds, Kf, rt, ri, row, beg, cnt, lane, rec, col,
+                force_row<TEAM, kSpecGeneric>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                      lds_types, acc);
                 break;
             }
@@ -1522,7 +1603,7 @@ void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st) {
 void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check,
                       int prune) {
     PackArgs pk{pack && d.nm > 0, check, d.maxm, d.bead_slot, d.bead_hs2, d.refx, d.refy, d.refz, d.rec,
-                d.prx, d.pry, d.prz, d.prune_hm2, d.nm > 0 && d.col_in ? prune : kPruneNone};
+                d.rec32, d.prx, d.pry, d.prz, d.prune_hm2, d.nm > 0 && d.col_in ? prune : kPruneNone};
     k_integrate<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
                                                  d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
                                                  pin, pout, pending, d.err, pk, d.sys0, d.nsys == 1 ? 1 : 0);
@@ -1537,7 +1618,7 @@ void launch_pack(const DevPtrs& d, hipStream_t st) {
     if (d.nm == 0) return;
     k_pack<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.groups, d.member_group, d.member_bead, d.member_type,
                                             d.member_slot, d.px, d.py, d.pz, d.refx, d.refy, d.refz, d.rec,
-                                            d.err);
+                                            d.rec32, d.err);
 }
 
 static void launch_scan(const DevPtrs& d, hipStream_t st, const int* in, int* out, int n, int* total) {
@@ -1596,10 +1677,11 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
         k_scan_cells_small<<<1, kSmallScanThreads, 0, st>>>(d.ncell, d.cell_count, d.cell_start);
     else
         launch_scan(d, st, d.cell_count, d.cell_start, d.ncell, d.cell_start + d.ncell);
-    k_place<<<nblk(std::max(d.nm, 1), 256), 256, 0, st>>>(d.nm, d.member_bead, d.member_type, d.member_cell,
-                                                          d.member_rank, d.cell_start, d.px, d.py, d.pz,
-                                                          d.member_slot, d.slot_member, d.member_kpos, d.bead_slot,
-                                                          d.rec, d.refx, d.refy, d.refz, d.cell_count, d.ncell);
+    k_place<<<nblk(std::max(d.nm, 1), 256), 256, 0, st>>>(d.nm, d.groups, d.member_group, d.member_bead,
+                                                          d.member_type, d.member_cell, d.member_rank, d.cell_start,
+                                                          d.px, d.py, d.pz, d.member_slot, d.slot_member,
+                                                          d.member_kpos, d.bead_slot, d.rec, d.rec32, d.refx, d.refy,
+                                                          d.refz, d.cell_count, d.ncell);
     // work items appended with atomics: the order interleaves the groups, which
     // measured faster than cell order
     k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_start, d.tile_items, d.tile_nitems);
@@ -1628,7 +1710,7 @@ static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, u
 #define OCM_PAIR(T)                                                                                     \
     k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col,     \
                                            d.types_f, d.types_d, f, d.energy, groups, d.err,            \
-                                           d.n_pair_blocks, stamp, bd)
+                                           d.n_pair_blocks, d.nm, stamp, bd)
     switch (d.team) {
     case 4: OCM_PAIR(4); break;
     case 8: OCM_PAIR(8); break;
@@ -1640,12 +1722,13 @@ static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, u
 }
 
 void launch_prune(const DevPtrs& d, hipStream_t st) {
-    if (d.n_pair_blocks == 0 || !d.col_in) return;
-    const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
-#define OCM_PRUNE(T)                                                                                      \
-    k_prune<T><<<grid, block, 0, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, d.row_count, d.col,   \
-                                       d.row_count_in, d.col_in, d.n_pair_blocks)
-    switch (d.team) {
+    if (d.nm == 0 || !d.col_in) return;
+    const dim3 block(kPairBlock);
+#define OCM_PRUNE(T)                                                                                           \
+    k_prune<T><<<dim3(static_cast<unsigned>((static_cast<long long>(d.nm) * T + kPairBlock - 1) / kPairBlock)), \
+                 block, 0, st>>>(d.nm, d.groups, d.member_group, d.rec, d.rec32, d.row_start, d.row_count,       \
+                                 d.col, d.row_count_in, d.col_in)
+    switch (d.prune_team) {
     case 4: OCM_PRUNE(4); break;
     case 8: OCM_PRUNE(8); break;
     case 32: OCM_PRUNE(32); break;

@@ -47,6 +47,7 @@ struct DevPtrs {
     int *excl_start = nullptr, *excl_count = nullptr, *excl = nullptr;
     double *refx = nullptr, *refy = nullptr, *refz = nullptr;   // positions at the last build, by bead
     PairRec* rec = nullptr;
+    float4* rec32 = nullptr;       // periodic groups: box-wrapped FP32 positions by slot (the prune's input)
     int* slot_member = nullptr;
     int *cell_count = nullptr, *cell_start = nullptr, *cell_group = nullptr;
     int *row_count = nullptr, *row_start = nullptr, *col = nullptr;
@@ -68,6 +69,7 @@ struct DevPtrs {
     int2* pair_blocks = nullptr;   // (group, first slot) per pair-kernel block
     int n_pair_blocks = 0;
     int team = 16;
+    int prune_team = 32;           // lanes per row of the inner-list prune
     long long* energy = nullptr;   // nsys * 32 fixed point
     long long* ke = nullptr;       // nsys fixed point
     int* dump_count = nullptr;
