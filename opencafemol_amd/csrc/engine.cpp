@@ -514,6 +514,10 @@ void Engine::pack() {
     if (member_bead.size() >= (1u << 26))   // record offsets are 32-bit byte offsets
         throw Error(OCM_ERR_OOM, "more than 2^26 pair-group members");
     d_.nm = static_cast<int>(member_bead.size());
+    // large lists: 64-row work items stage each stencil once for a whole cell
+    // (there are still far more items than workgroups); small ones keep 16-row
+    // items so the build spreads over the device
+    d_.tile_chunk = d_.nm > (1 << 17) ? 64 : 16;
     d_.ncell = std::max(ncell_total, 1);
     d_.ngroups = static_cast<int>(groups.size());
     d_.max_types = std::max(max_types, 1);

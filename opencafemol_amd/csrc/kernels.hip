@@ -386,16 +386,16 @@ __device__ inline bool excluded(const int* __restrict__ excl, int beg, int cnt, 
 constexpr int kTileCap = 1024;
 constexpr int kMaxStencil = 27;
 
-constexpr int kTileChunk = 16;  // rows per work item
+constexpr int kTileChunkMax = 64;  // rows per work item: 16, or 64 for large lists (staging amortised)
 
 // Work items of the tile build: one per (non-empty cell, 16-row chunk),
 // appended by k_tile_items after the cell scan; the tile kernels run on a
 // fixed grid and loop over them.
 __global__ void k_tile_items(int ncell, const int* __restrict__ cell_start, int2* __restrict__ items,
-                             int* __restrict__ n_items) {
+                             int* __restrict__ n_items, int chunk) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    const int n = (cell_start[c + 1] - cell_start[c] + kTileChunk - 1) / kTileChunk;
+    const int n = (cell_start[c + 1] - cell_start[c] + chunk - 1) / chunk;
     if (n == 0) return;
     const int at = atomicAdd(n_items, n);
     for (int q = 0; q < n; ++q) items[at + q] = make_int2(c, q);
@@ -417,16 +417,18 @@ constexpr float kListMargin = 1.001f;
 
 __device__ __forceinline__ double wrap_coord(double x, double L, double invL) { return x - L * floor(x * invL); }
 
+constexpr int kRowBlock = 4;   // rows a wave tests against each staged candidate
+
 __global__ __launch_bounds__(256) void k_nl_tiles(
     const int2* __restrict__ items, const int* __restrict__ n_items, const DevPairGroup* __restrict__ groups,
     const int* __restrict__ cell_group,
     const int* __restrict__ cell_start, const PairRec* __restrict__ rec,
     const int* __restrict__ slot_member, const int* __restrict__ excl_start, const int* __restrict__ excl_count,
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
-    int* __restrict__ group_need, int* __restrict__ col, int* __restrict__ flags) {
+    int* __restrict__ group_need, int* __restrict__ col, int* __restrict__ flags, int chunk) {
     __shared__ float4 cpos[kTileCap];      // x, y, z relative to the item's first row; w = slot (bits)
     __shared__ int2 cinfo[kTileCap];       // {type word, bead}
-    __shared__ int row_run[kTileChunk];
+    __shared__ int row_run[kTileChunkMax];
     __shared__ int st_start[kMaxStencil], st_pre[kMaxStencil + 1];
     __shared__ float st_shift[kMaxStencil][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
@@ -434,8 +436,8 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     for (int item = blockIdx.x; item < nit; item += gridDim.x) {
     __syncthreads();  // LDS of the previous item is no longer read
     const int home = items[item].x;
-    const int h0 = cell_start[home] + items[item].y * kTileChunk;
-    const int nr = min(kTileChunk, cell_start[home + 1] - h0);
+    const int h0 = cell_start[home] + items[item].y * chunk;
+    const int nr = min(chunk, cell_start[home + 1] - h0);
     const int gid = cell_group[home];
     const DevPairGroup& G = groups[gid];
     const int cap = G.stride;
@@ -507,60 +509,98 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
             cinfo[k] = make_int2(r.type, r.bead);
         }
         __syncthreads();
-        for (int r = wave; r < nr; r += nwave) {
-            const int i = h0 + r;
-            const PairRec ri = rec[i];
-            const float xi = static_cast<float>((periodic ? wrap_coord(ri.x, L[0], iL[0]) : ri.x) - o0);
-            const float yi = static_cast<float>((periodic ? wrap_coord(ri.y, L[1], iL[1]) : ri.y) - o1);
-            const float zi = static_cast<float>((periodic ? wrap_coord(ri.z, L[2], iL[2]) : ri.z) - o2);
-            const uint32_t amask = G.allow[ri.type >> 16];
-            const int m = slot_member[i];
-            const int eb = excl_start[m], ec = excl_count[m];
-            // the row's excluded partners (bead ids), one per lane when <= 64
-            const int mine = lane < ec ? excl[eb + lane] : -1;
-            const bool few = ec <= 64;
-            const int base = row_start[i];
-            int run = row_run[r];
+        // a wave takes kRowBlock rows at once: each staged candidate is read
+        // from LDS once and tested against all of them
+        for (int r0 = wave * kRowBlock; r0 < nr; r0 += nwave * kRowBlock) {
+            float xi[kRowBlock], yi[kRowBlock], zi[kRowBlock];
+            uint32_t amask[kRowBlock];
+            int base[kRowBlock], run[kRowBlock], self[kRowBlock];
+#pragma unroll
+            for (int q = 0; q < kRowBlock; ++q) {
+                const int r = r0 + q;
+                if (r < nr) {
+                    const int i = h0 + r;
+                    const PairRec ri = rec[i];
+                    xi[q] = static_cast<float>((periodic ? wrap_coord(ri.x, L[0], iL[0]) : ri.x) - o0);
+                    yi[q] = static_cast<float>((periodic ? wrap_coord(ri.y, L[1], iL[1]) : ri.y) - o1);
+                    zi[q] = static_cast<float>((periodic ? wrap_coord(ri.z, L[2], iL[2]) : ri.z) - o2);
+                    amask[q] = G.allow[ri.type >> 16];
+                    base[q] = row_start[i];
+                    run[q] = row_run[r];
+                    self[q] = i;
+                } else {
+                    xi[q] = yi[q] = zi[q] = 0.f;
+                    amask[q] = 0u;   // no candidate passes
+                    base[q] = run[q] = 0;
+                    self[q] = -1;
+                }
+            }
             for (int kb = 0; kb < nw; kb += 64) {
                 const int k = kb + lane;
-                bool pass = false;
+                float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+                int2 inf = make_int2(0, -1);
                 int sl = -1;
                 if (k < nw) {
-                    const float4 c = cpos[k];
-                    const int2 inf = cinfo[k];
+                    c = cpos[k];
+                    inf = cinfo[k];
                     sl = __float_as_int(c.w);
-                    float dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
-                    if (any_mi) {
-                        if (mi0) dx -= Lf0 * rintf(dx * iLf0);
-                        if (mi1) dy -= Lf1 * rintf(dy * iLf1);
-                        if (mi2) dz -= Lf2 * rintf(dz * iLf2);
-                    }
-                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    pass = sl != i && ((amask >> (inf.x >> 16)) & 1u) && d2 < rl2f;
-                    if (pass && ec) {
-                        if (few) {
-                            bool hit = false;
-                            for (int e = 0; e < ec; ++e) hit |= __builtin_amdgcn_readlane(mine, e) == inf.y;
-                            pass = !hit;
-                        } else {
-                            pass = !excluded(excl, eb, ec, inf.y);
-                        }
-                    }
                 }
-                const unsigned long long mask = __ballot(pass);
-                const int pos = run + __popcll(mask & below);
-                if (pass && pos < cap) col[base + pos] = sl;
-                run += __popcll(mask);
+#pragma unroll
+                for (int q = 0; q < kRowBlock; ++q) {
+                    bool pass = false;
+                    if (k < nw) {
+                        float dx = c.x - xi[q], dy = c.y - yi[q], dz = c.z - zi[q];
+                        if (any_mi) {
+                            if (mi0) dx -= Lf0 * rintf(dx * iLf0);
+                            if (mi1) dy -= Lf1 * rintf(dy * iLf1);
+                            if (mi2) dz -= Lf2 * rintf(dz * iLf2);
+                        }
+                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        pass = sl != self[q] && ((amask[q] >> (inf.x >> 16)) & 1u) && d2 < rl2f;
+                    }
+                    const unsigned long long mask = __ballot(pass);
+                    const int pos = run[q] + __popcll(mask & below);
+                    if (pass && pos < cap) col[base[q] + pos] = sl;
+                    run[q] += __popcll(mask);
+                }
             }
-            if (lane == 0) row_run[r] = run;
-            // last window: pad the row to a multiple of 4 entries with -1
-            if (wbeg + kTileCap >= ntot) {
-                const int padded = (run + 3) & ~3;
-                if (padded <= cap && lane < padded - run) col[base + run + lane] = -1;
+#pragma unroll
+            for (int q = 0; q < kRowBlock; ++q) {
+                const int r = r0 + q;
+                if (r >= nr) continue;
+                if (lane == 0) row_run[r] = run[q];
+                // last window: pad the row to a multiple of 4 entries with -1
+                if (wbeg + kTileCap >= ntot) {
+                    const int padded = (run[q] + 3) & ~3;
+                    if (padded <= cap && lane < padded - run[q]) col[base[q] + run[q] + lane] = -1;
+                }
             }
         }
     }
     __syncthreads();
+    // excluded partners: the written entries of each row whose bead is in the
+    // row's exclusion list become -1, which every reader skips (a few per row,
+    // against a test per candidate in the loop above)
+    for (int r = wave; r < nr; r += nwave) {
+        const int i = h0 + r;
+        const int m = slot_member[i];
+        const int eb = excl_start[m], ec = excl_count[m];
+        const int n = min(row_run[r], cap);
+        if (ec == 0 || n == 0) continue;
+        const int base = row_start[i];
+        const int mine = lane < ec ? excl[eb + lane] : -1;   // one partner per lane when <= 64
+        for (int e = lane; e < n; e += 64) {
+            const int sl = col[base + e];
+            const int bj = rec[sl].bead;
+            bool hit = false;
+            if (ec <= 64) {
+                for (int k = 0; k < ec; ++k) hit |= __builtin_amdgcn_readlane(mine, k) == bj;
+            } else {
+                hit = excluded(excl, eb, ec, bj);
+            }
+            if (hit) col[base + e] = -1;
+        }
+    }
     for (int r = tid; r < nr; r += blockDim.x) {
         const int padded = (row_run[r] + 3) & ~3;
         if (padded <= cap) {
@@ -1684,10 +1724,12 @@ void launch_rebuild(const DevPtrs& d, hipStream_t st) {
                                                           d.refz, d.cell_count, d.ncell);
     // work items appended with atomics: the order interleaves the groups, which
     // measured faster than cell order
-    k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_start, d.tile_items, d.tile_nitems);
+    k_tile_items<<<nblk(d.ncell, 256), 256, 0, st>>>(d.ncell, d.cell_start, d.tile_items, d.tile_nitems,
+                                                     d.tile_chunk);
     k_nl_tiles<<<d.tile_grid, 256, 0, st>>>(d.tile_items, d.tile_nitems, d.groups, d.cell_group, d.cell_start,
                                             d.rec, d.slot_member, d.excl_start, d.excl_count, d.excl,
-                                            d.row_start, d.row_count, d.group_need, d.col, d.flags);
+                                            d.row_start, d.row_count, d.group_need, d.col, d.flags,
+                                            d.tile_chunk);
 }
 
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups) {

@@ -60,6 +60,7 @@ struct DevPtrs {
     int2* tile_items = nullptr;    // (cell, chunk) work items of the list build
     int* tile_nitems = nullptr;
     int tile_grid = 2048;          // workgroups of the list-build kernels
+    int tile_chunk = 16;           // member rows per list-build work item (16 or 64)
     int pair_grid = 2048;          // workgroups of the pair kernel
     int* scan_partial = nullptr;
     int* scan_total = nullptr;     // [0] cells total, [1] list total

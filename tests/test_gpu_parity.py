@@ -186,3 +186,37 @@ def test_numeric_overflow_is_reported(oracle):
         code = c.step(3)
         assert code == abi.OCM_ERR_NUMERIC
         assert c.status(0) == abi.OCM_ERR_NUMERIC
+
+
+def test_large_list_build_pairs():
+    """A list with more than 2^17 rows is built from 64-row work items (each
+    stages a stencil once for a whole cell).  Its pairs within the cutoff
+    must be those a periodic k-d tree finds, less the exclusions; pairs
+    within 1e-12 of the cutoff may fall either way, since the tree's
+    distance arithmetic is not the kernel's."""
+    from scipy.spatial import cKDTree
+    s, it, x = configs.hps_condensate(n_chains=140, chain_len=1000, box=(30.0, 30.0, 300.0), seed=5)
+    box = np.array(s.box)
+    xw = np.mod(x, box)
+    n = np.int64(s.n)
+
+    def keys(pairs):
+        pairs = np.sort(np.asarray(pairs, dtype=np.int64).reshape(-1, 2), axis=1)
+        return np.unique(pairs[:, 0] * n + pairs[:, 1])
+
+    with _ctx(s, it, x) as c:
+        assert c.list_stats()[1] > 2 ** 17
+        for fi, f in enumerate(s.forces):
+            if f.kind not in (abi.LJ_REPULSIVE, abi.DEBYE_HUCKEL):
+                continue
+            got = c.neighbor_pairs(fi)
+            g = keys(got)
+            assert len(g) == len(got)
+            idx = np.arange(s.n) if f.category is None else np.where(f.category >= 0)[0]
+            tree = cKDTree(xw[idx], boxsize=box)
+            ex = keys(f.exclusions) if f.exclusions is not None and len(f.exclusions) else np.zeros(0, np.int64)
+            inner = np.setdiff1d(keys(idx[tree.query_pairs(f.cutoff * (1 - 1e-12), output_type="ndarray")]), ex)
+            outer = np.setdiff1d(keys(idx[tree.query_pairs(f.cutoff * (1 + 1e-12), output_type="ndarray")]), ex)
+            assert np.isin(inner, g).all(), (f.name, int((~np.isin(inner, g)).sum()))
+            assert np.isin(g, outer).all(), (f.name, int((~np.isin(g, outer)).sum()))
+            assert len(inner) > 100_000
