@@ -195,11 +195,14 @@ int ocm_get_state(ocm_context* ctx, int32_t what, uint32_t groups,
                   ocm_state* out);
 
 /* LocalEnergyMinimizer::minimize(context, tolerance, max_iterations),
- * src/Simulator.cpp:42-46.  Capped steepest descent on the device: each
- * iteration moves every mobile bead along its force by min(alpha |F|, 0.01 nm);
- * stops when the RMS force component of every system is <= tolerance
- * (kJ/mol/nm) or after max_iterations (0 = 100000).  Not OpenMM's L-BFGS: the
- * minimum reached is a local one, not the same point.                       */
+ * src/Simulator.cpp:42-46.  L-BFGS (10 correction pairs) over the mobile
+ * beads, with device forces and FP64 device energies, a backtracking line
+ * search, and no bead moving more than 0.05 nm per iteration (so starts with
+ * overlapping beads relax safely).  Stops when the RMS force component of
+ * every system is <= tolerance (kJ/mol/nm), as OpenMM's does, or after
+ * max_iterations (0 = 100000).  OpenMM's implementation differs in its
+ * line search and history, so the point reached is a nearby minimum, not the
+ * same bits.                                                                */
 int ocm_minimize(ocm_context* ctx, double tolerance, int32_t max_iterations);
 
 /* Neighbor pairs (bead index pairs i<j, r < cutoff) that the device list of

@@ -1036,48 +1036,140 @@ int Engine::step(long long n) {
     return status;
 }
 
+// Potential energy of every system at the current positions, by the FP64
+// energy pass over the list the last force evaluation built.
+std::vector<double> Engine::potentials() {
+    launch_zero_energy(d_, stream_);
+    launch_bonded(d_, stream_, true, d_.f2, 0xFFFFFFFFu);
+    launch_pair(d_, stream_, true, d_.f2, 0xFFFFFFFFu);
+    sync();
+    check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
+    std::vector<long long> e(static_cast<size_t>(d_.nsys) * 32);
+    check_hip(hipMemcpy(e.data(), d_.energy, e.size() * sizeof(long long), hipMemcpyDeviceToHost), "energy");
+    std::vector<double> out(d_.nsys, 0.0);
+    for (int s = 0; s < d_.nsys; ++s)
+        for (int g = 0; g < 32; ++g) out[s] += e[static_cast<size_t>(s) * 32 + g] * kFixInv;
+    return out;
+}
+
+// LocalEnergyMinimizer::minimize (src/Simulator.cpp:42-46) is OpenMM's
+// L-BFGS; this is an L-BFGS of its own over the coordinates of all mobile
+// beads of the batch (the systems are independent, so their summed energy
+// has the same minima), with device forces and FP64 device energies:
+//   - direction from the last kHistory (s, y) pairs (two-loop recursion),
+//     steepest descent when it is not a descent direction;
+//   - the step capped so that no bead moves more than kMaxMove per
+//     iteration (starts with overlapping beads have enormous gradients);
+//   - backtracking line search on the sufficient-decrease condition;
+//   - stop when the RMS force component of every system is <= tol (the
+//     reference's criterion) or after max_iter iterations.
 int Engine::minimize(double tol, int max_iter) {
     if (!(tol >= 0.0)) invalid("tolerance must be >= 0");
     if (max_iter < 0) invalid("max_iterations must be >= 0");
     if (max_iter == 0) max_iter = 100000;
-    ensure_forces();
-    const double alpha = 5e-6, dmax = 0.01;  // nm per kJ/mol/nm, nm
-    std::vector<long long> f(3 * static_cast<size_t>(d_.nb));
-    int it = 0;
-    while (true) {
-        sync();
-        check_hip(hipMemcpy(f.data(), d_.f.fx, f.size() * sizeof(long long), hipMemcpyDeviceToHost), "forces");
-        bool done = true;
-        for (int s = 0; s < d_.nsys && done; ++s) {
+    constexpr int kHistory = 10;
+    constexpr double kMaxMove = 0.05;   // nm per bead per iteration
+    const size_t nb = static_cast<size_t>(d_.nb), n3 = 3 * nb;
+    std::vector<char> mobile(nb, 0);
+    for (int s = 0; s < d_.nsys; ++s)
+        for (int i = 0; i < sys_[s].n; ++i) mobile[bead_begin_[s] + i] = sys_[s].masses[i] != 0.0;
+    std::vector<long long> fbuf(n3);
+
+    // energy and gradient (-force; 0 for fixed beads) at x, device left at x
+    auto evaluate = [&](const std::vector<double>& x, std::vector<double>& g) {
+        check_hip(hipMemcpy(d_.px, x.data(), n3 * sizeof(double), hipMemcpyHostToDevice), "positions");
+        forces_dirty_ = true;
+        ensure_forces();
+        check_hip(hipMemcpy(fbuf.data(), d_.f.fx, n3 * sizeof(long long), hipMemcpyDeviceToHost), "forces");
+        const std::vector<double> pe = potentials();
+        for (size_t k = 0; k < n3; ++k) g[k] = mobile[k % nb] ? -(fbuf[k] * kFixInv) : 0.0;
+        double e = 0;
+        for (double v : pe) e += v;
+        return e;
+    };
+    auto converged = [&](const std::vector<double>& g) {
+        for (int s = 0; s < d_.nsys; ++s) {
             const HostSystem& H = sys_[s];
             double ss = 0.0;
             for (int i = 0; i < H.n; ++i) {
                 const size_t b = bead_begin_[s] + i;
-                if (H.masses[i] == 0.0) continue;
-                for (int c = 0; c < 3; ++c) {
-                    const double v = f[c * static_cast<size_t>(d_.nb) + b] * kFixInv;
-                    ss += v * v;
-                }
+                for (int c = 0; c < 3; ++c) ss += g[c * nb + b] * g[c * nb + b];
             }
-            if (std::sqrt(ss / (3.0 * H.n)) > tol) done = false;
+            if (H.n > 0 && std::sqrt(ss / (3.0 * H.n)) > tol) return false;
         }
-        if (done || it >= max_iter) break;
-        const int burst = std::min(50, max_iter - it);
-        for (int k = 0; k < burst; ++k) {
-            launch_relax(d_, stream_, alpha, dmax);
-            enqueue_forces(d_.f, true, 0xFFFFFFFFu);
+        return true;
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double r = 0;
+        for (size_t k = 0; k < n3; ++k) r += a[k] * b[k];
+        return r;
+    };
+
+    std::vector<double> x(n3), g(n3), d(n3), xn(n3), gn(n3), q(n3);
+    sync();
+    check_hip(hipMemcpy(x.data(), d_.px, n3 * sizeof(double), hipMemcpyDeviceToHost), "positions");
+    double e = evaluate(x, g);
+    std::vector<std::vector<double>> S, Y;
+    std::vector<double> rho;
+    int it = 0;
+    while (!converged(g) && it < max_iter) {
+        // two-loop recursion: d = -H g
+        q = g;
+        std::vector<double> alpha(S.size());
+        for (int k = static_cast<int>(S.size()) - 1; k >= 0; --k) {
+            alpha[k] = rho[k] * dot(S[k], q);
+            for (size_t j = 0; j < n3; ++j) q[j] -= alpha[k] * Y[k][j];
         }
-        it += burst;
-        sync();
-        int flags = 0;
-        check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
-        if (flags & kFlagListOverflow) {
-            grow_rows();
-            check_hip(hipMemset(d_.flags, 0, sizeof(int)), "memset");
-            launch_zero_forces(d_, d_.f, stream_);
-            enqueue_forces(d_.f, true, 0xFFFFFFFFu);
+        const double gamma = S.empty() ? 1.0 : dot(S.back(), Y.back()) / dot(Y.back(), Y.back());
+        for (size_t j = 0; j < n3; ++j) q[j] *= gamma;
+        for (size_t k = 0; k < S.size(); ++k) {
+            const double beta = rho[k] * dot(Y[k], q);
+            for (size_t j = 0; j < n3; ++j) q[j] += (alpha[k] - beta) * S[k][j];
         }
+        for (size_t j = 0; j < n3; ++j) d[j] = -q[j];
+        double gd = dot(g, d);
+        if (!(gd < 0.0)) {   // not a descent direction: restart from steepest descent
+            S.clear(); Y.clear(); rho.clear();
+            for (size_t j = 0; j < n3; ++j) d[j] = -g[j];
+            gd = dot(g, d);
+        }
+        // cap the largest bead displacement
+        double dmax2 = 0.0;
+        for (size_t b = 0; b < nb; ++b)
+            dmax2 = std::max(dmax2, d[b] * d[b] + d[nb + b] * d[nb + b] + d[2 * nb + b] * d[2 * nb + b]);
+        double t = dmax2 > kMaxMove * kMaxMove ? kMaxMove / std::sqrt(dmax2) : 1.0;
+        bool accepted = false;
+        double en = e;
+        for (int ls = 0; ls < 30; ++ls) {
+            for (size_t j = 0; j < n3; ++j) xn[j] = x[j] + t * d[j];
+            en = evaluate(xn, gn);
+            if (std::isfinite(en) && en <= e + 1e-4 * t * gd) { accepted = true; break; }
+            t *= 0.5;
+        }
+        ++it;
+        if (!accepted) {
+            // no decrease along d even for tiny steps: restart the history, or
+            // stop if this already was steepest descent
+            check_hip(hipMemcpy(d_.px, x.data(), n3 * sizeof(double), hipMemcpyHostToDevice), "positions");
+            if (S.empty()) break;
+            S.clear(); Y.clear(); rho.clear();
+            continue;
+        }
+        std::vector<double> sv(n3), yv(n3);
+        for (size_t j = 0; j < n3; ++j) { sv[j] = xn[j] - x[j]; yv[j] = gn[j] - g[j]; }
+        const double sy = dot(sv, yv);
+        if (sy > 1e-12 * std::sqrt(dot(sv, sv) * dot(yv, yv))) {
+            if (static_cast<int>(S.size()) == kHistory) { S.erase(S.begin()); Y.erase(Y.begin()); rho.erase(rho.begin()); }
+            S.push_back(std::move(sv));
+            Y.push_back(std::move(yv));
+            rho.push_back(1.0 / sy);
+        }
+        x.swap(xn);
+        g.swap(gn);
+        e = en;
     }
+    // the device holds the last evaluated point; leave it at the accepted one
+    check_hip(hipMemcpy(d_.px, x.data(), n3 * sizeof(double), hipMemcpyHostToDevice), "positions");
     // relaxation may pass through flagged overlaps; stepping starts from a
     // fresh evaluation at the relaxed positions
     check_hip(hipMemset(d_.err, 0, sizeof(int) * d_.nsys), "memset");

@@ -75,6 +75,7 @@ public:
     // Capped steepest descent until the RMS force component of every system
     // is <= tol or max_iter iterations ran.  Returns iterations done.
     int minimize(double tol, int max_iter);
+    std::vector<double> potentials();   // per system, FP64 energy pass on the current list
     void get_state(int s, int what, uint32_t groups, ocm_state* out);
     long long neighbor_pairs(int s, int force_index, int32_t* out, long long cap);
     int system_status(int s) const { return sys_status_.at(s); }

@@ -1614,21 +1614,6 @@ __global__ void k_velocity_noise(int nb, int sys, const int* __restrict__ sys_of
 
 // Capped steepest-descent move: x += min(alpha, dmax/|F|) F for mobile beads;
 // clears the force for the next evaluation.
-__global__ void k_relax(int nb, const double* __restrict__ inv_m, double* __restrict__ px,
-                        double* __restrict__ py, double* __restrict__ pz, long long* __restrict__ fx,
-                        long long* __restrict__ fy, long long* __restrict__ fz, double alpha, double dmax) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const double f0 = fx[b] * kFixInv, f1 = fy[b] * kFixInv, f2 = fz[b] * kFixInv;
-    fx[b] = 0; fy[b] = 0; fz[b] = 0;
-    if (inv_m[b] == 0.0) return;
-    const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-    if (!(fm > 0.0)) return;
-    const double s = fmin(alpha, dmax / fm);
-    px[b] += s * f0;
-    py[b] += s * f1;
-    pz[b] += s * f2;
-}
 
 // ------------------------------------------------------------ launchers
 static inline int nblk(long long n, int b) { return static_cast<int>((n + b - 1) / b); }
@@ -1823,8 +1808,5 @@ void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT,
                                                       d.vx, d.vy, d.vz);
 }
 
-void launch_relax(const DevPtrs& d, hipStream_t st, double alpha, double dmax) {
-    k_relax<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.inv_m, d.px, d.py, d.pz, d.f.fx, d.f.fy, d.f.fz, alpha, dmax);
-}
 
 }  // namespace ocm

@@ -108,7 +108,6 @@ void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int n
 void launch_zero_energy(const DevPtrs& d, hipStream_t st);
 void launch_kinetic(const DevPtrs& d, hipStream_t st);
 void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members);
-void launch_relax(const DevPtrs& d, hipStream_t st, double alpha, double dmax);
 void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed);
 
 }  // namespace ocm
