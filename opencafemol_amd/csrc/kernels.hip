@@ -757,7 +757,7 @@ template <bool ENERGY, typename R>
 __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double* __restrict__ px,
                                    const double* __restrict__ py, const double* __restrict__ pz,
                                    const FlaTable* __restrict__ fla, const FldTable* __restrict__ fld,
-                                   const ForceOut& out, long long* __restrict__ energy, int sys,
+                                   const ForceOut& out, long long* __restrict__ efix, int sys,
                                    int& bad) {
     const bool pbc = (t.meta >> 5) & 1;
     const int group = t.meta & 31;
@@ -889,9 +889,25 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
     default:
         break;
     }
-    if (ENERGY) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&energy[sys * 32 + group]),
-                  static_cast<unsigned long long>(to_fix_d(static_cast<double>(e), bad)));
+    if (ENERGY) *efix = to_fix_d(static_cast<double>(e), bad);   // the caller adds it to group `group` of `sys`
+}
+
+// Adds v to energy[key] for every lane with key >= 0: lanes with the same key
+// are summed across the wavefront first, so there is one atomic per distinct
+// key instead of one per lane (energies of a pass land on a few words).
+// Every lane of the wavefront must call it.
+__device__ inline void wave_add_energy(long long* energy, int key, long long v) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    unsigned long long todo = __ballot(key >= 0);
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const bool mine = key == lk;
+        const unsigned long long peers = __ballot(mine) & todo;
+        const long long sum = warp_sum(mine ? v : 0LL, 64);
+        if (lane == leader)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&energy[lk]), static_cast<unsigned long long>(sum));
+        todo &= ~peers;
     }
 }
 
@@ -903,17 +919,24 @@ __global__ void k_bonded(int nt, const BTerm* __restrict__ terms, const DevSys* 
                          ForceOut out, long long* __restrict__ energy, uint32_t groups,
                          int* __restrict__ err) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    const BTerm T = terms[t];
-    if (!((groups >> (T.meta & 31)) & 1u)) return;
-    const int s = sys_of[T.idx[0]];
-    const DevSys S = sys[s];
-    int bad = 0;
-    if (ENERGY)
-        bonded_term<true, double>(T, S, px, py, pz, fla, fld, out, energy, s, bad);
-    else
-        bonded_term<false, float>(T, S, px, py, pz, fla, fld, out, energy, s, bad);
-    if (bad) atomicOr(&err[s], kErrNumeric);
+    long long ef = 0;
+    int key = -1;   // energy word of this term (ENERGY)
+    if (t < nt) {
+        const BTerm T = terms[t];
+        if ((groups >> (T.meta & 31)) & 1u) {
+            const int s = sys_of[T.idx[0]];
+            const DevSys S = sys[s];
+            int bad = 0;
+            if (ENERGY) {
+                bonded_term<true, double>(T, S, px, py, pz, fla, fld, out, &ef, s, bad);
+                key = s * 32 + (T.meta & 31);
+            } else {
+                bonded_term<false, float>(T, S, px, py, pz, fla, fld, out, nullptr, s, bad);
+            }
+            if (bad) atomicOr(&err[s], kErrNumeric);
+        }
+    }
+    if (ENERGY) wave_add_energy(energy, key, ef);
 }
 
 // ------------------------------------------------------------------- pair
@@ -1400,7 +1423,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
                     const int s = bd.sys_of[T.idx[0]];
                     int bad = 0;
                     bonded_term<false, float>(T, bd.sys[s], bd.px, bd.py, bd.pz, bd.fla, bd.fld,
-                                              ForceOut{out.fx, out.fy, out.fz}, energy, s, bad);
+                                              ForceOut{out.fx, out.fy, out.fz}, nullptr, s, bad);
                     if (bad) atomicOr(&err[s], kErrNumeric);
                 }
             }
@@ -1496,11 +1519,13 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
         }
     }
     if (ENERGY) {
-        ae0 = warp_sum(ae0, TEAM);
-        ae1 = warp_sum(ae1, TEAM);
-        ae2 = warp_sum(ae2, TEAM);
-        ae3 = warp_sum(ae3, TEAM);
-        if (lane == 0 && bead_i >= 0) {
+        // every row of a wavefront belongs to this block's group: one sum and
+        // one atomic per kind for the whole wavefront
+        ae0 = warp_sum(ae0, 64);
+        ae1 = warp_sum(ae1, 64);
+        ae2 = warp_sum(ae2, 64);
+        ae3 = warp_sum(ae3, 64);
+        if ((threadIdx.x & 63u) == 0) {
             const long long v[4] = {ae0, ae1, ae2, ae3};
 #pragma unroll
             for (int k = 0; k < 4; ++k)
@@ -1625,11 +1650,15 @@ void launch_zero_forces(const DevPtrs& d, const ForceBuf& f, hipStream_t st) {
     k_fill_i64<<<nblk(d.nb, 256), 256, 0, st>>>(f.fz, 0, d.nb);
 }
 
+// Small workgroups: a step touches each bead once, so the launch is one
+// latency chain per wave, and more workgroups reach more CUs.
+constexpr int kIntegrateBlock = 64;
+
 void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check,
                       int prune) {
     PackArgs pk{pack && d.nm > 0, check, d.maxm, d.bead_slot, d.bead_hs2, d.refx, d.refy, d.refz, d.rec,
                 d.rec32, d.prx, d.pry, d.prz, d.prune_hm2, d.nm > 0 && d.col_in ? prune : kPruneNone};
-    k_integrate<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
+    k_integrate<<<nblk(d.nb, kIntegrateBlock), kIntegrateBlock, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
                                                  d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
                                                  pin, pout, pending, d.err, pk, d.sys0, d.nsys == 1 ? 1 : 0);
 }
