@@ -139,7 +139,10 @@ typedef struct ocm_options {
     double  skin;              /* nm, neighbor-list buffer beyond cutoff     */
     int32_t team_size;         /* lanes per pair-list row: 4,8,16,32,64; 0 = auto */
     int32_t use_graph;         /* capture step chunks as hipGraphs           */
-    int32_t chunk_steps;       /* steps between host checks of device flags  */
+    int32_t chunk_steps;       /* steps between host checks of device flags;
+                                  over 2^17 pair-group members at most
+                                  max(2 * rebuild_interval, 32), since a
+                                  stale list redoes the whole chunk         */
     int32_t prune_interval;    /* steps between prunes of the inner list
                                   (cutoff + 0.1 nm, capped by the skin);
                                   <= 1: every pair pass reads the full list  */

@@ -11,6 +11,10 @@
 
 namespace ocm {
 
+// Pair-group members above which a system counts as large: 64-row list-build
+// work items, and chunks capped at two list lifetimes.
+constexpr int kLargeMembers = 1 << 17;
+
 // ------------------------------------------------------------ kind tables
 bool is_bonded(int k) { return k >= OCM_HARMONIC_BOND && k <= OCM_FLEXIBLE_LOCAL_DIHEDRAL; }
 bool is_pair(int k) { return k >= OCM_EXCLUDED_VOLUME && k <= OCM_LJ_ATTRACTIVE; }
@@ -217,6 +221,11 @@ Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(s
         throw Error(OCM_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     pack();
+    // A stale list rolls the whole chunk back.  For large systems (a step of
+    // milliseconds) chunks are capped at two list lifetimes, so a rollback
+    // costs at most that; the host check per chunk stays negligible beside it.
+    chunk_ = opt_.chunk_steps;
+    if (d_.nm > kLargeMembers) chunk_ = std::min(chunk_, std::max(2 * opt_.rebuild_interval, 32));
 }
 
 Engine::~Engine() {
@@ -517,7 +526,7 @@ void Engine::pack() {
     // large lists: 64-row work items stage each stencil once for a whole cell
     // (there are still far more items than workgroups); small ones keep 16-row
     // items so the build spreads over the device
-    d_.tile_chunk = d_.nm > (1 << 17) ? 64 : 16;
+    d_.tile_chunk = d_.nm > kLargeMembers ? 64 : 16;
     d_.ncell = std::max(ncell_total, 1);
     d_.ngroups = static_cast<int>(groups.size());
     d_.max_types = std::max(max_types, 1);
@@ -935,7 +944,7 @@ int Engine::run_chunk(int k) {
         const int prune = prune_ > 1 ? prune_ : 0;
         const auto key = std::make_tuple(k, timing_mode_ == 2 ? -interval : interval, prune);
         const bool graph = opt_.use_graph && !timing_ &&
-                           (k == opt_.chunk_steps || graphs_.count(key) || graphs_.size() < kMaxGraphs);
+                           (k == chunk_ || graphs_.count(key) || graphs_.size() < kMaxGraphs);
         if (graph) {
             auto it = graphs_.find(key);
             if (it == graphs_.end()) {
@@ -1028,7 +1037,7 @@ int Engine::step(long long n) {
     int status = OCM_OK;
     long long remaining = n;
     while (remaining > 0) {
-        const int k = static_cast<int>(std::min<long long>(remaining, opt_.chunk_steps));
+        const int k = static_cast<int>(std::min<long long>(remaining, chunk_));
         const int st = run_chunk(k);
         if (st != OCM_OK) status = st;
         remaining -= k;

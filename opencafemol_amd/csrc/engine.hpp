@@ -136,6 +136,7 @@ private:
     std::vector<int64_t> sys_step_origin_;
     bool forces_dirty_ = true;
     int interval_ = 20;
+    int chunk_ = 200;           // steps per chunk (chunk_steps, capped for large systems)
     int clean_chunks_ = 0;      // chunks since the interval was last shortened
     int recover_after_ = 4;     // clean chunks before the interval doubles back
     int prune_ = 0;             // steps between inner-list prunes (0: no inner list; 1: suspended)

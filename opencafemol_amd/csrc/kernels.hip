@@ -418,6 +418,54 @@ constexpr float kListMargin = 1.001f;
 __device__ __forceinline__ double wrap_coord(double x, double L, double invL) { return x - L * floor(x * invL); }
 
 constexpr int kRowBlock = 4;   // rows a wave tests against each staged candidate
+constexpr int kExclReg = 2;    // exclusions per row tested during the sweep; rows with more are filtered after
+
+// One wave's sweep of a staged window for kRowBlock rows whose values are
+// wave-uniform (scalar registers).  Window entries past the staged count are
+// padding with no category bit, so every lane tests its candidate without a
+// bounds branch.  A passing candidate is stored through the row's buffer
+// resource at run + (passing lanes below it); the resource's range is the
+// row's capacity, so a position past it is dropped by the hardware, and lanes
+// that do not pass store to an offset past every range.
+struct NlRow {
+    float x, y, z;
+    uint32_t amask;        // categories this row's category pairs with
+    int self;              // the row's own slot
+    int ex[kExclReg];      // excluded partner beads (-2: none; a row with more is filtered after)
+    __amdgpu_buffer_rsrc_t out;
+};
+
+template <bool MI>
+__device__ __forceinline__ void nl_sweep(const float4* __restrict__ cpos, const int2* __restrict__ cinfo, int nw,
+                                         int lane, NlRow (&rw)[kRowBlock], int (&run)[kRowBlock],
+                                         float rl2f, float Lf0, float Lf1, float Lf2, float iLf0, float iLf1,
+                                         float iLf2, bool mi0, bool mi1, bool mi2) {
+    for (int kb = 0; kb < nw; kb += 64) {
+        const int k = kb + lane;
+        const float4 c = cpos[k];
+        const int2 inf = cinfo[k];   // {category bit, bead}
+        const int sl = __float_as_int(c.w);
+#pragma unroll
+        for (int q = 0; q < kRowBlock; ++q) {
+            float dx = c.x - rw[q].x, dy = c.y - rw[q].y, dz = c.z - rw[q].z;
+            if (MI) {
+                if (mi0) dx -= Lf0 * rintf(dx * iLf0);
+                if (mi1) dy -= Lf1 * rintf(dy * iLf1);
+                if (mi2) dz -= Lf2 * rintf(dz * iLf2);
+            }
+            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            bool pass = (d2 < rl2f) & (sl != rw[q].self) & ((rw[q].amask & static_cast<uint32_t>(inf.x)) != 0u);
+#pragma unroll
+            for (int e = 0; e < kExclReg; ++e) pass = pass & (inf.y != rw[q].ex[e]);
+            const unsigned long long mask = __builtin_amdgcn_ballot_w64(pass);
+            const int below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+            const int off = pass ? (run[q] + below) * 4 : static_cast<int>(0x80000000u);
+            __builtin_amdgcn_raw_buffer_store_b32(sl, rw[q].out, off, 0, 0);
+            run[q] += __popcll(mask);
+        }
+    }
+}
 
 __global__ __launch_bounds__(256) void k_nl_tiles(
     const int2* __restrict__ items, const int* __restrict__ n_items, const DevPairGroup* __restrict__ groups,
@@ -427,11 +475,12 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     const int* __restrict__ excl, const int* __restrict__ row_start, int* __restrict__ row_count,
     int* __restrict__ group_need, int* __restrict__ col, int* __restrict__ flags, int chunk) {
     __shared__ float4 cpos[kTileCap];      // x, y, z relative to the item's first row; w = slot (bits)
-    __shared__ int2 cinfo[kTileCap];       // {type word, bead}
+    __shared__ int2 cinfo[kTileCap];       // {1 << category, bead}; padding {0, -1}
     __shared__ int row_run[kTileChunkMax];
     __shared__ int st_start[kMaxStencil], st_pre[kMaxStencil + 1];
     __shared__ float st_shift[kMaxStencil][3];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, nwave = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nit = *n_items;
     for (int item = blockIdx.x; item < nit; item += gridDim.x) {
     __syncthreads();  // LDS of the previous item is no longer read
@@ -483,12 +532,17 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     }
     __syncthreads();
     const int ntot = st_pre[nst];
-    const unsigned long long below = (1ull << lane) - 1ull;
     for (int r = tid; r < nr; r += blockDim.x) row_run[r] = 0;
     for (int wbeg = 0; wbeg < ntot; wbeg += kTileCap) {
         const int nw = min(kTileCap, ntot - wbeg);
+        const int nw64 = (nw + 63) & ~63;
         __syncthreads();
-        for (int k = tid; k < nw; k += blockDim.x) {
+        for (int k = tid; k < nw64; k += blockDim.x) {
+            if (k >= nw) {   // padding: no category bit, never passes
+                cpos[k] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                cinfo[k] = make_int2(0, -1);
+                continue;
+            }
             const int flat = wbeg + k;
             int lo = 0, hi = nst - 1;  // last stencil cell with st_pre <= flat
             while (lo < hi) {
@@ -506,87 +560,69 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
             cpos[k] = make_float4(static_cast<float>(x - o0) + st_shift[lo][0],
                                   static_cast<float>(y - o1) + st_shift[lo][1],
                                   static_cast<float>(z - o2) + st_shift[lo][2], __int_as_float(sl));
-            cinfo[k] = make_int2(r.type, r.bead);
+            cinfo[k] = make_int2(static_cast<int>(1u << (r.type >> 16)), r.bead);
         }
         __syncthreads();
         // a wave takes kRowBlock rows at once: each staged candidate is read
         // from LDS once and tested against all of them
-        for (int r0 = wave * kRowBlock; r0 < nr; r0 += nwave * kRowBlock) {
-            float xi[kRowBlock], yi[kRowBlock], zi[kRowBlock];
-            uint32_t amask[kRowBlock];
-            int base[kRowBlock], run[kRowBlock], self[kRowBlock];
+        for (int rb = wave * kRowBlock; rb < nr; rb += nwave * kRowBlock) {
+            NlRow rw[kRowBlock];
+            int run[kRowBlock];
 #pragma unroll
             for (int q = 0; q < kRowBlock; ++q) {
-                const int r = r0 + q;
+                const int r = rb + q;
                 if (r < nr) {
                     const int i = h0 + r;
                     const PairRec ri = rec[i];
-                    xi[q] = static_cast<float>((periodic ? wrap_coord(ri.x, L[0], iL[0]) : ri.x) - o0);
-                    yi[q] = static_cast<float>((periodic ? wrap_coord(ri.y, L[1], iL[1]) : ri.y) - o1);
-                    zi[q] = static_cast<float>((periodic ? wrap_coord(ri.z, L[2], iL[2]) : ri.z) - o2);
-                    amask[q] = G.allow[ri.type >> 16];
-                    base[q] = row_start[i];
-                    run[q] = row_run[r];
-                    self[q] = i;
-                } else {
-                    xi[q] = yi[q] = zi[q] = 0.f;
-                    amask[q] = 0u;   // no candidate passes
-                    base[q] = run[q] = 0;
-                    self[q] = -1;
-                }
-            }
-            for (int kb = 0; kb < nw; kb += 64) {
-                const int k = kb + lane;
-                float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-                int2 inf = make_int2(0, -1);
-                int sl = -1;
-                if (k < nw) {
-                    c = cpos[k];
-                    inf = cinfo[k];
-                    sl = __float_as_int(c.w);
-                }
+                    rw[q].x = static_cast<float>((periodic ? wrap_coord(ri.x, L[0], iL[0]) : ri.x) - o0);
+                    rw[q].y = static_cast<float>((periodic ? wrap_coord(ri.y, L[1], iL[1]) : ri.y) - o1);
+                    rw[q].z = static_cast<float>((periodic ? wrap_coord(ri.z, L[2], iL[2]) : ri.z) - o2);
+                    rw[q].amask = G.allow[ri.type >> 16];
+                    rw[q].self = i;
+                    rw[q].out = __builtin_amdgcn_make_buffer_rsrc(col + row_start[i], 0, cap * 4, 0x00020000);
+                    const int m = slot_member[i];
+                    const int eb = excl_start[m], ec = excl_count[m];
 #pragma unroll
-                for (int q = 0; q < kRowBlock; ++q) {
-                    bool pass = false;
-                    if (k < nw) {
-                        float dx = c.x - xi[q], dy = c.y - yi[q], dz = c.z - zi[q];
-                        if (any_mi) {
-                            if (mi0) dx -= Lf0 * rintf(dx * iLf0);
-                            if (mi1) dy -= Lf1 * rintf(dy * iLf1);
-                            if (mi2) dz -= Lf2 * rintf(dz * iLf2);
-                        }
-                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        pass = sl != self[q] && ((amask[q] >> (inf.x >> 16)) & 1u) && d2 < rl2f;
-                    }
-                    const unsigned long long mask = __ballot(pass);
-                    const int pos = run[q] + __popcll(mask & below);
-                    if (pass && pos < cap) col[base[q] + pos] = sl;
-                    run[q] += __popcll(mask);
+                    for (int e = 0; e < kExclReg; ++e) rw[q].ex[e] = e < ec ? excl[eb + e] : -2;
+                    run[q] = row_run[r];
+                } else {
+                    rw[q].x = rw[q].y = rw[q].z = 0.f;
+                    rw[q].amask = 0u;   // no candidate passes
+                    rw[q].self = -1;
+                    rw[q].out = __builtin_amdgcn_make_buffer_rsrc(col, 0, 0, 0x00020000);
+#pragma unroll
+                    for (int e = 0; e < kExclReg; ++e) rw[q].ex[e] = -2;
+                    run[q] = 0;
                 }
             }
+            if (any_mi)
+                nl_sweep<true>(cpos, cinfo, nw, lane, rw, run, rl2f, Lf0, Lf1, Lf2, iLf0, iLf1, iLf2, mi0, mi1,
+                               mi2);
+            else
+                nl_sweep<false>(cpos, cinfo, nw, lane, rw, run, rl2f, Lf0, Lf1, Lf2, iLf0, iLf1, iLf2, mi0,
+                                mi1, mi2);
 #pragma unroll
             for (int q = 0; q < kRowBlock; ++q) {
-                const int r = r0 + q;
+                const int r = rb + q;
                 if (r >= nr) continue;
                 if (lane == 0) row_run[r] = run[q];
                 // last window: pad the row to a multiple of 4 entries with -1
                 if (wbeg + kTileCap >= ntot) {
                     const int padded = (run[q] + 3) & ~3;
-                    if (padded <= cap && lane < padded - run[q]) col[base[q] + run[q] + lane] = -1;
+                    if (padded <= cap && lane < padded - run[q]) col[row_start[h0 + r] + run[q] + lane] = -1;
                 }
             }
         }
     }
     __syncthreads();
-    // excluded partners: the written entries of each row whose bead is in the
-    // row's exclusion list become -1, which every reader skips (a few per row,
-    // against a test per candidate in the loop above)
+    // rows with more than kExclReg excluded partners: the written entries whose
+    // bead is in the row's exclusion list become -1, which every reader skips
     for (int r = wave; r < nr; r += nwave) {
         const int i = h0 + r;
         const int m = slot_member[i];
         const int eb = excl_start[m], ec = excl_count[m];
         const int n = min(row_run[r], cap);
-        if (ec == 0 || n == 0) continue;
+        if (ec <= kExclReg || n == 0) continue;
         const int base = row_start[i];
         const int mine = lane < ec ? excl[eb + lane] : -1;   // one partner per lane when <= 64
         for (int e = lane; e < n; e += 64) {
