@@ -318,20 +318,25 @@ __global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
     }
     const int cell = G.cell_begin + (c[2] * G.ncell[1] + c[1]) * G.ncell[0] + c[0];
     member_cell[m] = cell;
-    // consecutive members (beads along a chain) mostly share a cell: one
-    // atomic per distinct cell of the wavefront instead of one per member
+    // consecutive members (beads along a chain) mostly share a cell: each run
+    // of equal cells in the wavefront takes its ranks with one atomic, and all
+    // runs issue theirs together.  Active lanes are a prefix of the wave (only
+    // m >= nm leave), so lane - 1 is active whenever lane is.
     const int lane = threadIdx.x & 63;
-    unsigned long long todo = __ballot(true);
-    while (todo) {
-        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
-        const int lc = __shfl(cell, leader, 64);
-        const unsigned long long peers = __ballot(cell == lc) & todo;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&cell_count[lc], __popcll(peers));
-        base = __shfl(base, leader, 64);
-        if (cell == lc) member_rank[m] = base + __popcll(peers & ((1ull << lane) - 1ull));
-        todo &= ~peers;
-    }
+    const unsigned long long active = __builtin_amdgcn_ballot_w64(true);
+    const int prev = __shfl_up(cell, 1, 64);
+    const bool lead = lane == 0 || prev != cell;
+    const unsigned long long leaders = __builtin_amdgcn_ballot_w64(lead);
+    // this lane's run starts at the highest leader at or below it and ends
+    // before the next leader (or the last active lane)
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int first = 63 - __clzll(static_cast<long long>(leaders & upto));
+    const unsigned long long after = leaders & ~upto;
+    const int end = after ? __ffsll(static_cast<long long>(after)) - 1 : 64 - __clzll(static_cast<long long>(active));
+    int base = 0;
+    if (lead) base = atomicAdd(&cell_count[cell], end - lane);
+    base = __shfl(base, first, 64);
+    member_rank[m] = base + lane - first;
 }
 
 __global__ void k_place(int nm, const DevPairGroup* __restrict__ groups, const int* __restrict__ member_group,
@@ -435,7 +440,9 @@ struct NlRow {
     __amdgpu_buffer_rsrc_t out;
 };
 
-template <bool MI>
+// CAT: the group has more than one category (test the allow mask; with one,
+// only padding, whose category bit is 0, fails).
+template <bool MI, bool CAT>
 __device__ __forceinline__ void nl_sweep(const float4* __restrict__ cpos, const int2* __restrict__ cinfo, int nw,
                                          int lane, NlRow (&rw)[kRowBlock], int (&run)[kRowBlock],
                                          float rl2f, float Lf0, float Lf1, float Lf2, float iLf0, float iLf1,
@@ -454,10 +461,16 @@ __device__ __forceinline__ void nl_sweep(const float4* __restrict__ cpos, const 
                 if (mi2) dz -= Lf2 * rintf(dz * iLf2);
             }
             const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool pass = (d2 < rl2f) & (sl != rw[q].self) & ((rw[q].amask & static_cast<uint32_t>(inf.x)) != 0u);
-#pragma unroll
-            for (int e = 0; e < kExclReg; ++e) pass = pass & (inf.y != rw[q].ex[e]);
-            const unsigned long long mask = __builtin_amdgcn_ballot_w64(pass);
+            // the wave mask is the AND of the tests' own masks (each ballot is
+            // its compare's result), so no per-lane flag is rebuilt for it
+            const bool near = d2 < rl2f, other = sl != rw[q].self;
+            const bool cat = CAT ? (rw[q].amask & static_cast<uint32_t>(inf.x)) != 0u : inf.x != 0;
+            const bool ex0 = inf.y != rw[q].ex[0], ex1 = inf.y != rw[q].ex[1];
+            static_assert(kExclReg == 2, "the sweep tests two exclusions");
+            const unsigned long long mask = __builtin_amdgcn_ballot_w64(near) & __builtin_amdgcn_ballot_w64(other) &
+                                            __builtin_amdgcn_ballot_w64(cat) & __builtin_amdgcn_ballot_w64(ex0) &
+                                            __builtin_amdgcn_ballot_w64(ex1);
+            const bool pass = near & other & cat & ex0 & ex1;
             const int below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
             const int off = pass ? (run[q] + below) * 4 : static_cast<int>(0x80000000u);
@@ -501,6 +514,7 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
     const float Lf0 = static_cast<float>(L[0]), Lf1 = static_cast<float>(L[1]), Lf2 = static_cast<float>(L[2]);
     const float iLf0 = static_cast<float>(iL[0]), iLf1 = static_cast<float>(iL[1]), iLf2 = static_cast<float>(iL[2]);
     const float rl2f = static_cast<float>(G.rl2) * kListMargin;
+    const bool multi_cat = !(G.n_cat <= 1 && (G.allow[0] & 1u));   // else every member pair is allowed
     const int cbeg = G.cell_begin;
     // origin: the item's first row, wrapped into the box on periodic axes
     const PairRec r0 = rec[h0];
@@ -595,12 +609,14 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
                     run[q] = 0;
                 }
             }
-            if (any_mi)
-                nl_sweep<true>(cpos, cinfo, nw, lane, rw, run, rl2f, Lf0, Lf1, Lf2, iLf0, iLf1, iLf2, mi0, mi1,
-                               mi2);
-            else
-                nl_sweep<false>(cpos, cinfo, nw, lane, rw, run, rl2f, Lf0, Lf1, Lf2, iLf0, iLf1, iLf2, mi0,
-                                mi1, mi2);
+#define OCM_SWEEP(MI, CAT) \
+    nl_sweep<MI, CAT>(cpos, cinfo, nw, lane, rw, run, rl2f, Lf0, Lf1, Lf2, iLf0, iLf1, iLf2, mi0, mi1, mi2)
+            if (any_mi) {
+                if (multi_cat) OCM_SWEEP(true, true); else OCM_SWEEP(true, false);
+            } else {
+                if (multi_cat) OCM_SWEEP(false, true); else OCM_SWEEP(false, false);
+            }
+#undef OCM_SWEEP
 #pragma unroll
             for (int q = 0; q < kRowBlock; ++q) {
                 const int r = rb + q;
@@ -1247,6 +1263,20 @@ __device__ __forceinline__ PairRec load_rec(__amdgpu_buffer_rsrc_t rs, int slot)
     return r;
 }
 
+// Column entry e of a row of cnt (> 0) entries, -1 past its end: the load is
+// clamped into the row, so it is issued unconditionally (no branch around it).
+__device__ __forceinline__ int next_col(const int* __restrict__ crow, int e, int cnt) {
+    const int v = crow[min(e, cnt - 1)];
+    return e < cnt ? v : -1;
+}
+
+// FP32 box-wrapped position of slot j (prune input); slot -1 reads past the
+// range and returns zeros.
+__device__ __forceinline__ float4 load_rec32(__amdgpu_buffer_rsrc_t rs, int j) {
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>(static_cast<unsigned>(j) * 16u), 0, 0);
+    return make_float4(__uint_as_float(a[0]), __uint_as_float(a[1]), __uint_as_float(a[2]), 0.f);
+}
+
 struct RowGeom {
     double bx, by, bz, ibx, iby, ibz, rc2, rin2;
     bool periodic;
@@ -1272,6 +1302,8 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
     // of them is used (the scheduling barrier keeps the arithmetic below
     // them), and the next round's column indices are fetched while this
     // round computes.
+    // (the next round's loads stay conditional here: clamped unconditional
+    // loads, as in the prune, measured 4-6% slower in this loop)
     const int* crow = col + beg;
     const __amdgpu_buffer_rsrc_t rs = rec_rsrc(rec, nrec);
     int cn[4];
@@ -1339,10 +1371,10 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
 // an exact FP64 test would keep.  Otherwise the FP64 records and rin2.
 template <int TEAM, bool F32>
 __device__ __forceinline__ void prune_row(const DevPairGroup& G, int row, int lane, unsigned long long team_lanes,
-                                          unsigned long long below, const PairRec* __restrict__ rec,
+                                          const PairRec* __restrict__ rec,
                                           const float4* __restrict__ rec32, const int* __restrict__ row_start,
                                           const int* __restrict__ row_count, const int* __restrict__ col,
-                                          int* __restrict__ row_count_w, int* __restrict__ col_w) {
+                                          int* __restrict__ row_count_w, int* __restrict__ col_w, int nrec) {
     const int beg = row_start[row];
     const int cnt = row_count[row];
     double xi = 0, yi = 0, zi = 0;
@@ -1357,50 +1389,63 @@ __device__ __forceinline__ void prune_row(const DevPairGroup& G, int row, int la
     // every lane of a team runs the same rounds (the loop bound is the
     // row's), so the ballots below see the whole team
     int n = 0;   // entries the team has written to the inner row
+    if (cnt == 0) {
+        if (lane == 0) row_count_w[row] = 0;
+        return;
+    }
     const int* crow = col + beg;
+    int* wrow = col_w + beg;
+    const __amdgpu_buffer_rsrc_t rs32 = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float4*>(rec32), 0, F32 ? nrec * 16 : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs = rec_rsrc(rec, F32 ? 0 : nrec);
     int cn[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
+    for (int u = 0; u < 4; ++u) cn[u] = next_col(crow, lane + u * TEAM, cnt);
     for (int base = 0; base < cnt; base += 4 * TEAM) {
         const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
         double px[4], py[4], pz[4];
         float4 qj[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int j = cs[u] >= 0 ? cs[u] : row;
-            if (F32) qj[u] = rec32[j];
-            else { px[u] = rec[j].x; py[u] = rec[j].y; pz[u] = rec[j].z; }
+            // padding (-1) reads past the buffer's range: zeros, and keep is false
+            if (F32) qj[u] = load_rec32(rs32, cs[u]);
+            else {
+                const PairRec r = load_rec(rs, cs[u]);
+                px[u] = r.x; py[u] = r.y; pz[u] = r.z;
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = base + 4 * TEAM + lane + u * TEAM;
-            cn[u] = e < cnt ? crow[e] : -1;
-        }
+        for (int u = 0; u < 4; ++u) cn[u] = next_col(crow, base + 4 * TEAM + lane + u * TEAM, cnt);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            bool keep;
+            bool in;
             if (F32) {
                 float dx = qj[u].x - qi.x, dy = qj[u].y - qi.y, dz = qj[u].z - qi.z;
                 dx = fmaf(-fbx, rintf(dx * fibx), dx);
                 dy = fmaf(-fby, rintf(dy * fiby), dy);
                 dz = fmaf(-fbz, rintf(dz * fibz), dz);
-                keep = cs[u] >= 0 && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rin2f;
+                in = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rin2f;
             } else {
                 const double dx = min_image(px[u] - xi, G.box[0], G.inv_box[0]);
                 const double dy = min_image(py[u] - yi, G.box[1], G.inv_box[1]);
                 const double dz = min_image(pz[u] - zi, G.box[2], G.inv_box[2]);
-                keep = cs[u] >= 0 && dist2(dx, dy, dz) < G.rin2;
+                in = dist2(dx, dy, dz) < G.rin2;
             }
-            // entry base + u*T + l precedes base + u*T + l' for l < l'
-            const unsigned long long m = __ballot(keep) & team_lanes;
-            if (keep) col_w[beg + n + __popcll(m & below)] = cs[u];
+            // entry base + u*T + l precedes base + u*T + l' for l < l'; the
+            // wave mask is the AND of the two tests' own masks
+            const bool valid = cs[u] >= 0, keep = valid & in;
+            const unsigned long long m =
+                __builtin_amdgcn_ballot_w64(valid) & __builtin_amdgcn_ballot_w64(in) & team_lanes;
+            const int pos = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+            if (keep) wrow[n + pos] = cs[u];
             n += __popcll(m);
         }
     }
     if (lane == 0) {
         int p = n;
-        for (; p & 3; ++p) col_w[beg + p] = -1;
+        for (; p & 3; ++p) wrow[p] = -1;
         row_count_w[row] = p;
     }
 }
@@ -1418,17 +1463,16 @@ __global__ __launch_bounds__(kPairBlock) void k_prune(int nm, const DevPairGroup
                                                       const int* __restrict__ col, int* __restrict__ row_count_w,
                                                       int* __restrict__ col_w) {
     const int wl = static_cast<int>(threadIdx.x & 63u);
-    // the team's lanes within the wave, and those below this lane
+    // the team's lanes within the wave
     const unsigned long long team_lanes = TEAM == 64 ? ~0ull : (((1ull << TEAM) - 1ull) << (wl / TEAM * TEAM));
-    const unsigned long long below = (1ull << wl) - 1ull;
     const int lane = threadIdx.x % TEAM;
     const int row = static_cast<int>((static_cast<long long>(blockIdx.x) * kPairBlock + threadIdx.x) / TEAM);
     if (row >= nm) return;   // whole teams leave together
     const DevPairGroup& G = groups[member_group[row]];
-    if (G.periodic) prune_row<TEAM, true>(G, row, lane, team_lanes, below, rec, rec32, row_start, row_count, col,
-                                          row_count_w, col_w);
-    else prune_row<TEAM, false>(G, row, lane, team_lanes, below, rec, rec32, row_start, row_count, col,
-                                row_count_w, col_w);
+    if (G.periodic) prune_row<TEAM, true>(G, row, lane, team_lanes, rec, rec32, row_start, row_count, col,
+                                          row_count_w, col_w, nm);
+    else prune_row<TEAM, false>(G, row, lane, team_lanes, rec, rec32, row_start, row_count, col,
+                                row_count_w, col_w, nm);
 }
 
 // At least 5 waves per SIMD (<= 96 VGPRs): the record gathers need the latency hiding.
