@@ -203,18 +203,23 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     // a batch of one passes its system record by value: one dependent load less
     const int s = single ? 0 : sys_of[b];
     const DevSys S = single ? sys0 : sys[s];
+    // Every input is loaded before anything is stored or branched on, so the
+    // step is one memory round trip: the bead's state, its force words, and
+    // the record-refresh inputs.  Fixed (massless) beads run the same
+    // arithmetic and keep their position and velocity by selection.
     const double im = inv_m[b];
     const PackIn pin_ = pack_load(pk, b);
-    const double f0 = fx[b] * kFixInv, f1 = fy[b] * kFixInv, f2 = fz[b] * kFixInv;
-    fx[b] = 0; fy[b] = 0; fz[b] = 0;
-    if (im == 0.0) {
-        if (pk.prune == kPruneWrite) {
-            pk.prx[b] = px[b]; pk.pry[b] = py[b]; pk.prz[b] = pz[b];
-        }
-        pack_bead(pk, pin_, b, s, S, px[b], py[b], pz[b], err);
-        return;
-    }
-    double v0 = vx[b], v1 = vy[b], v2 = vz[b];
+    const long long F0 = fx[b], F1 = fy[b], F2 = fz[b];
+    const double X0 = px[b], X1 = py[b], X2 = pz[b];
+    const double V0 = vx[b], V1 = vy[b], V2 = vz[b];
+    const uint32_t lid = static_cast<uint32_t>(local_id[b]);
+    // an empty statement that needs every loaded value: the loads are all
+    // issued before it and waited for once (otherwise the compiler sinks the
+    // mobile-bead inputs behind a wait on the mass and a branch)
+    asm volatile("" ::"v"(im), "v"(F0), "v"(F1), "v"(F2), "v"(X0), "v"(X1), "v"(X2), "v"(V0), "v"(V1), "v"(V2),
+                 "v"(lid));
+    const double f0 = F0 * kFixInv, f1 = F1 * kFixInv, f2 = F2 * kFixInv;
+    double v0 = V0, v1 = V1, v2 = V2;
     if (pending) {
         v0 = half_kick(v0, S.half_dt, f0, im);
         v1 = half_kick(v1, S.half_dt, f1, im);
@@ -223,11 +228,11 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     v0 = half_kick(v0, S.half_dt, f0, im);
     v1 = half_kick(v1, S.half_dt, f1, im);
     v2 = half_kick(v2, S.half_dt, f2, im);
-    double x0 = px[b] + S.half_dt * v0;
-    double x1 = py[b] + S.half_dt * v1;
-    double x2 = pz[b] + S.half_dt * v2;
+    double x0 = X0 + S.half_dt * v0;
+    double x1 = X1 + S.half_dt * v1;
+    double x2 = X2 + S.half_dt * v2;
     double g[3];
-    gauss3_fast(S.seed, static_cast<uint64_t>(step - S.step_origin), static_cast<uint32_t>(local_id[b]), g);
+    gauss3_fast(S.seed, static_cast<uint64_t>(step - S.step_origin), lid, g);
     const double sig = sqrt(S.noise_kT * im);
     v0 = S.c1 * v0 + sig * g[0];
     v1 = S.c1 * v1 + sig * g[1];
@@ -235,14 +240,21 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     x0 += S.half_dt * v0;
     x1 += S.half_dt * v1;
     x2 += S.half_dt * v2;
-    if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) atomicOr(&err[s], kErrNumeric);
+    const bool fixed = im == 0.0;
+    if (fixed) {
+        x0 = X0; x1 = X1; x2 = X2;
+        v0 = V0; v1 = V1; v2 = V2;
+    } else if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) {
+        atomicOr(&err[s], kErrNumeric);
+    }
+    fx[b] = 0; fy[b] = 0; fz[b] = 0;
     px[b] = x0; py[b] = x1; pz[b] = x2;
     vx[b] = v0; vy[b] = v1; vz[b] = v2;
     // inner list: a pair dropped at the prune (r >= cutoff + margin) cannot be
     // inside the cutoff while both beads stay within margin/2 of where they were
     if (pk.prune == kPruneWrite) {
         pk.prx[b] = x0; pk.pry[b] = x1; pk.prz[b] = x2;
-    } else if (pk.prune == kPruneCheck && pin_.slot0 >= 0) {
+    } else if (pk.prune == kPruneCheck && pin_.slot0 >= 0 && !fixed) {
         const double dx = x0 - pin_.qx, dy = x1 - pin_.qy, dz = x2 - pin_.qz;
         if (!(dist2(dx, dy, dz) <= pk.prune_hm2)) atomicOr(&err[s], kErrPrune);
     }
@@ -1353,7 +1365,11 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
             acc.bad |= in && !ok;
             float gg = ok ? f : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
             gg = in ? gg : 0.0f;   // a masked entry adds exactly the bias constant
-            const double gs = static_cast<double>(gg) * 4294967296.0;
+            // scaling by 2^32 is exact in FP32 (the result stays far below
+            // FP32's largest value; a subnormal gg, flushed or not, rounds to
+            // a zero contribution either way), so one FP32 multiply replaces
+            // an FP64 one with the same bits
+            const double gs = static_cast<double>(gg * 4294967296.0f);
             acc.ux += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic)));
             acc.uy += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic)));
             acc.uz += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic)));
