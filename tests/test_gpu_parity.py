@@ -102,6 +102,52 @@ def test_go_neighbor_pairs_bit_exact(oracle):
         assert np.array_equal(got, ref)
 
 
+def test_mixed_exclusion_counts(oracle):
+    """Rows with up to two excluded partners are filtered while the list is
+    swept; rows with more have their entries checked after compaction.  Here
+    each pair group's rows carry their chain exclusions plus up to four random
+    nearby partners, so both paths run in one group: the pair sets
+    must still equal the oracle's bit for bit, and forces must match."""
+    import dataclasses
+    s0, it, x = _small_hps(4)
+    rng = np.random.default_rng(12)
+    s = dataclasses.replace(s0, forces=[], group_ids={})
+    box = np.array(s0.box)
+    extras = {}   # one set per (cutoff, members): the two LJ halves share theirs, as they share a group
+
+    def extra_exclusions(f):
+        members = np.arange(s0.n) if f.category is None else np.where(f.category >= 0)[0]
+        key = (f.cutoff, tuple(members))
+        if key not in extras:
+            out = []
+            for i in members:
+                k = int(rng.integers(0, 5))
+                d = x[members] - x[i]
+                d = np.linalg.norm(d - box * np.rint(d / box), axis=1)
+                near = members[(d < f.cutoff) & (members != i)]
+                for j in rng.choice(near, size=min(k, len(near)), replace=False):
+                    out.append((min(i, j), max(i, j)))
+            extras[key] = np.array(out, np.int32).reshape(-1, 2)
+        return extras[key]
+
+    for f in s0.forces:
+        if f.particle_params is None:
+            s.add(dataclasses.replace(f))
+            continue
+        ex = np.unique(np.concatenate([np.asarray(f.exclusions).reshape(-1, 2), extra_exclusions(f)]), axis=0)
+        s.add(dataclasses.replace(f, exclusions=ex))
+    counts = np.bincount(np.asarray(s.forces[1].exclusions).ravel(), minlength=s.n)
+    assert (counts <= 2).any() and (counts > 2).any()
+    for fi, f in enumerate(s.forces):
+        if f.particle_params is None:
+            continue
+        with _ctx(s, it, x) as c:
+            got = c.neighbor_pairs(fi)
+        ref = oracle.neighbor_pairs(s, fi, x)
+        assert np.array_equal(got, ref), f.name
+    _check_forces_energies(s, it, thermal(x, 4, 0.01), oracle)
+
+
 def test_hps_slab_full_size_forces(oracle):
     # 100 chains x 140 beads in a 15 x 15 x 150 nm periodic box
     s, it, x = configs.hps_slab()
