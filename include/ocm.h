@@ -166,7 +166,7 @@ typedef struct ocm_state {
 
 typedef struct ocm_context ocm_context;
 
-/* Fills `opt` with defaults (skin 0.6 nm, rebuild every 40 steps, inner
+/* Fills `opt` with defaults (skin 0.7 nm, rebuild every 40 steps, inner
  * list pruned every 5 steps, ...).
  * Skin and interval change speed only: forces are fixed-point sums over the
  * pairs inside the cutoff, so any valid list gives the same bits.  When a bead

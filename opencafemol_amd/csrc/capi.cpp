@@ -64,7 +64,7 @@ void ocm_options_default(ocm_options* o) {
     if (!o) return;
     o->device = 0;
     o->rebuild_interval = 40;
-    o->skin = 0.6;
+    o->skin = 0.7;
     o->team_size = 0;
     o->use_graph = 1;
     o->chunk_steps = 200;

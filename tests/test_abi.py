@@ -56,7 +56,7 @@ def test_options_default_and_version():
     lib = abi.load_library()
     o = abi.Options()
     lib.ocm_options_default(C.byref(o))
-    assert (o.rebuild_interval, o.skin, o.use_graph, o.chunk_steps, o.prune_interval) == (40, 0.6, 1, 200, 5)
+    assert (o.rebuild_interval, o.skin, o.use_graph, o.chunk_steps, o.prune_interval) == (40, 0.7, 1, 200, 5)
     assert b"gfx950" in lib.ocm_version()
 
 
