@@ -14,6 +14,8 @@ namespace ocm {
 // Pair-group members above which a system counts as large: 64-row list-build
 // work items, and chunks capped at two list lifetimes.
 constexpr int kLargeMembers = 1 << 17;
+// Systems in a batch from which the pair kernel's table is split into XCD slices.
+constexpr int kXcdSliceSystems = 16;
 
 // ------------------------------------------------------------ kind tables
 bool is_bonded(int k) { return k >= OCM_HARMONIC_BOND && k <= OCM_FLEXIBLE_LOCAL_DIHEDRAL; }
@@ -652,10 +654,26 @@ void Engine::build_pair_blocks(int team) {
     d_.team = team;
     const int rpb = kPairBlock / team;
     std::vector<int2> blocks;
-    // bonded-term chunks first: latency-bound, they start while the pair rows run
-    for (int t = 0; t < d_.nt; t += kPairBlock) blocks.push_back(make_int2(-1, t));
-    for (int g = 0; g < d_.ngroups; ++g) {
-        for (int r = 0; r < group_member_count_[g]; r += rpb) blocks.push_back(make_int2(g, group_member_begin_[g] + r));
+    // Batches of many systems: the pair kernel's workgroups walk the table in
+    // 8 contiguous slices, one per XCD (k_pair's SLICED walk), so the table is
+    // laid out as 8 slices with the same mix of work: each holds an eighth of
+    // the bonded-term chunks (leading: latency-bound, they start while the pair
+    // rows run) and an eighth of every group's rows, that is, whole systems.
+    // Measured: 256 Go replicas +6-8%; a single slab, whose long rows sit in
+    // the middle of the cell order, lost 6% when split, so single systems and
+    // small batches keep one slice and the plain walk.  Cuts fall on item
+    // boundaries.
+    d_.xcd_sliced = d_.nsys >= kXcdSliceSystems ? 1 : 0;
+    const int slices = d_.xcd_sliced ? 8 : 1;
+    const auto cut = [&](long long n_items, int x) { return static_cast<int>(n_items * x / slices); };
+    const long long n_bonded = (d_.nt + kPairBlock - 1) / kPairBlock;
+    for (int x = 0; x < slices; ++x) {
+        for (int c = cut(n_bonded, x); c < cut(n_bonded, x + 1); ++c) blocks.push_back(make_int2(-1, c * kPairBlock));
+        for (int g = 0; g < d_.ngroups; ++g) {
+            const long long n_items = (group_member_count_[g] + rpb - 1) / rpb;
+            for (int c = cut(n_items, x); c < cut(n_items, x + 1); ++c)
+                blocks.push_back(make_int2(g, group_member_begin_[g] + c * rpb));
+        }
     }
     // the block table buffer is reallocated only when it grows
     static_assert(sizeof(int2) == 8, "int2 layout");

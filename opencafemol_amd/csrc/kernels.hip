@@ -58,6 +58,26 @@ __device__ inline T warp_sum(T v, int width) {
     return v;
 }
 
+// XCD-sliced walk of the pair kernel's block table (batches of many
+// systems).  Workgroups are dispatched to the device's 8 XCDs round-robin,
+// and each XCD has its own L2; workgroup b takes entries of slice b % 8, one
+// contiguous eighth of the table that the host fills with whole systems, so
+// table-adjacent work stays together.  Every entry is taken exactly once
+// (a grid smaller than 8 walks the whole table with a plain stride).
+constexpr int kXcds = 8;
+struct XcdWalk {
+    int first, end, stride;
+};
+__device__ __forceinline__ XcdWalk xcd_walk(int n) {
+    const int g = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    if (g < kXcds) return {b, n, g};
+    const int x = b % kXcds, j = b / kXcds;
+    const int nbx = (g - x + kXcds - 1) / kXcds;   // workgroups on XCD x
+    const int lo = static_cast<int>(static_cast<long long>(x) * n / kXcds);
+    const int hi = static_cast<int>(static_cast<long long>(x + 1) * n / kXcds);
+    return {lo + j, hi, nbx};
+}
+
 // ------------------------------------------------------------------ fills
 __global__ void k_fill_i32(int* p, int v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1492,7 +1512,8 @@ __global__ __launch_bounds__(kPairBlock) void k_prune(int nm, const DevPairGroup
 }
 
 // At least 5 waves per SIMD (<= 96 VGPRs): the record gathers need the latency hiding.
-template <bool ENERGY, int TEAM>
+// SLICED: walk the table in XCD slices (xcd_walk); otherwise a plain stride.
+template <bool ENERGY, int TEAM, bool SLICED>
 __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_pair(
     const DevPairGroup* __restrict__ groups, const int2* __restrict__ blocks,
     const PairRec* __restrict__ rec, const int* __restrict__ row_start,
@@ -1505,8 +1526,11 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
     // optional duration stamps: each workgroup stores its start and end on the
     // device's constant-rate wall clock (plain stores to its own two words)
     if (stamp && threadIdx.x == 0) stamp[2 * blockIdx.x] = static_cast<unsigned long long>(wall_clock64());
-    // fixed grid: each workgroup walks the block table with stride gridDim.x
-    for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    // fixed grid: each workgroup walks the block table with stride gridDim.x,
+    // or its XCD's slice of it
+    const XcdWalk walk = SLICED ? xcd_walk(nblocks) : XcdWalk{static_cast<int>(blockIdx.x), nblocks,
+                                                              static_cast<int>(gridDim.x)};
+    for (int bi = walk.first; bi < walk.end; bi += walk.stride) {
     const int2 bt = blocks[bi];
     if (bt.x < 0) {
         // a chunk of bonded terms, one per thread (force pass only): their
@@ -1859,10 +1883,15 @@ static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, u
     const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
     const BondedArgs bd{d.nt, d.terms, d.sys, d.sys_of, d.px, d.py, d.pz, d.fla, d.fld};
     const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
-#define OCM_PAIR(T)                                                                                     \
-    k_pair<E, T><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col,     \
-                                           d.types_f, d.types_d, f, d.energy, groups, d.err,            \
-                                           d.n_pair_blocks, d.nm, stamp, bd)
+#define OCM_PAIR(T)                                                                                       \
+    if (d.xcd_sliced)                                                                                     \
+        k_pair<E, T, true><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col, \
+                                                     d.types_f, d.types_d, f, d.energy, groups, d.err,  \
+                                                     d.n_pair_blocks, d.nm, stamp, bd);                   \
+    else                                                                                                  \
+        k_pair<E, T, false><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col, \
+                                                      d.types_f, d.types_d, f, d.energy, groups, d.err, \
+                                                      d.n_pair_blocks, d.nm, stamp, bd)
     switch (d.team) {
     case 4: OCM_PAIR(4); break;
     case 8: OCM_PAIR(8); break;

@@ -201,6 +201,31 @@ def test_batch_alone_equals_in_mix(oracle):
             assert np.array_equal(st["energy_by_group"], alone[k]["energy_by_group"])
 
 
+def test_batch_of_many_systems_xcd_slices():
+    """From 16 systems a batch lays its pair-kernel table out as 8 XCD slices
+    (each XCD's workgroups walk one contiguous eighth).  Every system must
+    still step to the bits it reaches alone: 20 Go replicas and 2 HPS slabs
+    (periodic, with the inner list) in one batch."""
+    reps = configs.go_replicas(20, n_min=50, n_max=160, seed=3)
+    for seed in (5, 6):
+        s, it, x = _small_hps(seed)
+        reps.append((s, it, x))
+    what = abi.STATE_POSITIONS | abi.STATE_VELOCITIES | abi.STATE_FORCES | abi.STATE_ENERGY
+    alone = []
+    for s, it, x in reps:
+        with _ctx(s, it, x, chunk_steps=60) as c:
+            c.step(150)
+            alone.append(c.get_state(what))
+    with Batch([r[0] for r in reps], [r[1] for r in reps], chunk_steps=60) as b:
+        for k, (s, it, x) in enumerate(reps):
+            b.set_positions(k, x)
+        b.step(150)
+        for k in range(len(reps)):
+            st = b.state(k, what)
+            for key in ("positions", "velocities", "forces", "energy_by_group"):
+                assert np.array_equal(st[key], alone[k][key]), (k, key)
+
+
 def test_rebuild_interval_does_not_change_bits(oracle):
     s, it, x = _small_hps(4)
     res = []
