@@ -655,14 +655,16 @@ void Engine::build_pair_blocks(int team) {
     const int rpb = kPairBlock / team;
     std::vector<int2> blocks;
     // Batches of many systems: the pair kernel's workgroups walk the table in
-    // 8 contiguous slices, one per XCD (k_pair's SLICED walk), so the table is
-    // laid out as 8 slices with the same mix of work: each holds an eighth of
-    // the bonded-term chunks (leading: latency-bound, they start while the pair
-    // rows run) and an eighth of every group's rows, that is, whole systems.
-    // Measured: 256 Go replicas +6-8%; a single slab, whose long rows sit in
-    // the middle of the cell order, lost 6% when split, so single systems and
-    // small batches keep one slice and the plain walk.  Cuts fall on item
-    // boundaries.
+    // 8 contiguous slices, one per XCD (k_pair's SLICED walk), and the table
+    // is laid out as 8 slices with the same mix of work: each holds an eighth
+    // of the bonded-term chunks (leading: latency-bound, they start while the
+    // pair rows run) and an eighth of every group's rows.  Measured on 256 Go
+    // replicas: +5-8% over the plain walk (slices of whole systems, balanced
+    // by estimated work, gained only 2% there and lost 6% on the served mix,
+    // so the gain is not per-system L2 locality alone).  A single slab, whose
+    // long rows sit in the middle of the cell order, lost 6% when split, so
+    // single systems and small batches keep one slice and the plain walk.
+    // Cuts fall on item boundaries.
     d_.xcd_sliced = d_.nsys >= kXcdSliceSystems ? 1 : 0;
     const int slices = d_.xcd_sliced ? 8 : 1;
     const auto cut = [&](long long n_items, int x) { return static_cast<int>(n_items * x / slices); };

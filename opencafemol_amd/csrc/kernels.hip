@@ -61,9 +61,10 @@ __device__ inline T warp_sum(T v, int width) {
 // XCD-sliced walk of the pair kernel's block table (batches of many
 // systems).  Workgroups are dispatched to the device's 8 XCDs round-robin,
 // and each XCD has its own L2; workgroup b takes entries of slice b % 8, one
-// contiguous eighth of the table that the host fills with whole systems, so
-// table-adjacent work stays together.  Every entry is taken exactly once
-// (a grid smaller than 8 walks the whole table with a plain stride).
+// contiguous eighth of the table (the host gives every slice the same mix of
+// work), so table-adjacent work stays on one XCD.  Every entry is taken
+// exactly once (a grid smaller than 8 walks the whole table with a plain
+// stride).
 constexpr int kXcds = 8;
 struct XcdWalk {
     int first, end, stride;

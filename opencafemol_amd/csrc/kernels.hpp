@@ -69,7 +69,7 @@ struct DevPtrs {
     int max_types = 0;
     int2* pair_blocks = nullptr;   // (group, first slot) per pair-kernel block
     int n_pair_blocks = 0;
-    int xcd_sliced = 0;            // pair_blocks holds 8 contiguous XCD slices (batches of many systems)
+    int xcd_sliced = 0;            // pair_blocks holds 8 contiguous XCD slices of equal work mix (batches of many systems)
     int team = 16;
     int prune_team = 32;           // lanes per row of the inner-list prune
     long long* energy = nullptr;   // nsys * 32 fixed point
