@@ -145,7 +145,9 @@ typedef struct ocm_options {
                                   stale list redoes the whole chunk         */
     int32_t prune_interval;    /* steps between prunes of the inner list
                                   (cutoff + 0.1 nm, capped by the skin);
-                                  <= 1: every pair pass reads the full list  */
+                                  <= 1: every pair pass reads the full list,
+                                  as does a batch whose list is estimated
+                                  under 16,384 entries                      */
 } ocm_options;
 
 /* getState selection bits */

@@ -14,6 +14,8 @@ namespace ocm {
 // Pair-group members above which a system counts as large: 64-row list-build
 // work items, and chunks capped at two list lifetimes.
 constexpr int kLargeMembers = 1 << 17;
+// Estimated list entries (whole batch) below which no inner list is kept.
+constexpr long long kPruneMinEntries = 1 << 14;
 // Systems in a batch from which the pair kernel's table is split into XCD slices.
 constexpr int kXcdSliceSystems = 16;
 
@@ -524,6 +526,12 @@ void Engine::pack() {
     }
     if (member_bead.size() >= (1u << 26))   // record offsets are 32-bit byte offsets
         throw Error(OCM_ERR_OOM, "more than 2^26 pair-group members");
+    // a small list is read in the force pass's launch latency whatever its
+    // length, so a prune launch cannot pay (62-bead Go chain: 8.95 -> 8.34
+    // us/step without it); such batches read the full list
+    long long est_entries = 0;
+    for (long long e : group_list_estimate_) est_entries += e;
+    if (est_entries < kPruneMinEntries) prune_ = 0;
     d_.nm = static_cast<int>(member_bead.size());
     // large lists: 64-row work items stage each stencil once for a whole cell
     // (there are still far more items than workgroups); small ones keep 16-row

@@ -83,3 +83,13 @@ def test_batch_and_go_chain_with_inner_list():
     for a, c in zip(*out):
         assert np.array_equal(a["positions"], c["positions"])
         assert np.array_equal(a["velocities"], c["velocities"])
+
+
+def test_small_list_keeps_no_inner_list():
+    # a 62-bead chain's whole list is read within the force pass's launch
+    # latency: no inner list is kept, whatever prune_interval asks
+    s, it, x = configs.go_chain(62, seed=2)
+    with Context(s, it) as c:
+        c.set_positions(x)
+        assert c.step(100) == abi.OCM_OK
+        assert c.inner_list_stats() == (0, 0)
