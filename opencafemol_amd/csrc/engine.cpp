@@ -667,9 +667,10 @@ void Engine::build_pair_blocks(int team) {
     // is laid out as 8 slices with the same mix of work: each holds an eighth
     // of the bonded-term chunks (leading: latency-bound, they start while the
     // pair rows run) and an eighth of every group's rows.  Measured on 256 Go
-    // replicas: +5-8% over the plain walk (slices of whole systems, balanced
-    // by estimated work, gained only 2% there and lost 6% on the served mix,
-    // so the gain is not per-system L2 locality alone).  A single slab, whose
+    // replicas: +5-8% over the plain walk, from L1 locality: the workgroups on
+    // one CU take adjacent entries, and L2 requests fall by 18% with misses
+    // unchanged (slices of whole systems, balanced by estimated work, gained
+    // only 2% there and lost 6% on the served mix).  A single slab, whose
     // long rows sit in the middle of the cell order, lost 6% when split, so
     // single systems and small batches keep one slice and the plain walk.
     // Cuts fall on item boundaries.
