@@ -70,6 +70,16 @@ enum ocm_force_kind {
      * params {k} ; globals {c, a1, b1, a2, b2, a3, b3, Emin}.
      * FlexibleLocalDihedralForceFieldGenerator.cpp:5-63.                    */
     OCM_FLEXIBLE_LOCAL_DIHEDRAL = 7,
+    /* Go 12-10 contact with a linear cap: with c = capping_ratio and
+     * x = r / r0, E = k (a x + b) for x <= c, else k (5 x^-12 - 6 x^-10),
+     * a = -60 (c^-13 - c^-11), b = 65 c^-12 - 66 c^-10 (value and slope
+     * continuous at x = c) ; params {k, r0, capping_ratio}.
+     * CappedGoContactForceFieldGenerator.cpp:3-30.                         */
+    OCM_CAPPED_GO_CONTACT = 8,
+    /* E = k (1 - cos(n (phi - t0))), phi the torsion angle ; params {k, t0, n}.
+     * CosineDihedralForceFieldGenerator.cpp:5-43 (its TOML reader adds pi
+     * to v0, ReadTOMLForceFieldGenerator.cpp:600-611).                       */
+    OCM_COSINE_DIHEDRAL = 9,
 
     /* Nonbonded kinds: `particle_params` holds n_particles * npp doubles.
      * Energy is zero at r >= cutoff (OpenMM CutoffNonPeriodic/Periodic).   */
@@ -87,7 +97,26 @@ enum ocm_force_kind {
     /* E = e (step(r-r*) 4 (s^12 - s^6) - step(r*-r)) * S(r), S the OpenMM
      * switching function from switch_distance to cutoff ; npp 2 {epsilon,
      * sigma}.  LennardJonesAttractiveForceFieldGenerator.cpp:38-121.        */
-    OCM_LJ_ATTRACTIVE = 19
+    OCM_LJ_ATTRACTIVE = 19,
+    /* Weeks-Chandler-Andersen: E = step(2^(1/6) sig - r) (4 e (s^12 - s^6) + e),
+     * s = sig/r, sig = (sig_i+sig_j)/2, e = sqrt(e_i e_j) ; npp 2 {sigma,
+     * epsilon}.  WeeksChandlerAndersenForceFieldGenerator.cpp:32-114.       */
+    OCM_WCA = 20,
+    /* WCA with one sigma and epsilon for every pair of its categories:
+     * E = 4 e (s^12 - s^6 + 1/4), cutoff 2^(1/6) sig ; npp 0 ;
+     * globals {sigma, epsilon}.
+     * UniformWeeksChandlerAndersenForceFieldGenerator.cpp:160-185.          */
+    OCM_UNIFORM_WCA = 21,
+    /* E = e (step(r - r*) 4 (s^12 - s^6) - step(r* - r)) S(r), r* = 2^(1/6)
+     * sig, one sigma and epsilon, S the switch from switch_distance ; npp 0 ;
+     * globals {sigma, epsilon}.
+     * UniformLennardJonesAttractiveForceFieldGenerator.cpp:143-194.         */
+    OCM_UNIFORM_LJ_ATTRACTIVE = 22,
+    /* iSoLF attraction: b = 2^(1/6) sig, E = -e (step(b - r) + step(r - b)
+     * step(b + w - r) cos^2(pi (r - b) / (2 w))), sig and w the arithmetic
+     * means, e = sqrt(e_i e_j) ; npp 3 {sigma, epsilon, omega}.
+     * iSoLFAttractiveForceFieldGenerator.cpp:41-145.                       */
+    OCM_ISOLF_ATTRACTIVE = 23
 };
 
 /* One force (one reference ForceFieldGenerator::generate() result). */
@@ -192,8 +221,18 @@ int ocm_set_velocities(ocm_context* ctx, const double* xyz, int32_t n);
  * Boltzmann velocities from Philox stream 1 keyed by (seed, bead).       */
 int ocm_set_velocities_to_temperature(ocm_context* ctx, double temperature,
                                       uint64_t seed);
-/* Integrator::step(n), src/Simulator.cpp:60. */
+/* Integrator::step(n), src/Simulator.cpp:60.
+ * Steps run in chunks (chunk_steps) that are captured as hipGraphs.  The
+ * neighbor-list ages carry over from call to call: the list is rebuilt every
+ * rebuild_interval steps of the trajectory, not at the start of each call. */
 int ocm_step(ocm_context* ctx, int64_t nsteps);
+/* Builds the neighbor lists if needed and captures the step graphs that
+ * ocm_step(ctx, nsteps) will launch from the current state, without stepping,
+ * so that call spends no time on capture or instantiation (a rollback to a
+ * shorter list interval may still capture during it).  No counterpart in the
+ * reference, where OpenMM compiles its kernels when the Context is created
+ * (src/Simulator.cpp:17).                                                  */
+int ocm_prepare_steps(ocm_context* ctx, int64_t nsteps);
 /* Context::getState(types, false, groups), src/Observer.cpp:336-344.
  * Forces and energies are restricted to the groups in `groups`.          */
 int ocm_get_state(ocm_context* ctx, int32_t what, uint32_t groups,
@@ -212,7 +251,10 @@ int ocm_minimize(ocm_context* ctx, double tolerance, int32_t max_iterations);
 
 /* Neighbor pairs (bead index pairs i<j, r < cutoff) that the device list of
  * nonbonded force `force_index` interacts through, at the current positions.
- * Writes up to `cap` pairs to out_pairs (2 ints each); *count gets the total. */
+ * The list is rebuilt there (its rows widened if they no longer fit).  Writes
+ * the first `cap` pairs in (i, j) order to out_pairs (2 ints each); *count
+ * gets the total.  No counterpart in the reference, whose lists live inside
+ * OpenMM; the diagnostic that pins the list against brute force.           */
 int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index,
                            int32_t* out_pairs, int64_t cap, int64_t* count);
 
@@ -237,6 +279,13 @@ int ocm_get_list_stats(ocm_context* ctx, int64_t* list_entries,
  * the reference, whose list lives inside OpenMM; a diagnostic.            */
 int ocm_get_inner_list_stats(ocm_context* ctx, int64_t* inner_entries,
                              int32_t* prune_interval);
+/* Exact entry counts of the current lists, summed over all pair groups:
+ * counts[0] outer entries holding a partner (every listed pair sits in both
+ * of its rows, so unique listed pairs are counts[0] / 2), counts[1] outer
+ * entries including the -1 padding, counts[2] and counts[3] the same for the
+ * inner list (0 when it is off).  Copies the lists to the host; a diagnostic
+ * with no counterpart in the reference.                                    */
+int ocm_get_list_counts(ocm_context* ctx, int64_t* counts);
 
 /* ------------------------------------------------------------------ batch */
 /* Many independent systems packed into one set of device arrays; every

@@ -25,14 +25,20 @@ HARMONIC_ANGLE = 4
 FLEXIBLE_LOCAL_ANGLE = 5
 GAUSSIAN_DIHEDRAL = 6
 FLEXIBLE_LOCAL_DIHEDRAL = 7
+CAPPED_GO_CONTACT = 8
+COSINE_DIHEDRAL = 9
 EXCLUDED_VOLUME = 16
 DEBYE_HUCKEL = 17
 LJ_REPULSIVE = 18
 LJ_ATTRACTIVE = 19
+WCA = 20
+UNIFORM_WCA = 21
+UNIFORM_LJ_ATTRACTIVE = 22
+ISOLF_ATTRACTIVE = 23
 
-BONDED_ARITY = {1: 2, 2: 2, 3: 2, 4: 3, 5: 3, 6: 4, 7: 4}
-BONDED_NPARAMS = {1: 2, 2: 3, 3: 2, 4: 2, 5: 1, 6: 3, 7: 1}
-PAIR_NPP = {16: 1, 17: 1, 18: 2, 19: 2}
+BONDED_ARITY = {1: 2, 2: 2, 3: 2, 4: 3, 5: 3, 6: 4, 7: 4, 8: 2, 9: 4}
+BONDED_NPARAMS = {1: 2, 2: 3, 3: 2, 4: 2, 5: 1, 6: 3, 7: 1, 8: 3, 9: 3}
+PAIR_NPP = {16: 1, 17: 1, 18: 2, 19: 2, 20: 2, 21: 0, 22: 0, 23: 3}
 
 STATE_POSITIONS = 1
 STATE_VELOCITIES = 2
@@ -42,9 +48,9 @@ STATE_ENERGY = 8
 # Every symbol include/ocm.h declares.
 EXPORTED = [
     "ocm_options_default", "ocm_context_create", "ocm_context_destroy", "ocm_set_positions",
-    "ocm_set_velocities", "ocm_set_velocities_to_temperature", "ocm_step", "ocm_get_state",
+    "ocm_set_velocities", "ocm_set_velocities_to_temperature", "ocm_step", "ocm_prepare_steps", "ocm_get_state",
     "ocm_get_neighbor_pairs", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
-    "ocm_get_inner_list_stats",
+    "ocm_get_inner_list_stats", "ocm_get_list_counts",
     "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
     "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
     "ocm_version", "ocm_minimize", "ocm_server_options_default", "ocm_server_create",
@@ -141,6 +147,7 @@ def load_library(path=LIB_PATH):
         "ocm_batch_set_velocities": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), C.c_int32]),
         "ocm_set_velocities_to_temperature": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64]),
         "ocm_step": (C.c_int, [C.c_void_p, C.c_int64]),
+        "ocm_prepare_steps": (C.c_int, [C.c_void_p, C.c_int64]),
         "ocm_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(State)]),
         "ocm_batch_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, P(State)]),
         "ocm_get_neighbor_pairs": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), C.c_int64, P(C.c_int64)]),
@@ -148,6 +155,7 @@ def load_library(path=LIB_PATH):
         "ocm_get_timing": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_int64)]),
         "ocm_get_list_stats": (C.c_int, [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64)]),
         "ocm_get_inner_list_stats": (C.c_int, [C.c_void_p, P(C.c_int64), P(C.c_int32)]),
+        "ocm_get_list_counts": (C.c_int, [C.c_void_p, P(C.c_int64)]),
         "ocm_batch_system_status": (C.c_int, [C.c_void_p, C.c_int32]),
         "ocm_batch_size": (C.c_int32, [C.c_void_p]),
         "ocm_last_error": (C.c_char_p, []),

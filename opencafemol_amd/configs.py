@@ -403,3 +403,92 @@ def aicg2_chain(n=56, seed=0, temperature=300.0, periodic_box=None):
     for f in aicg2.flexible_local_dihedral_forces(quad, residues, 1.0):
         sysm.add(f)
     return sysm, integ, x
+
+
+def wca_cutoff(sigmas):
+    """(largest + second largest sigma) / 2 * 2^(1/6), both tracked with <=
+    (WeeksChandlerAndersenForceFieldGenerator.cpp:49-66,101-103)."""
+    mx = second = np.finfo(float).tiny
+    for s in sigmas:
+        if mx <= s:
+            second, mx = mx, s
+        elif second <= s:
+            second = s
+    return (mx + second) * 0.5 * 2.0 ** (1.0 / 6.0)
+
+
+def lipid_patch(n_lipids=216, box=(9.0, 9.0, 9.0), seed=0, temperature=300.0, friction=0.5, dt=0.01,
+                mass=150.0, spacing=1.5):
+    """Three-bead lipids (head + two tails) in a periodic box, with the
+    membrane potentials of the iSoLF family: WCA between every bead (per-bead
+    sigma and epsilon), iSoLF attraction between tail beads, and two uniform
+    (table) potentials between named bead types: WCA between heads and tails
+    (H-T) and a switched LJ attraction among tails (T-T).  Sizes follow the
+    reference's POPC comments (ReadTOMLForceFieldGenerator.cpp:1423-1432):
+    sigma_H 4.62 A, sigma_T 7.11 A, epsilon 0.416 kcal/mol, omega 9.87 A.
+    Lipid centres sit on a jittered cubic lattice, each lipid straight along
+    a random direction, 0.55 nm between bonded beads."""
+    rng = np.random.default_rng(seed)
+    per = int(round(n_lipids ** (1.0 / 3.0)))
+    grid = np.stack(np.meshgrid(*[np.arange(per)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n_lipids]
+    centers = (grid + 0.5) * spacing + rng.uniform(-0.1, 0.1, size=grid.shape)
+    dirs = _unit(rng, n_lipids)
+    x = np.concatenate([centers[:, None, :] + dirs[:, None, :] * (np.array([-0.55, 0.0, 0.55])[None, :, None])],
+                       axis=0).reshape(-1, 3) % np.asarray(box)
+    n = 3 * n_lipids
+    is_head = np.tile([True, False, False], n_lipids)
+    sysm = SystemSpec(masses=np.full(n, mass), box=tuple(box))
+    bonds = np.concatenate([np.stack([3 * np.arange(n_lipids) + k, 3 * np.arange(n_lipids) + k + 1], 1)
+                            for k in (0, 1)]).astype(np.int32)
+    sysm.add(ForceSpec(abi.HARMONIC_BOND, "HarmonicBond", use_periodic=True, indices=bonds,
+                       params=np.tile([0.55, 2.0 * 20.0 * KJ_PER_KCAL * 100.0], (len(bonds), 1))))
+    ang = np.stack([3 * np.arange(n_lipids), 3 * np.arange(n_lipids) + 1, 3 * np.arange(n_lipids) + 2], 1).astype(np.int32)
+    sysm.add(ForceSpec(abi.HARMONIC_ANGLE, "HarmonicAngle", use_periodic=True, indices=ang,
+                       params=np.tile([np.pi, 2.0 * 2.0 * KJ_PER_KCAL], (n_lipids, 1))))
+    excl = chain_exclusions([(3 * k, 3) for k in range(n_lipids)], 1)
+    sig = np.where(is_head, 4.62215, 7.111) * NM_PER_A
+    eps = np.where(is_head, 0.30, 0.416) * KJ_PER_KCAL
+    sysm.add(ForceSpec(abi.WCA, "WeeksChandlerAndersen", use_periodic=True,
+                       particle_params=np.stack([sig, eps], 1), cutoff=wca_cutoff(sig), exclusions=excl))
+    # iSoLF attraction among tails only (heads carry no parameters)
+    om = np.full(n, 9.867 * NM_PER_A)
+    cat = np.where(is_head, -1, 0).astype(np.int32)
+    tails = ~is_head
+    isolf_rc = wca_cutoff(sig[tails]) + om[0]
+    sysm.add(ForceSpec(abi.ISOLF_ATTRACTIVE, "iSoLFAttractive", use_periodic=True,
+                       particle_params=np.stack([sig, eps, om], 1), category=cat, n_categories=1,
+                       cutoff=isolf_rc, exclusions=excl))
+    # table potentials: WCA between heads (H) and tails (T), LJ attraction T-T
+    ht = np.where(is_head, 0, 1).astype(np.int32)
+    s_ht = 5.5 * NM_PER_A
+    sysm.add(ForceSpec(abi.UNIFORM_WCA, "UniformWeeksChandlerAndersen", use_periodic=True,
+                       globals=np.array([s_ht, 0.3 * KJ_PER_KCAL]), category=ht, n_categories=2,
+                       allow=np.array([[0, 1], [1, 0]], np.uint8), cutoff=s_ht * 2.0 ** (1.0 / 6.0),
+                       exclusions=excl))
+    s_tt, ratio = 6.0 * NM_PER_A, 2.5
+    tt = np.where(is_head, -1, 0).astype(np.int32)
+    sysm.add(ForceSpec(abi.UNIFORM_LJ_ATTRACTIVE, "UniformLennardJonesAttractive", use_periodic=True,
+                       globals=np.array([s_tt, 0.2 * KJ_PER_KCAL]), category=tt, n_categories=1,
+                       cutoff=s_tt * ratio, switch_distance=0.9 * s_tt * ratio, exclusions=excl))
+    return sysm, IntegratorSpec(temperature, friction, dt, seed), x
+
+
+def capped_go_chain(n=60, seed=0, capping_ratio=0.75, temperature=300.0):
+    """A Go chain (go_chain) whose native contacts are capped 12-10 contacts
+    and whose dihedrals carry cosine terms besides the Gaussian ones, in the
+    TOML reader's conventions: k in kcal/mol, v0 + pi as t0 and integer n
+    (ReadTOMLForceFieldGenerator.cpp:226-277, :562-622)."""
+    sysm, integ, x = go_chain(n, seed=seed, temperature=temperature)
+    rng = np.random.default_rng(seed + 7)
+    for f in sysm.forces:
+        if f.kind == abi.GO_CONTACT:
+            f.kind = abi.CAPPED_GO_CONTACT
+            f.name = "CappedGoContact"
+            f.params = np.concatenate([f.params, np.full((f.n_terms, 1), capping_ratio)], axis=1)
+    quad = np.stack([np.arange(n - 3), np.arange(1, n - 2), np.arange(2, n - 1), np.arange(3, n)], 1).astype(np.int32)
+    v0 = rng.uniform(-math.pi, math.pi, n - 3)
+    ks = rng.uniform(-1.0, 1.0, n - 3) * KJ_PER_KCAL
+    ns = rng.integers(1, 4, n - 3).astype(float)
+    sysm.add(ForceSpec(abi.COSINE_DIHEDRAL, "CosineDihedral", indices=quad,
+                       params=np.stack([ks, v0 + math.pi, ns], 1)))
+    return sysm, integ, x

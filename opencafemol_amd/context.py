@@ -68,6 +68,10 @@ class Batch:
             check(code)
         return code
 
+    def prepare(self, n):
+        """Captures the graphs step(n) will launch, without stepping."""
+        check(self.lib.ocm_prepare_steps(self.h, int(n)))
+
     def status(self, s):
         return self.lib.ocm_batch_system_status(self.h, s)
 
@@ -102,6 +106,12 @@ class Batch:
         n = C.c_int64()
         check(self.lib.ocm_get_timing(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def list_counts(self):
+        """{outer_valid, outer_padded, inner_valid, inner_padded} entry counts."""
+        c = (C.c_int64 * 4)()
+        check(self.lib.ocm_get_list_counts(self.h, c))
+        return {"outer_valid": c[0], "outer_padded": c[1], "inner_valid": c[2], "inner_padded": c[3]}
 
     def list_stats(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()

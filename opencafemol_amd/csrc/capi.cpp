@@ -141,6 +141,13 @@ int ocm_set_velocities_to_temperature(ocm_context* ctx, double T, uint64_t seed)
     });
 }
 
+int ocm_prepare_steps(ocm_context* ctx, int64_t nsteps) {
+    return guarded([&] {
+        eng(ctx).prepare(nsteps);
+        return OCM_OK;
+    });
+}
+
 int ocm_step(ocm_context* ctx, int64_t nsteps) {
     return guarded([&] {
         auto& e = eng(ctx);
@@ -205,6 +212,16 @@ int ocm_get_list_stats(ocm_context* ctx, int64_t* entries, int64_t* members, int
         if (entries) *entries = a;
         if (members) *members = b;
         if (rebuilds) *rebuilds = c;
+        return OCM_OK;
+    });
+}
+
+int ocm_get_list_counts(ocm_context* ctx, int64_t* counts) {
+    return guarded([&] {
+        need(counts, "counts");
+        long long c[4];
+        eng(ctx).list_counts(c);
+        for (int k = 0; k < 4; ++k) counts[k] = c[k];
         return OCM_OK;
     });
 }

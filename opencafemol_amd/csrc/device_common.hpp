@@ -19,8 +19,10 @@ constexpr int kErrPrune = 4;     // a bead moved more than margin/2 since the in
 // Batch-wide flag word bits.
 constexpr int kFlagListOverflow = 1;
 
-// Pair-group kind bits.
+// Pair-group kind bits (bit k: nonbonded kind OCM_EXCLUDED_VOLUME + k).
 constexpr int kExv = 1, kDh = 2, kLjr = 4, kLja = 8;
+constexpr int kWca = 16, kUwca = 32, kUlja = 64, kIsolf = 128;
+constexpr int kPairKinds = 8;
 
 // Bonded term record (all kinds in one array, sorted by kind).
 struct BTerm {
@@ -73,7 +75,7 @@ struct DevPairGroup {
     int cell_begin;
     int ncell[3];
     int reach[3];          // 1 when the cell grid has >= 3 cells along an axis
-    int egroup[4];         // energy group of the EXV, DH, LJR, LJA member forces
+    int egroup[kPairKinds];   // energy group of the member force of each kind
     int n_cat;
     int stride;            // list entries reserved per row (multiple of 4)
     int lj_shared;         // LJ repulsive and attractive members share sigma for every type
@@ -92,13 +94,16 @@ struct DevPairGroup {
     double dh_K, dh_inv_lambda, dh_corr;
     double ljr_rsw, lja_rsw, exv_rsw, dh_rsw;   // switching distances (<= 0: none)
     double lja_inv_sw;     // 1 / (cutoff - lja_rsw)
+    double u_sigma, u_eps; // the uniform WCA / LJ-attractive kinds' sigma and epsilon
     uint32_t allow[32];    // category allow masks
 };
 
 // Per-type parameters: {exv_radius, dh_charge, 0, 0, ljr_eps, ljr_sigma, lja_eps, lja_sigma};
-// the four LJ values sit 16-byte aligned for one vector read.
+// the four LJ values sit 16-byte aligned for one vector read.  A group of
+// WCA or iSoLF (which holds no other kind) uses {sigma, epsilon, omega}.
 constexpr int kTypeStride = 8;
 constexpr int kTyExv = 0, kTyQ = 1, kTyLjrEps = 4, kTyLjrSig = 5, kTyLjaEps = 6, kTyLjaSig = 7;
+constexpr int kTySig = 0, kTyEps = 1, kTyOmega = 2;
 
 // ------------------------------------------------------------------ Philox
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

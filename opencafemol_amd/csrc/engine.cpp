@@ -20,13 +20,13 @@ constexpr long long kPruneMinEntries = 1 << 14;
 constexpr int kXcdSliceSystems = 16;
 
 // ------------------------------------------------------------ kind tables
-bool is_bonded(int k) { return k >= OCM_HARMONIC_BOND && k <= OCM_FLEXIBLE_LOCAL_DIHEDRAL; }
-bool is_pair(int k) { return k >= OCM_EXCLUDED_VOLUME && k <= OCM_LJ_ATTRACTIVE; }
+bool is_bonded(int k) { return k >= OCM_HARMONIC_BOND && k <= OCM_COSINE_DIHEDRAL; }
+bool is_pair(int k) { return k >= OCM_EXCLUDED_VOLUME && k <= OCM_ISOLF_ATTRACTIVE; }
 int bonded_arity(int k) {
     switch (k) {
-    case OCM_HARMONIC_BOND: case OCM_GAUSSIAN_BOND: case OCM_GO_CONTACT: return 2;
+    case OCM_HARMONIC_BOND: case OCM_GAUSSIAN_BOND: case OCM_GO_CONTACT: case OCM_CAPPED_GO_CONTACT: return 2;
     case OCM_HARMONIC_ANGLE: case OCM_FLEXIBLE_LOCAL_ANGLE: return 3;
-    case OCM_GAUSSIAN_DIHEDRAL: case OCM_FLEXIBLE_LOCAL_DIHEDRAL: return 4;
+    case OCM_GAUSSIAN_DIHEDRAL: case OCM_FLEXIBLE_LOCAL_DIHEDRAL: case OCM_COSINE_DIHEDRAL: return 4;
     default: return 0;
     }
 }
@@ -39,6 +39,8 @@ int bonded_nparams(int k) {
     case OCM_FLEXIBLE_LOCAL_ANGLE: return 1;
     case OCM_GAUSSIAN_DIHEDRAL: return 3;
     case OCM_FLEXIBLE_LOCAL_DIHEDRAL: return 1;
+    case OCM_CAPPED_GO_CONTACT: return 3;
+    case OCM_COSINE_DIHEDRAL: return 3;
     default: return 0;
     }
 }
@@ -51,16 +53,23 @@ int pair_npp(int k) {
     switch (k) {
     case OCM_EXCLUDED_VOLUME: return 1;
     case OCM_DEBYE_HUCKEL: return 1;
-    case OCM_LJ_REPULSIVE: case OCM_LJ_ATTRACTIVE: return 2;
-    default: return 0;
+    case OCM_LJ_REPULSIVE: case OCM_LJ_ATTRACTIVE: case OCM_WCA: return 2;
+    case OCM_ISOLF_ATTRACTIVE: return 3;
+    default: return 0;   // the uniform kinds carry their parameters as globals
     }
 }
 static int pair_nglobals(int k) {
     if (k == OCM_EXCLUDED_VOLUME) return 1;
     if (k == OCM_DEBYE_HUCKEL) return 3;
+    if (k == OCM_UNIFORM_WCA || k == OCM_UNIFORM_LJ_ATTRACTIVE) return 2;
     return 0;
 }
+// The four kinds of round shape (EXV, DH, LJ repulsive and attractive) merge
+// into one pair group when their lists agree; the others each get a group of
+// their own (their per-type parameters reuse the first type-row slots).
+static bool solo_pair_kind(int k) { return k >= OCM_WCA; }
 static int pair_bit(int k) { return 1 << (k - OCM_EXCLUDED_VOLUME); }
+static_assert(OCM_ISOLF_ATTRACTIVE - OCM_EXCLUDED_VOLUME < kPairKinds, "pair kinds fit the kind bits");
 
 // Natural cubic spline second derivatives on uniform knots (the fit OpenMM's
 // Continuous1DFunction gives the FLA y-table).
@@ -128,7 +137,7 @@ HostSystem copy_system(const ocm_system_desc& d, const ocm_integrator_desc& it) 
                 if (!std::isfinite(g)) invalid("force global is not finite");
         } else if (is_pair(f.kind)) {
             const int npp = pair_npp(f.kind), ng = pair_nglobals(f.kind);
-            if (!f.particle_params) invalid("nonbonded force needs particle_params");
+            if (npp > 0 && !f.particle_params) invalid("nonbonded force needs particle_params");
             if (f.n_globals != ng || (ng > 0 && !f.globals)) {
                 std::ostringstream o;
                 o << "force kind " << f.kind << " needs " << ng << " globals, got " << f.n_globals;
@@ -137,10 +146,10 @@ HostSystem copy_system(const ocm_system_desc& d, const ocm_integrator_desc& it) 
             if (!(f.cutoff > 0.0)) invalid("nonbonded cutoff must be > 0");
             h.cutoff = f.cutoff;
             h.switch_distance = f.switch_distance > 0 ? f.switch_distance : 0.0;
-            if (h.switch_distance > 0 && f.kind != OCM_LJ_ATTRACTIVE)
-                invalid("switching is supported for LennardJonesAttractive only");
+            if (h.switch_distance > 0 && f.kind != OCM_LJ_ATTRACTIVE && f.kind != OCM_UNIFORM_LJ_ATTRACTIVE)
+                invalid("switching is supported for the Lennard-Jones attractive kinds only");
             if (h.switch_distance >= h.cutoff) invalid("switch distance must be below the cutoff");
-            h.particle_params.assign(f.particle_params, f.particle_params + static_cast<size_t>(npp) * s.n);
+            if (npp > 0) h.particle_params.assign(f.particle_params, f.particle_params + static_cast<size_t>(npp) * s.n);
             if (ng) h.globals.assign(f.globals, f.globals + ng);
             if (f.category) {
                 if (f.n_categories < 1 || f.n_categories > 32) invalid("n_categories must be in 1..32");
@@ -233,15 +242,19 @@ Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(s
 }
 
 Engine::~Engine() {
+    // nothing may still run on the stream when its graphs and buffers go
+    if (stream_) (void)hipStreamSynchronize(stream_);
     drop_graphs();
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
-    if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocs_) (void)hipFree(p);
+    if (status_host_) (void)hipHostFree(status_host_);
+    if (stamp_host_) (void)hipHostFree(stamp_host_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+// Callers guarantee the stream is idle (every chunk ends with a sync).
 void Engine::drop_graphs() {
-    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second.exec);
     graphs_.clear();
 }
 
@@ -358,6 +371,7 @@ void Engine::pack() {
             bool placed = false;
             for (auto& g : merged) {
                 const HostForce& h = H.forces[g[0]];
+                if (solo_pair_kind(f.kind) || solo_pair_kind(h.kind)) continue;
                 bool kind_taken = false;
                 for (int o : g) kind_taken |= H.forces[o].kind == f.kind;
                 if (!kind_taken && same_list(h, f)) { g.push_back(fi); placed = true; break; }
@@ -398,8 +412,8 @@ void Engine::pack() {
                 G.box[c] = H.box[c];
                 G.inv_box[c] = G.periodic ? 1.0 / H.box[c] : 0.0;
             }
-            G.egroup[0] = G.egroup[1] = G.egroup[2] = G.egroup[3] = 0;
-            const HostForce* byk[4] = {nullptr, nullptr, nullptr, nullptr};
+            for (int k = 0; k < kPairKinds; ++k) G.egroup[k] = 0;
+            const HostForce* byk[kPairKinds] = {};
             for (int fi : g) {
                 const HostForce& f = H.forces[fi];
                 const int k = f.kind - OCM_EXCLUDED_VOLUME;
@@ -418,10 +432,21 @@ void Engine::pack() {
                 G.dh_inv_lambda = 1.0 / byk[1]->globals[1];
                 G.dh_corr = byk[1]->globals[2];
             }
-            if (byk[3]) {
-                G.lja_rsw = byk[3]->switch_distance;
+            // the attractive LJ kinds' switch (a group holds at most one of them)
+            for (const HostForce* lja : {byk[3], byk[OCM_UNIFORM_LJ_ATTRACTIVE - OCM_EXCLUDED_VOLUME]}) {
+                if (!lja) continue;
+                G.lja_rsw = lja->switch_distance;
                 if (G.lja_rsw > 0) G.lja_inv_sw = 1.0 / (f0.cutoff - G.lja_rsw);
             }
+            for (int k : {OCM_UNIFORM_WCA, OCM_UNIFORM_LJ_ATTRACTIVE}) {
+                const HostForce* u = byk[k - OCM_EXCLUDED_VOLUME];
+                if (!u) continue;
+                G.u_sigma = u->globals[0];
+                G.u_eps = u->globals[1];
+                if (!(G.u_sigma > 0)) invalid("uniform sigma must be > 0");
+            }
+            const bool solo = solo_pair_kind(f0.kind);
+            const int npp0 = pair_npp(f0.kind);
             // categories
             G.n_cat = f0.n_categories;
             for (int a = 0; a < G.n_cat; ++a) {
@@ -439,22 +464,41 @@ void Engine::pack() {
                 const int cat = f0.category.empty() ? 0 : f0.category[i];
                 if (cat < 0) continue;
                 std::array<double, 6> tup{0, 0, 0, 0, 0, 0};
-                if (byk[0]) tup[0] = byk[0]->particle_params[i];
-                if (byk[1]) tup[1] = byk[1]->particle_params[i];
-                if (byk[2]) { tup[2] = byk[2]->particle_params[2 * i]; tup[3] = byk[2]->particle_params[2 * i + 1]; }
-                if (byk[3]) { tup[4] = byk[3]->particle_params[2 * i]; tup[5] = byk[3]->particle_params[2 * i + 1]; }
+                if (solo) {
+                    // WCA {sigma, epsilon}, iSoLF {sigma, epsilon, omega}; the
+                    // uniform kinds have one type
+                    for (int k = 0; k < npp0; ++k) tup[k] = f0.particle_params[static_cast<size_t>(npp0) * i + k];
+                } else {
+                    if (byk[0]) tup[0] = byk[0]->particle_params[i];
+                    if (byk[1]) tup[1] = byk[1]->particle_params[i];
+                    if (byk[2]) { tup[2] = byk[2]->particle_params[2 * i]; tup[3] = byk[2]->particle_params[2 * i + 1]; }
+                    if (byk[3]) { tup[4] = byk[3]->particle_params[2 * i]; tup[5] = byk[3]->particle_params[2 * i + 1]; }
+                }
                 auto it = tmap.find(tup);
                 int ty;
                 if (it == tmap.end()) {
                     ty = static_cast<int>(tmap.size());
                     tmap.emplace(tup, ty);
-                    const double row[kTypeStride] = {tup[0], tup[1], 0.0, 0.0, tup[2], tup[3], tup[4], tup[5]};
-                    for (int k = 0; k < kTypeStride; ++k) {
-                        types_d.push_back(row[k]);
-                        // the force pass's FP32 rows hold half of each LJ epsilon and
-                        // sigma (its mixing rules are then single adds)
-                        const float v = static_cast<float>(row[k]);
-                        types_f.push_back(k >= kTyLjrEps ? 0.5f * v : v);
+                    if (solo) {
+                        // FP64 rows {sigma, epsilon, omega}; the force pass's FP32
+                        // rows {sigma/2, sqrt(epsilon), omega/2}: the arithmetic
+                        // means are then single adds and the geometric mean a product
+                        const double row[kTypeStride] = {tup[0], tup[1], tup[2], 0, 0, 0, 0, 0};
+                        for (int k = 0; k < kTypeStride; ++k) types_d.push_back(row[k]);
+                        if (tup[1] < 0) invalid("WCA / iSoLF epsilon must be >= 0");
+                        const float rowf[kTypeStride] = {0.5f * static_cast<float>(tup[0]),
+                                                         static_cast<float>(std::sqrt(tup[1])),
+                                                         0.5f * static_cast<float>(tup[2]), 0.f, 0.f, 0.f, 0.f, 0.f};
+                        for (int k = 0; k < kTypeStride; ++k) types_f.push_back(rowf[k]);
+                    } else {
+                        const double row[kTypeStride] = {tup[0], tup[1], 0.0, 0.0, tup[2], tup[3], tup[4], tup[5]};
+                        for (int k = 0; k < kTypeStride; ++k) {
+                            types_d.push_back(row[k]);
+                            // the force pass's FP32 rows hold half of each LJ epsilon and
+                            // sigma (its mixing rules are then single adds)
+                            const float v = static_cast<float>(row[k]);
+                            types_f.push_back(k >= kTyLjrEps ? 0.5f * v : v);
+                        }
                     }
                 } else {
                     ty = it->second;
@@ -555,10 +599,16 @@ void Engine::pack() {
     check_hip(hipMemset(pos, 0, sizeof(double) * n3), "memset");
     check_hip(hipMemset(vel, 0, sizeof(double) * n3), "memset");
     check_hip(hipMemset(frc, 0, sizeof(long long) * n3), "memset");
-    bx_ = dalloc<double>(n3);
-    bv_ = dalloc<double>(n3);
-    bf_ = dalloc<long long>(n3);
-    bstep_ = dalloc<long long>(2);
+    bstate_ = dalloc<double>(3 * static_cast<size_t>(n3) + 2);
+    {
+        // pinned, device-mapped and coherent: the chunk's last kernel writes
+        // the status words straight into it
+        void* h = nullptr;
+        check_hip(hipHostMalloc(&h, sizeof(int) * (1 + static_cast<size_t>(nsys)),
+                                hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
+        status_host_ = static_cast<int*>(h);
+        std::memset(status_host_, 0, sizeof(int) * (1 + static_cast<size_t>(nsys)));
+    }
     d_.sys = dalloc<DevSys>(nsys);
     upload(d_.sys, dsys);
     d_.sys0 = dsys[0];
@@ -717,7 +767,10 @@ void Engine::set_timing(int mode) {
     timing_mode_ = mode;
     timing_ = mode == 1;
     if (mode == 2 && wall_khz_ == 0) {
-        kstamp_red_ = dalloc<unsigned long long>(2 * static_cast<size_t>(opt_.chunk_steps));
+        void* h = nullptr;
+        check_hip(hipHostMalloc(&h, 2 * sizeof(unsigned long long) * static_cast<size_t>(opt_.chunk_steps),
+                                hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
+        stamp_host_ = static_cast<unsigned long long*>(h);
         int khz = 0;
         check_hip(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, opt_.device), "wall clock rate");
         if (khz <= 0) throw Error(OCM_ERR_HIP, "device wall clock rate unknown");
@@ -771,14 +824,46 @@ void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
     timed(0, [&] { launch_pair(d_, stream_, false, f, groups, kListPrune); });
 }
 
-// Step s of the chunk rebuilds the outer list when s % interval == 0 and
-// prunes the inner list when (s % interval) % prune == 0 (so at every
-// rebuild, and at the chunk's first step); the other steps read the inner list.
-void Engine::enqueue_chunk(int k, int interval, int prune) {
+// The list schedule of a chunk of k steps.  Ages count integration steps:
+// after step s the outer list is rebuilt when it is due (age >= interval, or
+// the lists are stale) and the inner list pruned at every rebuild and when
+// its own age reaches the prune interval; other steps read the inner list.
+// Ages carry over from chunk to chunk, so the list lifetime does not depend
+// on how steps are grouped into calls.
+Engine::ChunkPlan Engine::plan_chunk(int k) const {
+    ChunkPlan p;
+    p.k = k;
+    p.interval = interval_;
+    p.prune = prune_ > 1 ? prune_ : 0;
+    p.fresh = fresh_;
+    p.age0 = fresh_ ? 0 : age_;
+    p.page0 = fresh_ || !p.prune ? 0 : page_;
+    p.rebuild.assign(k, 0);
+    p.pruned.assign(k, 0);
+    int age = p.age0, page = p.page0;
     for (int s = 0; s < k; ++s) {
-        const bool rebuild = s % interval == 0;
-        const bool pruned = prune > 1 && (s % interval) % prune == 0;
-        const int pr = prune > 1 ? (pruned ? kPruneWrite : kPruneCheck) : kPruneNone;
+        ++age;
+        ++page;
+        const bool rb = (s == 0 && p.fresh) || age >= p.interval;
+        if (rb) { age = 0; ++p.rebuilds; }
+        const bool pr = p.prune > 1 && (rb || page >= p.prune);
+        if (pr) page = 0;
+        p.rebuild[s] = rb;
+        p.pruned[s] = pr;
+    }
+    p.age1 = age;
+    p.page1 = page;
+    return p;
+}
+
+// One chunk: the snapshot a rollback restores, the steps, the closing kick,
+// and the status words into pinned host memory.
+void Engine::enqueue_chunk(const ChunkPlan& p) {
+    launch_snapshot(d_, stream_, bstate_);
+    for (int s = 0; s < p.k; ++s) {
+        const bool rebuild = p.rebuild[s] != 0;
+        const bool pruned = p.pruned[s] != 0;
+        const int pr = p.prune > 1 ? (pruned ? kPruneWrite : kPruneCheck) : kPruneNone;
         // the integrator refreshes the pair records itself on non-rebuild steps
         timed(2, [&] { launch_integrate(d_, stream_, s & 1, (s + 1) & 1, s > 0 ? 1 : 0, rebuild ? 0 : 1, 1, pr); });
         if (d_.nm > 0 && rebuild) timed(3, [&] { launch_rebuild(d_, stream_); });
@@ -786,26 +871,23 @@ void Engine::enqueue_chunk(int k, int interval, int prune) {
             timing_mode_ == 2 ? kstamp_ + 2 * static_cast<size_t>(kstamp_grid_) * s : nullptr;
         // bonded terms run inside the pair kernel's force pass
         if (pruned) timed(6, [&] { launch_prune(d_, stream_); });
-        const int list = prune > 1 ? kListInner : kListOuter;
+        const int list = p.prune > 1 ? kListInner : kListOuter;
         timed(0, [&] { launch_pair(d_, stream_, false, d_.f, 0xFFFFFFFFu, list, stamp); });
     }
-    timed(2, [&] { launch_half_kick(d_, stream_, k & 1); });
-}
-
-void Engine::snapshot() {
-    const size_t n3 = 3 * static_cast<size_t>(d_.nb);
-    check_hip(hipMemcpyAsync(bx_, d_.px, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "snapshot");
-    check_hip(hipMemcpyAsync(bv_, d_.vx, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "snapshot");
-    check_hip(hipMemcpyAsync(bf_, d_.f.fx, n3 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "snapshot");
-    check_hip(hipMemcpyAsync(bstep_, d_.gstep, 2 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "snapshot");
+    timed(2, [&] { launch_half_kick(d_, stream_, p.k & 1); });
+    if (timing_mode_ == 2) launch_reduce_stamps(stream_, kstamp_, p.k, kstamp_grid_, stamp_host_);
+    launch_status_out(d_, stream_, status_host_);
 }
 
 void Engine::restore() {
     const size_t n3 = 3 * static_cast<size_t>(d_.nb);
-    check_hip(hipMemcpyAsync(d_.px, bx_, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "restore");
-    check_hip(hipMemcpyAsync(d_.vx, bv_, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "restore");
-    check_hip(hipMemcpyAsync(d_.f.fx, bf_, n3 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
-    check_hip(hipMemcpyAsync(d_.gstep, bstep_, 2 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
+    const long long* b = reinterpret_cast<const long long*>(bstate_);
+    check_hip(hipMemcpyAsync(d_.px, bstate_, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "restore");
+    check_hip(hipMemcpyAsync(d_.vx, bstate_ + n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_), "restore");
+    check_hip(hipMemcpyAsync(d_.f.fx, b + 2 * n3, n3 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
+    check_hip(hipMemcpyAsync(d_.gstep, b + 3 * n3, 2 * sizeof(long long), hipMemcpyDeviceToDevice, stream_), "restore");
+    // the lists were built for states past the restored one
+    stale_lists();
 }
 
 // Row r of group g owns list slots [row_start[r], row_start[r] + stride_g);
@@ -891,6 +973,9 @@ void Engine::ensure_forces() {
             continue;
         }
         forces_dirty_ = false;
+        // built and pruned outside a chunk (the prune positions are not
+        // recorded there): the next chunk starts with a rebuild
+        stale_lists();
         if (!team_tuned_) {
             autotune_team();   // clears the flags its scratch launches raised
             launch_zero_forces(d_, d_.f, stream_);
@@ -960,43 +1045,72 @@ void Engine::autotune_team() {
     sync();
 }
 
+// The captured graph of a chunk plan, or nullptr when it runs eagerly.
+// Graphs are kept per plan key (length, intervals, starting ages, stamping),
+// up to kMaxGraphs; past that the least recently used one is dropped.
+// Timing mode 1 (events around every launch) runs eagerly.
+hipGraphExec_t Engine::chunk_graph(const ChunkPlan& p, bool may_capture) {
+    constexpr std::size_t kMaxGraphs = 48;
+    if (!opt_.use_graph || timing_) return nullptr;
+    const auto key = p.key(timing_mode_ == 2 ? 1 : 0);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        if (!may_capture) return nullptr;
+        if (graphs_.size() >= kMaxGraphs) {
+            auto old = graphs_.begin();
+            for (auto g = graphs_.begin(); g != graphs_.end(); ++g)
+                if (g->second.used < old->second.used) old = g;
+            (void)hipGraphExecDestroy(old->second.exec);   // the stream is idle between chunks
+            graphs_.erase(old);
+        }
+        hipGraph_t g;
+        check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+        enqueue_chunk(p);
+        check_hip(hipStreamEndCapture(stream_, &g), "capture");
+        hipGraphExec_t ge;
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check_hip(e, "instantiate");
+        it = graphs_.emplace(key, GraphEntry{ge, 0}).first;
+    }
+    it->second.used = ++graph_clock_;
+    return it->second.exec;
+}
+
+void Engine::prepare(long long n) {
+    if (n < 0) invalid("negative step count");
+    if (n == 0) return;
+    ensure_forces();
+    if (timing_mode_ == 2) ensure_stamps();
+    // walk the chunks step(n) would run, with the ages they would leave
+    const int age = age_, page = page_, fresh = fresh_;
+    long long remaining = n;
+    while (remaining > 0) {
+        const int k = static_cast<int>(std::min<long long>(remaining, chunk_));
+        const ChunkPlan p = plan_chunk(k);
+        (void)chunk_graph(p, true);
+        age_ = p.age1;
+        page_ = p.page1;
+        fresh_ = 0;
+        remaining -= k;
+    }
+    age_ = age;
+    page_ = page;
+    fresh_ = fresh;
+}
+
 int Engine::run_chunk(int k) {
     for (int attempt = 0; attempt < 16; ++attempt) {
-        snapshot();
-        check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
-        const int interval = std::min(interval_, k);
         if (timing_mode_ == 2) ensure_stamps();
-        // Stretch lengths set by callers vary; graphs are kept for at most
-        // kMaxGraphs shapes besides the full chunk, other shapes run eagerly.
-        // Graphs with pair-kernel stamps are keyed apart (negative interval).
-        constexpr std::size_t kMaxGraphs = 8;
-        const int prune = prune_ > 1 ? prune_ : 0;
-        const auto key = std::make_tuple(k, timing_mode_ == 2 ? -interval : interval, prune);
-        const bool graph = opt_.use_graph && !timing_ &&
-                           (k == chunk_ || graphs_.count(key) || graphs_.size() < kMaxGraphs);
-        if (graph) {
-            auto it = graphs_.find(key);
-            if (it == graphs_.end()) {
-                hipGraph_t g;
-                check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
-                enqueue_chunk(k, interval, prune);
-                check_hip(hipStreamEndCapture(stream_, &g), "capture");
-                hipGraphExec_t ge;
-                check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
-                (void)hipGraphDestroy(g);
-                it = graphs_.emplace(key, ge).first;
-            }
-            check_hip(hipGraphLaunch(it->second, stream_), "hipGraphLaunch");
-        } else {
-            enqueue_chunk(k, interval, prune);
-        }
-        if (d_.nm > 0) rebuilds_ += (k + interval - 1) / interval;
+        const ChunkPlan p = plan_chunk(k);
+        hipGraphExec_t ge = chunk_graph(p, true);
+        if (ge) check_hip(hipGraphLaunch(ge, stream_), "hipGraphLaunch");
+        else enqueue_chunk(p);
+        rebuilds_ += p.rebuilds;
         sync();
-        int flags = 0;
-        std::vector<int> err(d_.nsys);
-        check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
-        check_hip(hipMemcpy(err.data(), d_.err, sizeof(int) * d_.nsys, hipMemcpyDeviceToHost), "err");
-        check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
+        // the chunk's last kernel left the flag word and the error words here
+        const int flags = status_host_[0];
+        const int* err = status_host_ + 1;
         if (flags & kFlagListOverflow) {
             restore();
             grow_rows();
@@ -1008,39 +1122,38 @@ int Engine::run_chunk(int k) {
             if (err[s] & kErrStale) stale = true;
             if (err[s] & kErrPrune) pruned_stale = true;
         }
-        if (stale && interval > 1) {
+        if (stale && p.interval > 1) {
             // a bead moved more than skin/2 since the list was built: redo the
             // chunk with half the interval, and wait twice as long as last time
             // before trying a longer one again
             restore();
-            interval_ = std::max(1, interval / 2);
+            interval_ = std::max(1, p.interval / 2);
             clean_chunks_ = 0;
             recover_after_ = std::min(recover_after_ * 2, 64);
             continue;
         }
-        if (pruned_stale && prune > 1) {
+        if (pruned_stale && p.prune > 1) {
             // the same for the inner list: a bead moved more than margin/2
             // since the prune; below 2 the inner list is not used
             restore();
-            prune_ = prune / 2 > 1 ? prune / 2 : 1;
+            prune_ = p.prune / 2 > 1 ? p.prune / 2 : 1;
             prune_clean_ = 0;
             prune_recover_after_ = std::min(prune_recover_after_ * 2, 64);
             continue;
         }
         // accepted: report this chunk's flags and those carried from before it
+        age_ = p.age1;
+        page_ = p.page1;
+        fresh_ = 0;
         for (int s = 0; s < d_.nsys; ++s) {
             if ((err[s] | err_carry_[s]) & kErrNumeric) { sys_status_[s] = OCM_ERR_NUMERIC; numeric = true; }
             err_carry_[s] = 0;
         }
         if (timing_mode_ == 2) {
-            launch_reduce_stamps(stream_, kstamp_, k, kstamp_grid_, kstamp_red_);
-            sync();
-            std::vector<unsigned long long> st(2 * static_cast<size_t>(k));
-            check_hip(hipMemcpy(st.data(), kstamp_red_, st.size() * sizeof(unsigned long long),
-                                hipMemcpyDeviceToHost), "stamps");
             for (int q = 0; q < k; ++q) {
-                if (st[2 * q] == ~0ull || st[2 * q + 1] < st[2 * q]) continue;  // no block ran
-                timing_slots_[5].ms += static_cast<double>(st[2 * q + 1] - st[2 * q]) / wall_khz_;
+                const unsigned long long a = stamp_host_[2 * q], b = stamp_host_[2 * q + 1];
+                if (a == ~0ull || b < a) continue;  // no block ran
+                timing_slots_[5].ms += static_cast<double>(b - a) / wall_khz_;
                 timing_slots_[5].launches += 1;
             }
         }
@@ -1311,7 +1424,7 @@ void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
     }
     if (what & OCM_STATE_ENERGY) {
         launch_zero_energy(d_, stream_);
-        if (d_.nm > 0) { launch_rebuild(d_, stream_); ++rebuilds_; }
+        if (d_.nm > 0) { launch_rebuild(d_, stream_); ++rebuilds_; stale_lists(); }
         launch_bonded(d_, stream_, true, d_.f2, groups);
         launch_pair(d_, stream_, true, d_.f2, groups);
         launch_kinetic(d_, stream_);
@@ -1338,31 +1451,60 @@ void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
     }
 }
 
-long long Engine::neighbor_pairs(int s, int force_index, int32_t* out, long long cap) {
+// Pair group of nonbonded force `force_index` of system s.
+int Engine::pair_group_of(int s, int force_index) const {
     const HostSystem& H = sys_.at(s);
     if (force_index < 0 || force_index >= static_cast<int>(H.forces.size())) invalid("force index out of range");
     const int g = force_group_of_[s][force_index];
     if (g < 0) invalid("force is not a nonbonded force");
-    ensure_forces();
-    check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
+    return g;
+}
+
+// Rebuilds the outer list at the current positions; returns the flag word.
+int Engine::rebuild_now() {
+    launch_fill_i32(stream_, d_.flags, 0, 1);
     launch_rebuild(d_, stream_);
     ++rebuilds_;
-    long long want = std::max<long long>(cap, 1);
-    int2* buf = dalloc<int2>(static_cast<size_t>(want));
-    d_.dump_pairs = buf;
-    launch_neighbor_dump(d_, stream_, g, want, group_member_count_[g]);
+    stale_lists();
     sync();
+    int flags = 0;
+    check_hip(hipMemcpy(&flags, d_.flags, sizeof(int), hipMemcpyDeviceToHost), "flags");
+    return flags;
+}
+
+long long Engine::neighbor_pairs(int s, int force_index, int32_t* out, long long cap) {
+    const int g = pair_group_of(s, force_index);
+    ensure_forces();
+    // the list at the current positions, widened if a row no longer fits
+    for (int attempt = 0;; ++attempt) {
+        if (!(rebuild_now() & kFlagListOverflow)) break;
+        if (attempt == 8) throw Error(OCM_ERR_OOM, "neighbor list did not fit after repeated growth");
+        grow_rows();
+    }
+    // count, then dump every pair, so the first `cap` of the sorted set are returned
+    long long want = std::max<long long>(cap, 1024);
+    std::vector<int2> host;
     int cnt = 0;
-    check_hip(hipMemcpy(&cnt, d_.dump_count, sizeof(int), hipMemcpyDeviceToHost), "dump");
-    const long long ncopy = std::min<long long>(cnt, cap);
-    std::vector<int2> host(static_cast<size_t>(std::max<long long>(ncopy, 1)));
-    if (ncopy > 0) check_hip(hipMemcpy(host.data(), buf, ncopy * sizeof(int2), hipMemcpyDeviceToHost), "dump");
-    auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(buf));
-    if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
-    d_.dump_pairs = nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+        int2* buf = dalloc<int2>(static_cast<size_t>(want));
+        launch_neighbor_dump(d_, stream_, g, want, group_member_count_[g], d_.dump_count, buf);
+        sync();
+        check_hip(hipMemcpy(&cnt, d_.dump_count, sizeof(int), hipMemcpyDeviceToHost), "dump");
+        const bool fits = cnt <= want;
+        if (fits) {
+            host.resize(static_cast<size_t>(cnt));
+            if (cnt > 0) check_hip(hipMemcpy(host.data(), buf, cnt * sizeof(int2), hipMemcpyDeviceToHost), "dump");
+        }
+        auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void*>(buf));
+        if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+        if (fits) break;
+        want = cnt;
+    }
     std::vector<std::pair<int, int>> v;
-    for (long long k = 0; k < ncopy; ++k) v.emplace_back(host[k].x - bead_begin_[s], host[k].y - bead_begin_[s]);
+    v.reserve(host.size());
+    for (const int2& p : host) v.emplace_back(p.x - bead_begin_[s], p.y - bead_begin_[s]);
     std::sort(v.begin(), v.end());
+    const long long ncopy = std::min<long long>(cnt, cap);
     for (long long k = 0; k < ncopy; ++k) { out[2 * k] = v[k].first; out[2 * k + 1] = v[k].second; }
     return cnt;
 }
@@ -1377,6 +1519,28 @@ void Engine::inner_stats(long long* entries, int* prune) {
     }
     *entries = total;
     *prune = d_.col_in ? prune_ : 0;
+}
+
+// Exact counts of both lists: entries that hold a partner (each pair appears
+// once in each of its two rows) and entries including the -1 padding.
+void Engine::list_counts(long long out[4]) {
+    sync();
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    if (d_.nm == 0) return;
+    std::vector<int> start(d_.nm), cnt(d_.nm);
+    std::vector<int> col(static_cast<size_t>(d_.list_capacity));
+    check_hip(hipMemcpy(start.data(), d_.row_start, sizeof(int) * d_.nm, hipMemcpyDeviceToHost), "rows");
+    for (int inner = 0; inner < 2; ++inner) {
+        const int* rc = inner ? d_.row_count_in : d_.row_count;
+        const int* cl = inner ? d_.col_in : d_.col;
+        if (!rc || !cl) continue;
+        check_hip(hipMemcpy(cnt.data(), rc, sizeof(int) * d_.nm, hipMemcpyDeviceToHost), "rows");
+        check_hip(hipMemcpy(col.data(), cl, sizeof(int) * col.size(), hipMemcpyDeviceToHost), "list");
+        for (int r = 0; r < d_.nm; ++r) {
+            out[2 * inner + 1] += cnt[r];
+            for (int e = 0; e < cnt[r]; ++e) out[2 * inner] += col[static_cast<size_t>(start[r]) + e] >= 0;
+        }
+    }
 }
 
 void Engine::list_stats(long long* entries, long long* members, long long* rebuilds) {

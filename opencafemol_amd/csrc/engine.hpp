@@ -72,6 +72,9 @@ public:
     // Advances every system by n steps.  Returns OCM_OK or OCM_ERR_NUMERIC
     // (some system produced NaN/overflow; see system_status).
     int step(long long n);
+    // Captures the step graphs that step(n) will launch from the current
+    // state (assuming no rollback), so that call spends no time on capture.
+    void prepare(long long n);
     // Capped steepest descent until the RMS force component of every system
     // is <= tol or max_iter iterations ran.  Returns iterations done.
     int minimize(double tol, int max_iter);
@@ -95,18 +98,33 @@ public:
     TimingSlot timing(int k) const { return timing_slots_.at(k); }
     void list_stats(long long* entries, long long* members, long long* rebuilds);
     void inner_stats(long long* entries, int* prune);
+    void list_counts(long long out[4]);
 
 private:
     void pack();
     void alloc_device();
     void build_pair_blocks(int team);
     void ensure_forces();
+    int pair_group_of(int s, int force_index) const;
+    int rebuild_now();
     void carry_errors();
     void enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups);
-    void enqueue_chunk(int k, int interval, int prune);
+    // List schedule of one chunk: which steps rebuild the outer list and which
+    // prune the inner one, from the list ages carried in from earlier chunks.
+    struct ChunkPlan {
+        int k = 0, interval = 0, prune = 0;
+        int age0 = 0, page0 = 0, fresh = 0;   // ages at the chunk start; fresh: rebuild at its first step
+        int age1 = 0, page1 = 0;              // ages after the chunk
+        int rebuilds = 0;
+        std::vector<uint8_t> rebuild, pruned;
+        std::array<int, 7> key(int stamps) const { return {k, interval, prune, age0, page0, fresh, stamps}; }
+    };
+    ChunkPlan plan_chunk(int k) const;
+    void enqueue_chunk(const ChunkPlan& p);
+    hipGraphExec_t chunk_graph(const ChunkPlan& p, bool may_capture);
     int run_chunk(int k);
-    void snapshot();
     void restore();
+    void stale_lists() { fresh_ = 1; }   // the next chunk rebuilds and prunes at its first step
     void check_hip(hipError_t e, const char* what);
     void sync();
     void layout_rows();
@@ -148,7 +166,6 @@ private:
     std::array<TimingSlot, 7> timing_slots_{};
     int timing_mode_ = 0;
     unsigned long long* kstamp_ = nullptr;   // 2 words per pair workgroup per step of a chunk
-    unsigned long long* kstamp_red_ = nullptr;  // 2 words per step of a chunk
     long long kstamp_words_ = 0;
     int kstamp_grid_ = 0;
     void ensure_stamps();
@@ -158,12 +175,26 @@ private:
     size_t event_next_ = 0;
     bool team_tuned_ = false;
 
-    // snapshot buffers
-    double *bx_ = nullptr, *bv_ = nullptr;
-    long long* bf_ = nullptr;
-    long long* bstep_ = nullptr;
+    // snapshot buffers: the chunk-start state (positions, velocities, forces
+    // and the two step slots), written by the chunk's first kernel
+    double* bstate_ = nullptr;
+    // pinned host words the chunk's last kernel writes: the batch flag word,
+    // then one error word per system (cleared on the device as they are copied)
+    int* status_host_ = nullptr;
+    unsigned long long* stamp_host_ = nullptr;   // 2 words per step of a chunk (timing mode 2)
 
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_;   // (steps, interval, prune)
+    // ages of the lists, carried from chunk to chunk: steps since the outer
+    // list was built and since the inner list was pruned; fresh_ = 1 when the
+    // lists no longer match the state (after a rollback or an evaluation
+    // outside a chunk), so the next chunk rebuilds at its first step
+    int age_ = 0, page_ = 0, fresh_ = 1;
+    // captured chunks, least recently used first out when over kMaxGraphs
+    struct GraphEntry {
+        hipGraphExec_t exec;
+        long long used;
+    };
+    std::map<std::array<int, 7>, GraphEntry> graphs_;
+    long long graph_clock_ = 0;
     long long col_bytes_ = 0;
     std::size_t device_bytes_ = 0;
 };

@@ -89,6 +89,38 @@ __global__ void k_fill_i64(long long* p, long long v, int n) {
     if (i < n) p[i] = v;
 }
 
+// ------------------------------------------------------- chunk bracket
+// First kernel of a chunk: copies the chunk-start state (positions,
+// velocities and force words, 3*nb each, then the two step slots) into the
+// snapshot buffer a rollback restores, and clears the batch flag word.
+__global__ void k_snapshot(int n3, const double* __restrict__ pos, const double* __restrict__ vel,
+                           const long long* __restrict__ frc, const long long* __restrict__ gstep,
+                           double* __restrict__ snap, int* __restrict__ flags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        reinterpret_cast<long long*>(snap)[3 * static_cast<size_t>(n3)] = gstep[0];
+        reinterpret_cast<long long*>(snap)[3 * static_cast<size_t>(n3) + 1] = gstep[1];
+        *flags = 0;
+    }
+    if (t >= n3) return;
+    snap[t] = pos[t];
+    snap[n3 + t] = vel[t];
+    reinterpret_cast<long long*>(snap)[2 * static_cast<size_t>(n3) + t] = frc[t];
+}
+
+// Last kernel of a chunk: copies the flag word and the per-system error
+// words into pinned host memory (host[0] flags, host[1 + s] system s) and
+// clears the error words for the next chunk.
+__global__ void k_status_out(int nsys, const int* __restrict__ flags, int* __restrict__ err,
+                             int* __restrict__ host) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) host[0] = *flags;
+    for (int s = t; s < nsys; s += gridDim.x * blockDim.x) {
+        host[1 + s] = err[s];
+        err[s] = 0;
+    }
+}
+
 // ---------------------------------------------------- record refresh
 // The integrator refreshes the coordinates of every pair record of bead b
 // (type and bead id were written by the build) and, when requested, flags
@@ -849,7 +881,7 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
     const int table = t.meta >> 8;
     R e = 0;
     switch (t.kind) {
-    case 1: case 2: case 3: {  // harmonic bond, Gaussian bond, Go contact
+    case 1: case 2: case 3: case 8: {  // harmonic bond, Gaussian bond, Go contact, capped Go contact
         const V3T<R> d = bdisp<R>(px, py, pz, t.idx[0], t.idx[1], S, pbc);
         const R r = r_sqrt<R>(dot3(d, d));
         R dedr;
@@ -864,12 +896,23 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
             const R is2 = R(1) / (s * s);
             e = k * r_exp<R>(-R(0.5) * dr * dr * is2);
             dedr = -e * dr * is2;
-        } else {
+        } else if (t.kind == 3 || static_cast<R>(t.p[2]) - r / static_cast<R>(t.p[1]) < R(0)) {
+            // Go 12-10 (the capped kind beyond its cap, r / r0 > capping ratio)
             const R k = static_cast<R>(t.p[0]), r0 = static_cast<R>(t.p[1]);
             const R q = r0 / r;
             const R q2 = q * q, q4 = q2 * q2, q10 = q4 * q4 * q2, q12 = q10 * q2;
             e = k * (R(5) * q12 - R(6) * q10);
             dedr = R(60) * k * (q10 - q12) / r;
+        } else {
+            // capped Go contact inside its cap: the line through the 12-10
+            // value and slope at r = capping_ratio * r0
+            const R k = static_cast<R>(t.p[0]), r0 = static_cast<R>(t.p[1]);
+            const R ic = R(1) / static_cast<R>(t.p[2]);
+            const R ic2 = ic * ic, ic10 = ic2 * ic2 * ic2 * ic2 * ic2, ic12 = ic10 * ic2;
+            const R slope = R(-60) * (ic12 * ic - ic10 * ic);
+            const R intercept = R(65) * ic12 - R(66) * ic10;
+            e = k * (slope * (r / r0) + intercept);
+            dedr = k * slope / r0;
         }
         if (!ENERGY) {
             const R s = dedr / r;
@@ -920,7 +963,7 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
         }
         break;
     }
-    case 6: case 7: {  // Gaussian dihedral, flexible local dihedral
+    case 6: case 7: case 9: {  // Gaussian, flexible local and cosine dihedrals
         const V3T<R> b1 = bdisp<R>(px, py, pz, t.idx[0], t.idx[1], S, pbc);
         const V3T<R> b2 = bdisp<R>(px, py, pz, t.idx[1], t.idx[2], S, pbc);
         const V3T<R> b3 = bdisp<R>(px, py, pz, t.idx[2], t.idx[3], S, pbc);
@@ -936,6 +979,12 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
             const R is2 = R(1) / (s * s);
             e = k * r_exp<R>(-R(0.5) * dtp * dtp * is2);
             dedp = -e * dtp * is2;
+        } else if (t.kind == 9) {
+            const R k = static_cast<R>(t.p[0]), t0 = static_cast<R>(t.p[1]), nn = static_cast<R>(t.p[2]);
+            R sn, cs;
+            r_sincos<R>(nn * (phi - t0), &sn, &cs);
+            e = k * (R(1) - cs);
+            dedp = k * nn * sn;
         } else {
             const FldTable& T = fld[table];
             const R k = static_cast<R>(t.p[0]);
@@ -1037,6 +1086,7 @@ template <> __device__ inline double r_exp_fast<double>(double v) { return exp(v
 // once per block of rows rather than per pair.
 template <typename R> struct PairK {
     R exv_eps, exv_corr, dh_K, dh_il, dh_corr, lja_rsw, lja_inv_sw;
+    R u_sig2, u_eps;   // uniform kinds: sigma^2 and epsilon
     bool lja_switch;
     bool lj_shared;
 };
@@ -1049,6 +1099,8 @@ template <typename R> __device__ inline PairK<R> pair_constants(const DevPairGro
     k.dh_corr = static_cast<R>(G.dh_corr);
     k.lja_rsw = static_cast<R>(G.lja_rsw);
     k.lja_inv_sw = static_cast<R>(G.lja_inv_sw);
+    k.u_sig2 = static_cast<R>(G.u_sigma * G.u_sigma);
+    k.u_eps = static_cast<R>(G.u_eps);
     k.lja_switch = G.lja_rsw > 0;
     k.lj_shared = G.lj_shared != 0;
     return k;
@@ -1074,7 +1126,7 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
     const R ir2 = r_rcp<R>(r2);
     R f = 0;
     R r = 0, ir = 0;
-    if ((kinds & kDh) || ((kinds & kLja) && G.lja_switch)) {
+    if ((kinds & (kDh | kIsolf)) || ((kinds & (kLja | kUlja)) && G.lja_switch)) {
         ir = r_rsqrt<R>(r2);
         r = r2 * ir;
     }
@@ -1142,15 +1194,70 @@ __device__ inline void pair_eval(int kinds, const PairK<R>& G, const float* __re
         f += du;
         e[3] = u;
     }
+    // the kinds below each have a group of their own; their energy goes to e[0]
+    if (kinds & kWca) {
+        const R sb = R(0.5) * (P(true, kTySig) + P(false, kTySig));
+        const R eb = r_sqrt<R>(P(true, kTyEps) * P(false, kTyEps));
+        const R s2 = sb * sb * ir2, s6w = s2 * s2 * s2, s12w = s6w * s6w;
+        if (c13 * sb * sb - r2 >= R(0)) {
+            f += eb * (R(-48) * s12w + R(24) * s6w) * ir2;
+            e[0] = eb * (R(4) * (s12w - s6w) + R(1));
+        } else {
+            e[0] = 0;
+        }
+    }
+    if (kinds & kUwca) {
+        const R s2 = G.u_sig2 * ir2, s6w = s2 * s2 * s2, s12w = s6w * s6w;
+        f += G.u_eps * (R(-48) * s12w + R(24) * s6w) * ir2;
+        e[0] = R(4) * G.u_eps * (s12w - s6w + R(0.25));
+    }
+    if (kinds & kUlja) {
+        const R s2 = G.u_sig2 * ir2, s6u = s2 * s2 * s2, s12u = s6u * s6u;
+        const R th2 = c13 * G.u_sig2;
+        R u = 0, du = 0;
+        if (r2 - th2 >= R(0)) {
+            u += R(4) * G.u_eps * (s12u - s6u);
+            du += G.u_eps * (R(-48) * s12u + R(24) * s6u) * ir2;
+        }
+        if (th2 - r2 >= R(0)) u -= G.u_eps;
+        if (G.lja_switch && r > G.lja_rsw) {
+            const R inv = G.lja_inv_sw;
+            const R x = (r - G.lja_rsw) * inv;
+            const R x2 = x * x, x3 = x2 * x;
+            const R S = R(1) - x3 * (R(10) - R(15) * x + R(6) * x2);
+            const R dS = R(-30) * x2 * (R(1) - x) * (R(1) - x) * inv;
+            du = du * S + u * dS * ir;
+            u = u * S;
+        }
+        f += du;
+        e[0] = u;
+    }
+    if (kinds & kIsolf) {
+        const R sb = R(0.5) * (P(true, kTySig) + P(false, kTySig));
+        const R eb = r_sqrt<R>(P(true, kTyEps) * P(false, kTyEps));
+        const R om = R(0.5) * (P(true, kTyOmega) + P(false, kTyOmega));
+        const R b = static_cast<R>(1.1224620483093729814) * sb;   // 2^(1/6) sigma
+        const R pi = static_cast<R>(3.1415926535897932385);
+        R u = b - r >= R(0) ? -eb : R(0);
+        if (r - b >= R(0) && b + om - r >= R(0)) {
+            R sn, cs;
+            r_sincos<R>(R(0.5) * pi * (r - b) / om, &sn, &cs);
+            u -= eb * cs * cs;
+            f += eb * R(2) * sn * cs * (R(0.5) * pi / om) * ir;
+        }
+        e[0] = u;
+    }
     *dedr_r = f;
 }
 
 // The row's own type parameters, held in registers for the force pass.
+// (A group of WCA or iSoLF reads its {sigma/2, sqrt(epsilon), omega/2} as
+// exv, q and om.)
 struct RowType {
-    float exv, q, er, sr, ea, sa;
+    float exv, q, om, er, sr, ea, sa;
 };
 __device__ inline RowType row_type(const float* __restrict__ t) {
-    return {t[kTyExv], t[kTyQ], t[kTyLjrEps], t[kTyLjrSig], t[kTyLjaEps], t[kTyLjaSig]};
+    return {t[kTyExv], t[kTyQ], t[kTyOmega], t[kTyLjrEps], t[kTyLjrSig], t[kTyLjaEps], t[kTyLjaSig]};
 }
 
 // (dE/dr)/r of the force pass.  Same terms as pair_eval<float>; when the
@@ -1169,7 +1276,7 @@ __device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, co
                                             const RowType& ti, const float* __restrict__ tj, float r2) {
     const float ir2 = __builtin_amdgcn_rcpf(r2);
     float f = 0.f, r = 0.f, ir = 0.f;
-    if ((kinds & kDh) || ((kinds & kLja) && lsw)) {
+    if ((kinds & (kDh | kIsolf)) || ((kinds & (kLja | kUlja)) && lsw)) {
         ir = __builtin_amdgcn_rsqf(r2);
         r = r2 * ir;
     }
@@ -1243,6 +1350,48 @@ __device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, co
         }
         f += du;
     }
+    // kinds with a group of their own (FP32 rows {sigma/2, sqrt(eps), omega/2})
+    if (kinds & kWca) {
+        const float sb = ti.exv + tj[kTySig];
+        const float eb = ti.q * tj[kTyEps];
+        const float sb2 = sb * sb;
+        const float s2 = sb2 * ir2, s6w = s2 * s2 * s2, s12w = s6w * s6w;
+        if (c13 * sb2 - r2 >= 0.f) f += eb * (-48.f * s12w + 24.f * s6w) * ir2;
+    }
+    if (kinds & kUwca) {
+        const float s2 = G.u_sig2 * ir2, s6w = s2 * s2 * s2, s12w = s6w * s6w;
+        f += G.u_eps * (-48.f * s12w + 24.f * s6w) * ir2;
+    }
+    if (kinds & kUlja) {
+        const float s2 = G.u_sig2 * ir2, s6u = s2 * s2 * s2, s12u = s6u * s6u;
+        const float th2 = c13 * G.u_sig2;
+        float u = 0.f, du = 0.f;
+        if (r2 - th2 >= 0.f) {
+            u += 4.f * G.u_eps * (s12u - s6u);
+            du += G.u_eps * (-48.f * s12u + 24.f * s6u) * ir2;
+        }
+        if (th2 - r2 >= 0.f) u -= G.u_eps;
+        if (lsw) {
+            const float x = fmaxf((r - G.lja_rsw) * G.lja_inv_sw, 0.f);
+            const float x2 = x * x, x3 = x2 * x;
+            const float S = 1.f - x3 * (10.f - 15.f * x + 6.f * x2);
+            const float dS = -30.f * x2 * (1.f - x) * (1.f - x) * G.lja_inv_sw;
+            du = du * S + u * dS * ir;
+        }
+        f += du;
+    }
+    if (kinds & kIsolf) {
+        const float sb = ti.exv + tj[kTySig];
+        const float eb = ti.q * tj[kTyEps];
+        const float om = ti.om + tj[kTyOmega];
+        const float b = 1.1224620483093729814f * sb;   // 2^(1/6) sigma
+        if (r - b >= 0.f && b + om - r >= 0.f) {
+            // dE/dr = eb (pi / (2 om)) sin(pi (r - b) / om); the hardware sine
+            // takes revolutions, and (r - b) / (2 om) lies in [0, 1/2]
+            const float iom = __builtin_amdgcn_rcpf(om);
+            f += eb * (1.5707963267948966f * iom) * __builtin_amdgcn_sinf(0.5f * (r - b) * iom) * ir;
+        }
+    }
     return f;
 }
 
@@ -1272,7 +1421,10 @@ struct BondedArgs {
 
 // Force-pass row loops, specialized by pair-group kind so the per-entry body
 // has no branches and the four entries of a round interleave.
-constexpr int kSpecGeneric = 0, kSpecExvPeriodic = 1, kSpecExvOpen = 2, kSpecDh = 3, kSpecLjShared = 4;
+// kSpecGeneric covers any mix of the four round kinds (EXV, DH, LJ
+// repulsive, LJ attractive), kSpecSolo the kinds that sit alone in a group.
+constexpr int kSpecGeneric = 0, kSpecExvPeriodic = 1, kSpecExvOpen = 2, kSpecDh = 3, kSpecLjShared = 4,
+              kSpecSolo = 5;
 
 // Raw buffer over the pair records: 32-bit offsets instead of 64-bit
 // addresses, and a read past the end (the -1 padding entries) returns zeros
@@ -1364,7 +1516,7 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
             double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
             // a group without a box has inverse box 0, for which the minimum
             // image is the identity
-            if (SPEC == kSpecGeneric ? g.periodic : SPEC != kSpecExvOpen) {
+            if (SPEC == kSpecGeneric || SPEC == kSpecSolo ? g.periodic : SPEC != kSpecExvOpen) {
                 dx = min_image(dx, g.bx, g.ibx);
                 dy = min_image(dy, g.by, g.iby);
                 dz = min_image(dz, g.bz, g.ibz);
@@ -1377,7 +1529,9 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
             if (SPEC == kSpecExvPeriodic || SPEC == kSpecExvOpen) f = pair_force(kExv, false, false, Kf, rt, tfj, r2f);
             else if (SPEC == kSpecDh) f = pair_force(kDh, false, false, Kf, rt, tfj, r2f);
             else if (SPEC == kSpecLjShared) f = pair_force(kLjr | kLja, true, true, Kf, rt, tfj, r2f);
-            else f = pair_force(kinds, Kf.lj_shared, Kf.lja_switch, Kf, rt, tfj, r2f);
+            else if (SPEC == kSpecSolo) f = pair_force(kinds & ~(kExv | kDh | kLjr | kLja), false, Kf.lja_switch, Kf, rt,
+                                                       tfj, r2f);
+            else f = pair_force(kinds & (kExv | kDh | kLjr | kLja), Kf.lj_shared, Kf.lja_switch, Kf, rt, tfj, r2f);
             // |f d| <= |f| r: one range test covers the three components.  A
             // pair beyond the limit is flagged and clamped to it along its
             // direction (so relaxation still pushes the beads apart); NaN
@@ -1565,7 +1719,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
     }
     int kinds = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kPairKinds; ++k)
         if (((G.kinds >> k) & 1) && ((gmask >> G.egroup[k]) & 1u)) kinds |= 1 << k;
     const int lane = threadIdx.x % TEAM;
     const int row = bt.y + threadIdx.x / TEAM;
@@ -1585,6 +1739,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
             if (kinds == kExv) spec = geo.periodic ? kSpecExvPeriodic : kSpecExvOpen;
             else if (kinds == kDh && geo.periodic) spec = kSpecDh;
             else if (kinds == (kLjr | kLja) && Kf.lj_shared && Kf.lja_switch && geo.periodic) spec = kSpecLjShared;
+            else if (kinds & ~(kExv | kDh | kLjr | kLja)) spec = kSpecSolo;
             switch (spec) {
             case kSpecExvPeriodic:
                 force_row<TEAM, kSpecExvPeriodic>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
@@ -1601,6 +1756,10 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
             case kSpecLjShared:
                 force_row<TEAM, kSpecLjShared>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                       lds_types, acc);
+                break;
+            case kSpecSolo:
+                force_row<TEAM, kSpecSolo>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                                                  lds_types, acc);
                 break;
             default:
                 force_row<TEAM, kSpecGeneric>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
@@ -1648,11 +1807,12 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
         ae3 = warp_sum(ae3, 64);
         if ((threadIdx.x & 63u) == 0) {
             const long long v[4] = {ae0, ae1, ae2, ae3};
+            // kinds 4.. are alone in their groups and leave their energy in slot 0
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < kPairKinds; ++k)
                 if ((kinds >> k) & 1)
                     atomicAdd(reinterpret_cast<unsigned long long*>(&energy[G.sys * 32 + G.egroup[k]]),
-                              static_cast<unsigned long long>(v[k]));
+                              static_cast<unsigned long long>(v[k < 4 ? k : 0]));
         }
     } else {
         const unsigned long long off = acc.nadd * kFixMagicBits;
@@ -1782,6 +1942,15 @@ void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int p
     k_integrate<<<nblk(d.nb, kIntegrateBlock), kIntegrateBlock, 0, st>>>(d.nb, d.sys, d.sys_of, d.local_id, d.inv_m, d.px, d.py,
                                                  d.pz, d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.gstep,
                                                  pin, pout, pending, d.err, pk, d.sys0, d.nsys == 1 ? 1 : 0);
+}
+
+void launch_snapshot(const DevPtrs& d, hipStream_t st, double* snap) {
+    const int n3 = 3 * d.nb;
+    k_snapshot<<<nblk(std::max(n3, 1), 256), 256, 0, st>>>(n3, d.px, d.vx, d.f.fx, d.gstep, snap, d.flags);
+}
+
+void launch_status_out(const DevPtrs& d, hipStream_t st, int* host) {
+    k_status_out<<<1, 256, 0, st>>>(d.nsys, d.flags, d.err, host);
 }
 
 void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur) {
@@ -1951,11 +2120,16 @@ void launch_kinetic(const DevPtrs& d, hipStream_t st) {
     k_kinetic<<<nblk(d.nb, 256), 256, 0, st>>>(d.nb, d.sys_of, d.mass, d.vx, d.vy, d.vz, d.ke, d.err);
 }
 
-void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members) {
-    k_fill_i32<<<1, 64, 0, st>>>(d.dump_count, 0, 1);
+void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members,
+                          int* count, int2* out) {
+    k_fill_i32<<<1, 64, 0, st>>>(count, 0, 1);
     if (members == 0) return;
     k_neighbor_dump<<<nblk(members, 128), 128, 0, st>>>(group, d.groups, d.rec, d.row_start, d.row_count,
-                                                        d.col, d.dump_count, d.dump_pairs, cap);
+                                                        d.col, count, out, cap);
+}
+
+void launch_fill_i32(hipStream_t st, int* p, int v, int n) {
+    if (n > 0) k_fill_i32<<<nblk(n, 256), 256, 0, st>>>(p, v, n);
 }
 
 void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed) {

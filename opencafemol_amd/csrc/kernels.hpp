@@ -75,7 +75,6 @@ struct DevPtrs {
     long long* energy = nullptr;   // nsys * 32 fixed point
     long long* ke = nullptr;       // nsys fixed point
     int* dump_count = nullptr;
-    int2* dump_pairs = nullptr;
 };
 
 constexpr int kPairBlock = 128;
@@ -90,6 +89,11 @@ constexpr int kPruneNone = 0, kPruneWrite = 1, kPruneCheck = 2;
 void launch_integrate(const DevPtrs& d, hipStream_t st, int pin, int pout, int pending, int pack, int check,
                       int prune = kPruneNone);
 void launch_half_kick(const DevPtrs& d, hipStream_t st, int pcur);
+// Chunk bracket: snap receives positions, velocities, force words (3*nb each)
+// and the two step slots, and the flag word is cleared; host (pinned, 1 + nsys
+// words) receives the flag word and the error words, which are then cleared.
+void launch_snapshot(const DevPtrs& d, hipStream_t st, double* snap);
+void launch_status_out(const DevPtrs& d, hipStream_t st, int* host);
 void launch_pack(const DevPtrs& d, hipStream_t st);
 void launch_rebuild(const DevPtrs& d, hipStream_t st);
 void launch_bonded(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& f, uint32_t groups);
@@ -108,7 +112,11 @@ void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int n
                           unsigned long long* out);
 void launch_zero_energy(const DevPtrs& d, hipStream_t st);
 void launch_kinetic(const DevPtrs& d, hipStream_t st);
-void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members);
+// Interacting pairs (r < cutoff, i < j as global bead ids) of one pair group
+// into out (up to cap pairs); *count receives how many there are.
+void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members,
+                          int* count, int2* out);
+void launch_fill_i32(hipStream_t st, int* p, int v, int n);
 void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed);
 
 }  // namespace ocm

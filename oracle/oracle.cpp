@@ -18,6 +18,8 @@
 //   fla_min_energy   FlexibleLocalAngleForceFieldGenerator.cpp:11-19,47-68
 //   fld_min_energy   FlexibleLocalDihedralForceFieldGenerator.cpp:7-26
 //   natural_spline   OpenMM Continuous1DFunction (natural cubic spline fit)
+//   eval_cells       oracle_eval through cell lists (every bead or a sampled
+//                    subset; large systems), pinned against oracle_eval
 //   baoab            the Langevin step this library implements (BAOAB splitting
 //                    with Philox4x32-10 noise); the reference's own integrator is
 //                    OpenMM's LangevinIntegrator (IntegratorGenerator.cpp:5-14),
@@ -171,7 +173,8 @@ double eval_bonded(const ocm_force_desc& f, const double* x, const Box& box, dou
         switch (f.kind) {
         case OCM_HARMONIC_BOND:
         case OCM_GAUSSIAN_BOND:
-        case OCM_GO_CONTACT: {
+        case OCM_GO_CONTACT:
+        case OCM_CAPPED_GO_CONTACT: {
             const int i = f.indices[2 * t], j = f.indices[2 * t + 1];
             const V3 d = disp(x, i, j, box, pbc);
             const double r = norm(d);
@@ -184,12 +187,27 @@ double eval_bonded(const ocm_force_desc& f, const double* x, const Box& box, dou
                 const double k = f.params[3 * t], v0 = f.params[3 * t + 1], s = f.params[3 * t + 2];
                 e = k * std::exp(-(r - v0) * (r - v0) / (2.0 * s * s));
                 dedr = -e * (r - v0) / (s * s);
-            } else {  // GoContact {k, r0}
+            } else if (f.kind == OCM_GO_CONTACT) {  // {k, r0}
                 const double k = f.params[2 * t], r0 = f.params[2 * t + 1];
                 const double q2 = (r0 / r) * (r0 / r);
                 const double q10 = q2 * q2 * q2 * q2 * q2, q12 = q10 * q2;
                 e = k * (5.0 * q12 - 6.0 * q10);
                 dedr = 60.0 * k * (q10 - q12) / r;
+            } else {
+                // CappedGoContactForceFieldGenerator.cpp:5-26, params {k, r0, cap}:
+                // cap_val * cap + (1 - cap_val) * lj, cap_val = step(cap - r / r0),
+                // cap = k slope (r / r0) + k intercept
+                const double k = f.params[3 * t], r0 = f.params[3 * t + 1], c = f.params[3 * t + 2];
+                const double slope = -60.0 * (std::pow(1.0 / c, 13.0) - std::pow(1.0 / c, 11.0));
+                const double intercept = 65.0 * std::pow(1.0 / c, 12.0) - 66.0 * std::pow(1.0 / c, 10.0);
+                if (c - r / r0 >= 0.0) {
+                    e = k * slope * (r / r0) + k * intercept;
+                    dedr = k * slope / r0;
+                } else {
+                    const double q = r0 / r;
+                    e = k * (5.0 * std::pow(q, 12.0) - 6.0 * std::pow(q, 10.0));
+                    dedr = k * (-60.0 * std::pow(q, 12.0) + 60.0 * std::pow(q, 10.0)) / r;
+                }
             }
             E += e;
             const V3 fi = mul(d, dedr / r);  // F_i = dE/dr * d/r with d = x_j - x_i
@@ -229,7 +247,8 @@ double eval_bonded(const ocm_force_desc& f, const double* x, const Box& box, dou
             break;
         }
         case OCM_GAUSSIAN_DIHEDRAL:
-        case OCM_FLEXIBLE_LOCAL_DIHEDRAL: {
+        case OCM_FLEXIBLE_LOCAL_DIHEDRAL:
+        case OCM_COSINE_DIHEDRAL: {
             const int i = f.indices[4 * t], j = f.indices[4 * t + 1];
             const int k = f.indices[4 * t + 2], l = f.indices[4 * t + 3];
             V3 gi, gj, gk, gl;
@@ -241,6 +260,12 @@ double eval_bonded(const ocm_force_desc& f, const double* x, const Box& box, dou
                 const double dtp = dt - std::floor((dt + kPi) / (2.0 * kPi)) * (2.0 * kPi);
                 e = kk * std::exp(-dtp * dtp / (2.0 * s * s));
                 dedp = -e * dtp / (s * s);
+            } else if (f.kind == OCM_COSINE_DIHEDRAL) {
+                // CosineDihedralForceFieldGenerator.cpp:7, params {k, t0, n}:
+                // k (1 - cos(n (theta - t0)))
+                const double kk = f.params[3 * t], t0 = f.params[3 * t + 1], nn = f.params[3 * t + 2];
+                e = kk * (1.0 - std::cos(nn * (phi - t0)));
+                dedp = kk * nn * std::sin(nn * (phi - t0));
             } else {  // {k}; globals c,a1,b1,a2,b2,a3,b3,Emin
                 const double kk = f.params[t];
                 const double* g = f.globals;
@@ -344,6 +369,64 @@ void pair_energy(const ocm_force_desc& f, int i, int j, double r, double* e, dou
         *dedr = du;
         break;
     }
+    case OCM_WCA: {
+        // WeeksChandlerAndersenForceFieldGenerator.cpp:34-40, npp {sigma, eps}
+        const double sb = (pp[2 * i] + pp[2 * j]) * 0.5;
+        const double eb = std::sqrt(pp[2 * i + 1] * pp[2 * j + 1]);
+        const double sr = sb / r, s6 = std::pow(sr, 6), s12 = s6 * s6;
+        if (sb * std::pow(2.0, 1.0 / 6.0) - r >= 0.0) {
+            *e = 4.0 * eb * (s12 - s6) + eb;
+            *dedr = eb * 4.0 * (-12.0 * s12 + 6.0 * s6) / r;
+        } else {
+            *e = 0; *dedr = 0;
+        }
+        break;
+    }
+    case OCM_UNIFORM_WCA: {
+        // UniformWeeksChandlerAndersenForceFieldGenerator.cpp:162-167, globals {sigma, eps}
+        const double sg = f.globals[0], ep = f.globals[1];
+        const double sr = sg / r, s6 = std::pow(sr, 6), s12 = s6 * s6;
+        *e = 4.0 * ep * (s12 - s6 + 0.25);
+        *dedr = ep * 4.0 * (-12.0 * s12 + 6.0 * s6) / r;
+        break;
+    }
+    case OCM_UNIFORM_LJ_ATTRACTIVE: {
+        // UniformLennardJonesAttractiveForceFieldGenerator.cpp:145-151 and its
+        // switch from 0.9 cutoff (:186-187), globals {sigma, eps}
+        const double sg = f.globals[0], ep = f.globals[1];
+        const double th = sg * std::pow(2.0, 1.0 / 6.0);
+        const double sr = sg / r, s6 = std::pow(sr, 6), s12 = s6 * s6;
+        double u = 0, du = 0;
+        if (r - th >= 0.0) {
+            u += ep * 4.0 * (s12 - s6);
+            du += ep * 4.0 * (-12.0 * s12 + 6.0 * s6) / r;
+        }
+        if (th - r >= 0.0) u -= ep;
+        const double rsw = f.switch_distance, rc = f.cutoff;
+        if (rsw > 0.0 && r > rsw) {
+            const double xx = (r - rsw) / (rc - rsw);
+            const double S = 1.0 - 10.0 * xx * xx * xx + 15.0 * xx * xx * xx * xx - 6.0 * std::pow(xx, 5);
+            const double dS = (-30.0 * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx * xx * xx) / (rc - rsw);
+            du = du * S + u * dS;
+            u = u * S;
+        }
+        *e = u;
+        *dedr = du;
+        break;
+    }
+    case OCM_ISOLF_ATTRACTIVE: {
+        // iSoLFAttractiveForceFieldGenerator.cpp:43-51, npp {sigma, eps, omega}
+        const double sb = (pp[3 * i] + pp[3 * j]) * 0.5;
+        const double eb = std::sqrt(pp[3 * i + 1] * pp[3 * j + 1]);
+        const double om = (pp[3 * i + 2] + pp[3 * j + 2]) * 0.5;
+        const double base = sb * std::pow(2.0, 1.0 / 6.0);
+        const double step_in = base - r >= 0.0 ? 1.0 : 0.0;
+        const double step_tail = (r - base >= 0.0 ? 1.0 : 0.0) * (base + om - r >= 0.0 ? 1.0 : 0.0);
+        const double c = std::cos(0.5 * kPi * (r - base) / om);
+        *e = -eb * (step_in + step_tail * c * c);
+        *dedr = eb * step_tail * 2.0 * c * std::sin(0.5 * kPi * (r - base) / om) * 0.5 * kPi / om;
+        break;
+    }
     default:
         *e = 0; *dedr = 0;
     }
@@ -374,7 +457,7 @@ double eval_pairs(const ocm_force_desc& f, int n, const double* x, const Box& bo
     return E;
 }
 
-bool is_bonded(int kind) { return kind >= 1 && kind <= 7; }
+bool is_bonded(int kind) { return kind >= OCM_HARMONIC_BOND && kind <= OCM_COSINE_DIHEDRAL; }
 
 // ------------------------------------------------------------------ Philox
 inline void philox_round(uint32_t* c, const uint32_t* k) {
@@ -591,8 +674,30 @@ struct CellPairs {
     std::vector<std::vector<int>> excl;   // sorted partners per bead
 };
 
-void pair_forces_cells(const ocm_system_desc* s, const CellPairs& cp, const double* x, double* F,
-                       int nthreads) {
+CellPairs make_cell_pairs(const ocm_force_desc& f, int n) {
+    CellPairs cp{&f, std::vector<std::vector<int>>(n)};
+    for (int64_t e = 0; e < f.n_exclusions; ++e) {
+        const int a = f.exclusions[2 * e], b = f.exclusions[2 * e + 1];
+        if (a == b) continue;
+        cp.excl[a].push_back(b);
+        cp.excl[b].push_back(a);
+    }
+    for (auto& v2 : cp.excl) {
+        std::sort(v2.begin(), v2.end());
+        v2.erase(std::unique(v2.begin(), v2.end()), v2.end());
+    }
+    return cp;
+}
+
+// Nonbonded force f through a cell list: the same pairs (participating,
+// allowed, not excluded, minimum-image r^2 < cutoff^2 by r2_of) and the same
+// per-pair formulas (pair_energy) as eval_pairs, found from cells of edge >=
+// cutoff.  Forces are gathered per bead: on every bead (rows == NULL, F n*3
+// indexed by bead) or on beads rows[0..m) (F m*3 in that order), added to F.
+// Returns the energy of every pair once (i < j) when want_energy (rows must
+// then be NULL), else 0.  OpenMP over the rows.
+double pair_cells(const ocm_system_desc* s, const CellPairs& cp, const double* x, double* F, const int* rows,
+                  int m, int nthreads, bool want_energy) {
     const ocm_force_desc& f = *cp.f;
     const int n = s->n_particles;
     const Box box = make_box(s);
@@ -612,9 +717,10 @@ void pair_forces_cells(const ocm_system_desc* s, const CellPairs& cp, const doub
         }
         nc[c] = std::max(1, static_cast<int>(ext[c] / rc));
         if (pbc && nc[c] < 3) nc[c] = 1;
+        nc[c] = std::min(nc[c], 1024);
     }
-    const int ncell = nc[0] * nc[1] * nc[2];
-    std::vector<int> cell_of(n, -1), head(ncell, -1), next(n, -1);
+    const long long ncell = static_cast<long long>(nc[0]) * nc[1] * nc[2];
+    std::vector<int> cell_of(n, -1), head(static_cast<size_t>(ncell), -1), next(n, -1);
     for (int i = n - 1; i >= 0; --i) {
         if (!participates(f, i)) continue;
         int ci[3];
@@ -629,9 +735,12 @@ void pair_forces_cells(const ocm_system_desc* s, const CellPairs& cp, const doub
         next[i] = head[cell];
         head[cell] = i;
     }
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
-    for (int i = 0; i < n; ++i) {
-        if (cell_of[i] < 0) continue;
+    const int nrows = rows ? m : n;
+    double E = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64) reduction(+ : E)
+    for (int q = 0; q < nrows; ++q) {
+        const int i = rows ? rows[q] : q;
+        if (i < 0 || i >= n || cell_of[i] < 0) continue;
         const int cell = cell_of[i];
         const int cx = cell % nc[0], cy = (cell / nc[0]) % nc[1], cz = cell / (nc[0] * nc[1]);
         double acc[3] = {0, 0, 0};
@@ -651,21 +760,27 @@ void pair_forces_cells(const ocm_system_desc* s, const CellPairs& cp, const doub
                         if (j == i || !allowed(f, i, j)) continue;
                         const auto& ex = cp.excl[i];
                         if (!ex.empty() && std::binary_search(ex.begin(), ex.end(), j)) continue;
-                        const V3 d = disp(x, i, j, box, pbc);
+                        // displacement from the lower index, as eval_pairs takes it
+                        const int a = std::min(i, j), b = std::max(i, j);
+                        const V3 d = disp(x, a, b, box, pbc);
                         const double r2 = r2_of(d);
                         if (!(r2 < rc2)) continue;
                         const double r = std::sqrt(r2);
                         double e, dedr;
-                        pair_energy(f, i, j, r, &e, &dedr);
-                        acc[0] += d.x * dedr / r;
-                        acc[1] += d.y * dedr / r;
-                        acc[2] += d.z * dedr / r;
+                        pair_energy(f, a, b, r, &e, &dedr);
+                        const double sgn = i == a ? 1.0 : -1.0;
+                        acc[0] += sgn * d.x * dedr / r;
+                        acc[1] += sgn * d.y * dedr / r;
+                        acc[2] += sgn * d.z * dedr / r;
+                        if (want_energy && j > i) E += e;
                     }
                 }
             }
         }
-        F[3 * i] += acc[0]; F[3 * i + 1] += acc[1]; F[3 * i + 2] += acc[2];
+        double* Fi = F + 3 * static_cast<size_t>(rows ? q : i);
+        Fi[0] += acc[0]; Fi[1] += acc[1]; Fi[2] += acc[2];
     }
+    return E;
 }
 
 }  // namespace
@@ -678,23 +793,13 @@ int oracle_baoab_cells(const ocm_system_desc* s, const ocm_integrator_desc* it, 
     const int n = s->n_particles;
     const Box box = make_box(s);
     std::vector<CellPairs> cps;
-    for (int fi = 0; fi < s->n_forces; ++fi) {
-        const ocm_force_desc& f = s->forces[fi];
-        if (is_bonded(f.kind)) continue;
-        CellPairs cp{&f, std::vector<std::vector<int>>(n)};
-        for (int64_t e = 0; e < f.n_exclusions; ++e) {
-            const int a = f.exclusions[2 * e], b = f.exclusions[2 * e + 1];
-            cp.excl[a].push_back(b);
-            cp.excl[b].push_back(a);
-        }
-        for (auto& v2 : cp.excl) std::sort(v2.begin(), v2.end());
-        cps.push_back(std::move(cp));
-    }
+    for (int fi = 0; fi < s->n_forces; ++fi)
+        if (!is_bonded(s->forces[fi].kind)) cps.push_back(make_cell_pairs(s->forces[fi], n));
     auto forces = [&](std::vector<double>& F) {
         std::fill(F.begin(), F.end(), 0.0);
         for (int fi = 0; fi < s->n_forces; ++fi)
             if (is_bonded(s->forces[fi].kind)) eval_bonded(s->forces[fi], x, box, F.data());
-        for (const auto& cp : cps) pair_forces_cells(s, cp, x, F.data(), nthreads);
+        for (const auto& cp : cps) pair_cells(s, cp, x, F.data(), nullptr, 0, nthreads, false);
     };
     const double kB = 1.380649e-23 * 6.02214076e23 / 1000.0;
     const double kT = kB * it->temperature, h = 0.5 * it->dt;
@@ -722,6 +827,40 @@ int oracle_baoab_cells(const ocm_system_desc* s, const ocm_integrator_desc* it, 
         forces(F);
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < 3; ++c) v[3 * i + c] += h * F[3 * i + c] * im[i];
+    }
+    return 0;
+}
+
+// oracle_eval through cell lists: forces of the groups in `groups` on every
+// bead (rows == NULL; F n*3) or on beads rows[0..m) (F m*3, in that order),
+// and, when E_by_group is given (rows must be NULL), the energy per group.
+// F may be NULL.  OpenMP over beads for the nonbonded forces.
+int oracle_eval_cells(const ocm_system_desc* s, const double* x, uint32_t groups, const int32_t* rows, int32_t m,
+                      int32_t nthreads, double* F, double* E_by_group) {
+    const int n = s->n_particles;
+    const Box box = make_box(s);
+    if (rows && E_by_group) return -1;
+    const size_t nf = 3 * static_cast<size_t>(rows ? m : n);
+    if (F) std::memset(F, 0, sizeof(double) * nf);
+    if (E_by_group) std::memset(E_by_group, 0, sizeof(double) * 32);
+    std::vector<double> Fb(3 * static_cast<size_t>(n), 0.0), Fp(nf, 0.0);
+    for (int fi = 0; fi < s->n_forces; ++fi) {
+        const ocm_force_desc& f = s->forces[fi];
+        if (!((groups >> f.group) & 1u)) continue;
+        double e;
+        if (is_bonded(f.kind)) {
+            e = eval_bonded(f, x, box, Fb.data());
+        } else {
+            const CellPairs cp = make_cell_pairs(f, n);
+            e = pair_cells(s, cp, x, Fp.data(), rows, m, nthreads, E_by_group != nullptr);
+        }
+        if (E_by_group) E_by_group[f.group] += e;
+    }
+    if (F) {
+        for (size_t q = 0; q < nf / 3; ++q) {
+            const size_t b = rows ? static_cast<size_t>(rows[q]) : q;
+            for (int c = 0; c < 3; ++c) F[3 * q + c] = Fb[3 * b + c] + Fp[3 * q + c];
+        }
     }
     return 0;
 }

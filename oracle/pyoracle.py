@@ -42,6 +42,9 @@ def lib():
         L.oracle_baoab.restype = C.c_int
         L.oracle_kinetic.argtypes = [P(abi.SystemDesc), P(C.c_double)]
         L.oracle_kinetic.restype = C.c_double
+        L.oracle_eval_cells.argtypes = [P(abi.SystemDesc), P(C.c_double), C.c_uint32, P(C.c_int32), C.c_int32,
+                                        C.c_int32, P(C.c_double), P(C.c_double)]
+        L.oracle_eval_cells.restype = C.c_int
         _lib = L
     return _lib
 
@@ -58,6 +61,32 @@ def evaluate(system, x, groups=0xFFFFFFFF):
     E = np.zeros(32)
     lib().oracle_eval(C.byref(sd), _dp(x), groups, _dp(F), _dp(E))
     del keep
+    return F, E
+
+
+def evaluate_cells(system, x, groups=0xFFFFFFFF, rows=None, nthreads=None, energies=True):
+    """evaluate() through cell lists: (forces, energy_by_group).  With `rows`
+    the forces are those of beads rows (in that order) and no energies are
+    computed (energy_by_group is None)."""
+    sd, keep = system.descriptor()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    if rows is not None:
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        F = np.zeros((len(r), 3))
+        code = lib().oracle_eval_cells(C.byref(sd), _dp(x), groups, r.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       len(r), nthreads, _dp(F), None)
+        E = None
+    else:
+        F = np.zeros((system.n, 3))
+        E = np.zeros(32)
+        code = lib().oracle_eval_cells(C.byref(sd), _dp(x), groups, None, 0, nthreads, _dp(F),
+                                       _dp(E) if energies else None)
+        if not energies:
+            E = None
+    del keep
+    if code != 0:
+        raise ValueError("oracle_eval_cells: energies need every row")
     return F, E
 
 

@@ -359,3 +359,145 @@ def test_baoab_equipartition(oracle):
             kes.append(oracle.kinetic(s, v))
     kT = 1.380649e-23 * 6.02214076e23 / 1000 * 300.0
     assert np.mean(kes) == pytest.approx(1.5 * n * kT, rel=0.12)
+
+
+# ------------------------------------------------------- cell-list oracle
+def _cells_cases():
+    hps = configs.hps_slab(n_chains=12, chain_len=30, box=(8.0, 8.0, 30.0), slab_z=6.0, seed=4)
+    go = configs.aicg2_chain(56, seed=2)
+    x_go = go[2] + np.random.default_rng(2).normal(scale=0.02, size=go[2].shape)
+    return [(hps[0], hps[2]), (go[0], x_go)]
+
+
+def test_eval_cells_matches_brute_force(oracle):
+    """The cell-list evaluation (used on systems too large for the O(N^2)
+    oracle) finds the same pairs with the same formulas: forces and energies
+    equal the brute-force oracle up to summation order, on a periodic slab
+    (three pair kinds, a Debye-Hueckel group of charged beads only) and an
+    open AICG2+ chain (every bonded kind)."""
+    for s, x in _cells_cases():
+        Fb, Eb = oracle.evaluate(s, x)
+        Fc, Ec = oracle.evaluate_cells(s, x, nthreads=4)
+        scale = np.abs(Fb).max()
+        assert np.abs(Fc - Fb).max() <= 1e-12 * scale
+        assert np.abs(Ec - Eb).max() <= 1e-12 * np.abs(Eb).sum()
+        rows = np.random.default_rng(0).choice(s.n, 50, replace=False)
+        Fr, _ = oracle.evaluate_cells(s, x, rows=rows, nthreads=2)
+        assert np.array_equal(Fr, Fc[rows])
+
+
+def test_eval_cells_counts_each_pair_once_across_images(oracle):
+    """Two beads one cutoff-fraction apart across the periodic boundary:
+    the minimum image interacts, exactly once."""
+    s = SystemSpec(masses=np.full(2, 100.0), box=(10.0, 10.0, 10.0))
+    s.add(ForceSpec(abi.EXCLUDED_VOLUME, "ExcludedVolume", use_periodic=True,
+                    particle_params=np.array([[0.2], [0.2]]), cutoff=0.8, globals=np.array([1.0])))
+    x = np.array([[0.1, 5.0, 5.0], [9.7, 5.0, 5.0]])
+    Fb, Eb = oracle.evaluate(s, x)
+    Fc, Ec = oracle.evaluate_cells(s, x)
+    r = 0.4
+    assert Eb.sum() == pytest.approx(0.4 ** 12 * (r ** -12 - 0.8 ** -12))
+    assert np.allclose(Fc, Fb, rtol=1e-13) and np.allclose(Ec, Eb, rtol=1e-13)
+
+
+def test_baoab_cells_matches_baoab(oracle):
+    """oracle_baoab_cells (the CPU baseline's step) against oracle_baoab: the
+    same trajectory up to force summation order over 20 steps."""
+    s, it, x = configs.hps_slab(n_chains=12, chain_len=30, box=(8.0, 8.0, 30.0), slab_z=6.0, seed=6)
+    v = np.random.default_rng(6).normal(scale=0.2, size=x.shape)
+    x1, v1 = oracle.baoab(s, it, x, v, 3, 20)
+    x2, v2 = oracle.baoab_cells(s, it, x, v, 3, 20, 4)
+    assert np.abs(x2 - x1).max() <= 1e-9
+    assert np.abs(v2 - v1).max() <= 1e-7 * np.abs(v1).max()
+    assert np.abs(x1 - x).max() > 1e-3   # it moved
+
+
+# ------------------------------------------- membrane and capped potentials
+def test_capped_go_contact_cap_is_the_tangent_line(oracle):
+    """Inside its cap (r / r0 <= c) the capped contact is the tangent line of
+    the 12-10 curve at r = c r0: value and slope match there, the energy is
+    linear below it (zero second difference), and beyond the cap it is the
+    plain 12-10 contact (minimum -k at r0)."""
+    k, r0, c = 2.0, 0.6, 0.8
+    def E(r):
+        s, x = _two_body(abi.CAPPED_GO_CONTACT, [k, r0, c], r)
+        return _energy(oracle, s, x)
+    def go(r):
+        s, x = _two_body(abi.GO_CONTACT, [k, r0], r)
+        return _energy(oracle, s, x)
+    rc, h = c * r0, 1e-6
+    assert E(r0) == pytest.approx(-k, rel=1e-12)
+    assert E(0.9 * r0) == pytest.approx(go(0.9 * r0), rel=1e-12)
+    assert E(rc) == pytest.approx(go(rc), rel=1e-10)
+    slope_go = (go(rc + h) - go(rc - h)) / (2 * h)
+    slope_cap = (E(rc - h) - E(rc - 3 * h)) / (2 * h)
+    assert slope_cap == pytest.approx(slope_go, rel=1e-5)
+    a, b, m = E(0.2 * r0), E(0.4 * r0), E(0.3 * r0)
+    assert m == pytest.approx(0.5 * (a + b), rel=1e-12)   # linear below the cap
+    assert E(0.1 * r0) < 1e3 * k                          # no r^-12 wall inside the cap
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_cosine_dihedral_closed_form(oracle, n):
+    """k (1 - cos(n (phi - t0))): zero at t0, 2k half a period away, period 2 pi / n."""
+    t0 = 0.4
+    s = SystemSpec(masses=np.ones(4))
+    s.add(ForceSpec(abi.COSINE_DIHEDRAL, "c", indices=np.array([[0, 1, 2, 3]]), params=np.array([[1.5, t0, n]])))
+    assert _energy(oracle, s, _dihedral_positions(t0)) == pytest.approx(0.0, abs=1e-12)
+    assert _energy(oracle, s, _dihedral_positions(t0 + math.pi / n)) == pytest.approx(3.0, rel=1e-12)
+    p = -1.1
+    assert _energy(oracle, s, _dihedral_positions(p)) == pytest.approx(
+        _energy(oracle, s, _dihedral_positions(p + 2 * math.pi / n)), rel=1e-10, abs=1e-12)
+
+
+def test_wca_closed_form_and_mixing(oracle):
+    """WCA: epsilon mixed geometrically, sigma arithmetically; E(sigma) = eps,
+    zero from 2^(1/6) sigma on."""
+    pp = [[0.5, 1.0], [0.7, 4.0]]          # sigma, eps: sigma 0.6, eps sqrt(4) = 2
+    s = _pair_system(abi.WCA, pp, 0.7 * 2 ** (1 / 6))
+    assert _energy(oracle, s, _at(0.6)) == pytest.approx(2.0, rel=1e-12)
+    rstar = 0.6 * 2 ** (1 / 6)
+    assert _energy(oracle, s, _at(rstar * (1 - 1e-9))) == pytest.approx(0.0, abs=1e-6)
+    assert _energy(oracle, s, _at(rstar * 1.0001)) == 0.0
+
+
+def test_uniform_wca_and_lj_tables_act_between_their_categories(oracle):
+    """Uniform (table) WCA between categories 0 and 1 only: E(sigma) = eps for
+    an A-B pair, nothing for A-A.  Uniform LJ attraction: -eps inside r*, the
+    LJ tail outside, zero at the switched cutoff."""
+    sg, ep = 0.5, 1.3
+    s = SystemSpec(masses=np.ones(3))
+    s.add(ForceSpec(abi.UNIFORM_WCA, "u", globals=np.array([sg, ep]), category=np.array([0, 1, 0], np.int32),
+                    n_categories=2, allow=np.array([[0, 1], [1, 0]], np.uint8), cutoff=sg * 2 ** (1 / 6)))
+    x = np.array([[1.0, 1.0, 1.0], [1.0, 1.0 + sg, 1.0], [1.0, 1.0, 1.0 + 0.45]])   # 0-1 at sigma, 0-2 at 0.45
+    assert _energy(oracle, s, x) == pytest.approx(ep, rel=1e-12)
+    s2 = SystemSpec(masses=np.ones(2))
+    rc = 2.5 * sg
+    s2.add(ForceSpec(abi.UNIFORM_LJ_ATTRACTIVE, "l", globals=np.array([sg, ep]), cutoff=rc, switch_distance=0.9 * rc))
+    assert _energy(oracle, s2, _at(0.9 * sg)) == pytest.approx(-ep, rel=1e-12)
+    r = 1.0
+    assert _energy(oracle, s2, _at(r)) == pytest.approx(4 * ep * ((sg / r) ** 12 - (sg / r) ** 6), rel=1e-12)
+    assert _energy(oracle, s2, _at(rc * (1 - 1e-7))) == pytest.approx(0.0, abs=1e-9)
+
+
+def test_isolf_closed_form(oracle):
+    """iSoLF: -eps up to b = 2^(1/6) sigma, -eps cos^2(pi (r - b) / (2 omega))
+    out to b + omega, zero beyond; eps mixed geometrically."""
+    pp = [[0.6, 1.0, 0.9], [0.6, 9.0, 1.1]]   # sigma 0.6, eps 3, omega 1.0
+    b = 0.6 * 2 ** (1 / 6)
+    s = _pair_system(abi.ISOLF_ATTRACTIVE, pp, b + 1.2)
+    assert _energy(oracle, s, _at(0.9 * b)) == pytest.approx(-3.0, rel=1e-12)
+    assert _energy(oracle, s, _at(b + 0.5)) == pytest.approx(-1.5, rel=1e-12)      # cos^2(pi/4)
+    assert _energy(oracle, s, _at(b + 0.999999)) == pytest.approx(0.0, abs=1e-9)
+    assert _energy(oracle, s, _at(b + 1.1)) == 0.0
+
+
+def test_finite_differences_membrane_and_capped(oracle):
+    s, it, x = configs.lipid_patch(n_lipids=64, box=(6.0, 6.0, 6.0), seed=1)
+    x = x + np.random.default_rng(3).normal(scale=0.03, size=x.shape)
+    kinds = {f.kind for f in s.forces}
+    assert {abi.WCA, abi.ISOLF_ATTRACTIVE, abi.UNIFORM_WCA, abi.UNIFORM_LJ_ATTRACTIVE} <= kinds
+    _fd_check(oracle, s, x, range(0, s.n, 13), h=1e-7, tol=1e-5)
+    s, it, x = configs.capped_go_chain(50, seed=2, capping_ratio=1.0)
+    x = x + np.random.default_rng(4).normal(scale=0.02, size=x.shape)
+    _fd_check(oracle, s, x, range(0, 50, 3))

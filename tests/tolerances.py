@@ -4,7 +4,10 @@ Forces: per-bead relative error |F_gpu - F_ref| / max(|F_ref|, 0.01 F_rms)
 <= 1e-4.  The floor (1% of the RMS bead force) keeps beads whose net force
 cancels to nearly zero from dividing by ~0; without it a per-bead ratio is
 undefined for such beads.
-Energies: |E_gpu - E_ref| <= 1e-6 * sum_g |E_ref,g| per group and in total.
+Energies: the total within 1e-6 of |E_total| (the stated bar), and each
+group within 1e-6 of its own |E_g| plus 1e-5 kJ/mol (the energy pass sums
+terms in fixed point at 2^-32 kJ/mol, so a group of many terms carries an
+absolute rounding of that order).
 """
 import numpy as np
 
@@ -24,9 +27,15 @@ def assert_forces_close(Fg, Fo, rel=FORCE_REL):
     assert err.max() <= rel, f"bead {i}: rel err {err.max():.3e} gpu {Fg[i]} ref {Fo[i]}"
 
 
+ENERGY_ABS = 1e-5
+
+
 def assert_energies_close(Eg, Eo, rel=ENERGY_REL):
-    scale = np.sum(np.abs(Eo)) + 1e-9
-    d = np.abs(np.asarray(Eg) - np.asarray(Eo))
-    g = int(np.argmax(d))
-    assert d.max() <= rel * scale, f"group {g}: gpu {Eg[g]} ref {Eo[g]} (scale {scale})"
-    assert abs(np.sum(Eg) - np.sum(Eo)) <= rel * scale
+    Eg = np.asarray(Eg, dtype=np.float64)
+    Eo = np.asarray(Eo, dtype=np.float64)
+    d = np.abs(Eg - Eo)
+    bar = rel * np.abs(Eo) + ENERGY_ABS
+    g = int(np.argmax(d / bar))
+    assert np.all(d <= bar), f"group {g}: gpu {Eg[g]} ref {Eo[g]}"
+    total = abs(np.sum(Eo))
+    assert abs(np.sum(Eg) - np.sum(Eo)) <= rel * total + ENERGY_ABS, (np.sum(Eg), np.sum(Eo))
