@@ -332,16 +332,20 @@ def serpentine_chains(n_chains, chain_len, box, spacing=(0.38, 0.95, 0.95)):
     orthorhombic lattice (spacing nm along x, y, z) that fills the x-y extent of
     `box` and as many z layers as the beads need, centred in z.  Steps along a
     row are spacing[0] (the bond length, so chains start unstretched); row and
-    layer turns are one spacing[1] or spacing[2] step.  No two beads share a
-    site.  Vectorised: O(N) for a million beads.  Returns (N, 3) nm."""
+    layer turns are one spacing[1] or spacing[2] step.  One site is left empty
+    between consecutive chains, so unbonded beads start at least
+    2 * spacing[0] apart.  No two beads share a site.  Vectorised: O(N) for a
+    million beads.  Returns (N, 3) nm."""
     sx, sy, sz = spacing
     n = n_chains * chain_len
     nx, ny = int(box[0] // sx), int(box[1] // sy)
     per_layer = nx * ny
-    nz = -(-n // per_layer)
+    n_sites = n + n_chains - 1
+    nz = -(-n_sites // per_layer)
     if nz * sz > box[2]:
         raise ValueError("box too small for the requested beads")
     k = np.arange(n)
+    k = k + k // chain_len
     layer = k // per_layer
     r = k % per_layer
     row = r // nx

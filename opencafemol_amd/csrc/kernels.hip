@@ -882,17 +882,27 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
     R e = 0;
     switch (t.kind) {
     case 1: case 2: case 3: case 8: {  // harmonic bond, Gaussian bond, Go contact, capped Go contact
-        const V3T<R> d = bdisp<R>(px, py, pz, t.idx[0], t.idx[1], S, pbc);
-        const R r = r_sqrt<R>(dot3(d, d));
+        const V3T<double> dd = bdisp<double>(px, py, pz, t.idx[0], t.idx[1], S, pbc);
+        const V3T<R> d = {static_cast<R>(dd.x), static_cast<R>(dd.y), static_cast<R>(dd.z)};
+        const double r2d = fma(dd.z, dd.z, fma(dd.y, dd.y, dd.x * dd.x));
+        const R r = r_sqrt<R>(static_cast<R>(r2d));
+        // r - r0 of the two stiff kinds.  In FP32, r - r0 from an FP32 r
+        // would carry an absolute error of ~r0 * 2^-24 (2e-8 nm), which a
+        // 92,000 kJ/mol/nm^2 bond turns into 2e-3 kJ/mol/nm; (r^2 - r0^2) in
+        // FP64 over (r + r0) keeps the relative error of r - r0 itself ~1e-7.
+        auto stretch = [&](double r0d) -> R {
+            if (sizeof(R) == sizeof(double)) return static_cast<R>(sqrt(r2d) - r0d);
+            return static_cast<R>(fma(-r0d, r0d, r2d)) / (r + static_cast<R>(r0d));
+        };
         R dedr;
         if (t.kind == 1) {
-            const R r0 = static_cast<R>(t.p[0]), k = static_cast<R>(t.p[1]);
-            const R dr = r - r0;
+            const R k = static_cast<R>(t.p[1]);
+            const R dr = stretch(t.p[0]);
             e = R(0.5) * k * dr * dr;
             dedr = k * dr;
         } else if (t.kind == 2) {
-            const R k = static_cast<R>(t.p[0]), v0 = static_cast<R>(t.p[1]), s = static_cast<R>(t.p[2]);
-            const R dr = r - v0;
+            const R k = static_cast<R>(t.p[0]), s = static_cast<R>(t.p[2]);
+            const R dr = stretch(t.p[1]);
             const R is2 = R(1) / (s * s);
             e = k * r_exp<R>(-R(0.5) * dr * dr * is2);
             dedr = -e * dr * is2;
