@@ -258,6 +258,23 @@ int ocm_minimize(ocm_context* ctx, double tolerance, int32_t max_iterations);
 int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index,
                            int32_t* out_pairs, int64_t cap, int64_t* count);
 
+/* ocm_get_neighbor_pairs without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The pairs pass through a fixed device scratch array of
+ * OCM_NOALLOC_MAX_PAIRS pairs that each GPU holds until the process ends.
+ * Differences from ocm_get_neighbor_pairs:
+ *  - rows are not widened: if a row no longer fits, OCM_ERR_OOM (a step or
+ *    ocm_get_neighbor_pairs widens them);
+ *  - cap = 0 only counts; otherwise cap must be >= the pair count, and more
+ *    than OCM_NOALLOC_MAX_PAIRS pairs is an error (OCM_ERR_INVALID, with
+ *    *count set in both cases);
+ *  - every call on one GPU shares the scratch array: calls that overlap on
+ *    one GPU (from several threads) race.                                   */
+#define OCM_NOALLOC_MAX_PAIRS (1 << 21)
+int ocm_get_neighbor_pairs_noalloc(ocm_context* ctx, int32_t force_index,
+                                   int32_t* out_pairs, int64_t cap, int64_t* count);
+
 /* Per-kernel timing.  enable = 1: HIP events on the context's stream around
  * every launch (steps run eagerly), read with kernel 0 = pair forces,
  * 1 = bonded forces, 2 = integrator, 3 = list build, 4 = record pack,

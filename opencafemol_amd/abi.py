@@ -49,7 +49,7 @@ STATE_ENERGY = 8
 EXPORTED = [
     "ocm_options_default", "ocm_context_create", "ocm_context_destroy", "ocm_set_positions",
     "ocm_set_velocities", "ocm_set_velocities_to_temperature", "ocm_step", "ocm_prepare_steps", "ocm_get_state",
-    "ocm_get_neighbor_pairs", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
+    "ocm_get_neighbor_pairs", "ocm_get_neighbor_pairs_noalloc", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
     "ocm_get_inner_list_stats", "ocm_get_list_counts",
     "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
     "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
@@ -151,6 +151,8 @@ def load_library(path=LIB_PATH):
         "ocm_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(State)]),
         "ocm_batch_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, P(State)]),
         "ocm_get_neighbor_pairs": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), C.c_int64, P(C.c_int64)]),
+        "ocm_get_neighbor_pairs_noalloc": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), C.c_int64,
+                                                     P(C.c_int64)]),
         "ocm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
         "ocm_get_timing": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_int64)]),
         "ocm_get_list_stats": (C.c_int, [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64)]),

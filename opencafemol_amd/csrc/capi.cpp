@@ -188,6 +188,25 @@ int ocm_get_neighbor_pairs(ocm_context* ctx, int32_t force_index, int32_t* out_p
     });
 }
 
+int ocm_get_neighbor_pairs_noalloc(ocm_context* ctx, int32_t force_index, int32_t* out_pairs, int64_t cap,
+                                   int64_t* count) {
+    return guarded([&] {
+        auto& e = eng(ctx);
+        need(count, "count");
+        if (cap < 0) throw ocm::Error(OCM_ERR_INVALID, "cap must not be negative");
+        if (cap > 0) need(out_pairs, "out_pairs");
+        long long n = 0;
+        try {
+            e.neighbor_pairs_noalloc(0, force_index, out_pairs, cap, &n);
+        } catch (...) {
+            *count = n;
+            throw;
+        }
+        *count = n;
+        return OCM_OK;
+    });
+}
+
 int ocm_set_timing(ocm_context* ctx, int32_t enable) {
     return guarded([&] {
         eng(ctx).set_timing(enable);

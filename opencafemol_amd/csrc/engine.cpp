@@ -1509,6 +1509,46 @@ long long Engine::neighbor_pairs(int s, int force_index, int32_t* out, long long
     return cnt;
 }
 
+static_assert(kNoAllocMaxPairs == OCM_NOALLOC_MAX_PAIRS, "scratch size and the header's limit must agree");
+
+void Engine::neighbor_pairs_noalloc(int s, int force_index, int32_t* out, long long cap, long long* count) {
+    const int g = pair_group_of(s, force_index);
+    check_hip(hipSetDevice(opt_.device), "hipSetDevice");
+    int* dcount = nullptr;
+    int2* dpairs = nullptr;
+    check_hip(noalloc_scratch(opt_.device, &dcount, &dpairs), "hipGetSymbolAddress(scratch)");
+    // the list at the current positions, in the rows it has
+    if (rebuild_now() & kFlagListOverflow)
+        throw Error(OCM_ERR_OOM, "a neighbor-list row no longer fits its slots and this call does not widen "
+                                 "them; ocm_step or ocm_get_neighbor_pairs widens the rows");
+    launch_neighbor_dump(d_, stream_, g, kNoAllocMaxPairs, group_member_count_[g], dcount, dpairs);
+    sync();
+    int cnt = 0;
+    check_hip(hipMemcpy(&cnt, dcount, sizeof(int), hipMemcpyDeviceToHost), "dump");
+    *count = cnt;
+    if (cnt > kNoAllocMaxPairs)
+        throw Error(OCM_ERR_INVALID, "the force has " + std::to_string(cnt) + " pairs, more than the " +
+                                         std::to_string(kNoAllocMaxPairs) +
+                                         " the no-allocation dump holds; use ocm_get_neighbor_pairs");
+    if (cap == 0) return;
+    if (cap < cnt)
+        throw Error(OCM_ERR_INVALID, "cap " + std::to_string(cap) + " is smaller than the pair count " +
+                                         std::to_string(cnt));
+    if (cnt == 0) return;
+    check_hip(hipMemcpy(out, dpairs, static_cast<size_t>(cnt) * sizeof(int2), hipMemcpyDeviceToHost), "dump");
+    // local indices, sorted in place in the caller's array
+    struct Pair {
+        int32_t i, j;
+    };
+    Pair* v = reinterpret_cast<Pair*>(out);
+    const int base = bead_begin_[s];
+    for (int k = 0; k < cnt; ++k) {
+        v[k].i -= base;
+        v[k].j -= base;
+    }
+    std::sort(v, v + cnt, [](const Pair& a, const Pair& b) { return a.i != b.i ? a.i < b.i : a.j < b.j; });
+}
+
 void Engine::inner_stats(long long* entries, int* prune) {
     sync();
     long long total = 0;

@@ -81,6 +81,10 @@ public:
     std::vector<double> potentials();   // per system, FP64 energy pass on the current list
     void get_state(int s, int what, uint32_t groups, ocm_state* out);
     long long neighbor_pairs(int s, int force_index, int32_t* out, long long cap);
+    // The same pairs through the library's fixed device scratch: no device
+    // allocation (rows are not widened; cap must hold every pair).  *count is
+    // set before any size error is raised.
+    void neighbor_pairs_noalloc(int s, int force_index, int32_t* out, long long cap, long long* count);
     int system_status(int s) const { return sys_status_.at(s); }
     // Continues system s's step count (and its noise stream) from steps_done.
     void set_steps_done(int s, long long steps_done);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 namespace ocm {
 
@@ -2140,6 +2141,39 @@ void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long
 
 void launch_fill_i32(hipStream_t st, int* p, int v, int n) {
     if (n > 0) k_fill_i32<<<nblk(n, 256), 256, 0, st>>>(p, v, n);
+}
+
+// ------------------------------------------- pair dump without allocation
+// Fixed scratch of the no-allocation pair dump: the count word in the first
+// 256 bytes, then up to kNoAllocMaxPairs pairs.  Every GPU the library runs
+// on holds its own copy until the process ends.  In host code `scratch` names
+// a host-side stand-in, not a device address: kernels receive the address
+// hipGetSymbolAddress returns on the device, never the array's name.
+static __device__ __align__(256) int scratch[kNoAllocHeaderInts + 2 * kNoAllocMaxPairs];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, int** count, int2** pairs) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // once per device, with that device current
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<int*>(p);
+    }
+    *count = g_scratch_ptr[device];
+    *pairs = reinterpret_cast<int2*>(g_scratch_ptr[device] + kNoAllocHeaderInts);
+    return hipSuccess;
 }
 
 void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed) {

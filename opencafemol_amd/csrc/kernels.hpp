@@ -117,6 +117,13 @@ void launch_kinetic(const DevPtrs& d, hipStream_t st);
 void launch_neighbor_dump(const DevPtrs& d, hipStream_t st, int group, long long cap, int members,
                           int* count, int2* out);
 void launch_fill_i32(hipStream_t st, int* p, int v, int n);
+// Fixed device scratch of the no-allocation pair dump (a count word, then
+// kNoAllocMaxPairs int2 pairs), one copy per GPU for the life of the process.
+// The first call on a device, with that device current, looks its address up;
+// later calls return the kept address.  Calls that overlap on one GPU share it.
+constexpr int kNoAllocMaxPairs = 1 << 21;
+constexpr int kNoAllocHeaderInts = 64;
+hipError_t noalloc_scratch(int device, int** count, int2** pairs);
 void launch_velocity_noise(const DevPtrs& d, hipStream_t st, int sys, double kT, uint64_t seed);
 
 }  // namespace ocm
