@@ -44,6 +44,8 @@ HBM_PEAK_GBS = 8000.0
 # SIMD every 4 cycles (16 lanes wide); x 64 lanes x 2 flops per FMA this is
 # the published 78.6 TFLOP/s of non-packed vector math.
 VALU_PEAK_GINST = 256 * 4 * 2.4 / 4 * 1000.0
+VALU_CLOCK_HZ = 2.4e9
+VALU_SIMDS = 256 * 4
 KB = 1.380649e-23 * 6.02214076e23 / 1000.0   # kJ/mol/K
 
 
@@ -457,10 +459,14 @@ def single_run(args, workload):
     if vf and os.path.exists(vf):
         vj = json.load(open(vf))
         insts = vj["valu_insts_per_entry"] * read_padded
+        # SIMD issue cycles: 4 per wave64 instruction, 8 for an FP64 add/mul/fma
+        cycles = vj["valu_issue_cycles_per_entry"] * read_padded
+        simd_cycles = avg_us * 1e-6 * VALU_CLOCK_HZ * VALU_SIMDS
         valu = {"bound": "valu", "achieved": insts / (avg_us * 1e-6) / 1e9, "peak": VALU_PEAK_GINST,
                 "unit": "G wave-instructions/s", "frac": insts / (avg_us * 1e-6) / 1e9 / VALU_PEAK_GINST,
+                "issue_cycle_frac": cycles / simd_cycles,
                 "insts_per_launch": insts, "insts_per_entry": vj["valu_insts_per_entry"],
-                "source": os.path.relpath(vf, ROOT)}
+                "f64_share": vj["f64_share_of_valu_insts"], "source": os.path.relpath(vf, ROOT)}
 
     cpu = None
     if not args.no_cpu and len(systems) == 1:

@@ -6,6 +6,7 @@
 #include <iomanip>
 #include <iostream>
 #include <limits>
+#include <map>
 #include <random>
 #include <sstream>
 
@@ -501,6 +502,247 @@ ForceDescriptor LennardJonesAttractiveForceFieldGenerator::generate() const {
     d.cutoff = lj_cutoff(sigmas_, cutoff_ratio_);
     d.switch_distance = 0.9 * d.cutoff;
     std::cerr << "    LennardJonesAttractive        : cutoff distance is " << d.cutoff << " nm" << std::endl;
+    put_exclusions(ignore_list_, d);
+    return d;
+}
+
+// ------------------------------------------- membrane and table potentials
+CappedGoContactForceFieldGenerator::CappedGoContactForceFieldGenerator(const index_pairs_type& indices,
+                                                                       const std::vector<double>& ks,
+                                                                       const std::vector<double>& r0s,
+                                                                       double capping_ratio, bool use_periodic)
+    : indices_(indices), ks_(ks), r0s_(r0s), capping_ratio_(capping_ratio), use_periodic_(use_periodic) {
+    check_sizes("CappedGoContactForceFieldGenerator", indices.size(), ks, "k");
+    check_sizes("CappedGoContactForceFieldGenerator", indices.size(), r0s, "r0");
+    if (!(capping_ratio > 0.0)) throw std::runtime_error("[error] CappedGoContactForceFieldGenerator: capping_ratio must be positive");
+}
+ForceDescriptor CappedGoContactForceFieldGenerator::generate() const {
+    ForceDescriptor d;
+    d.kind = OCM_CAPPED_GO_CONTACT;
+    d.use_periodic = use_periodic_;
+    d.n_terms = static_cast<int>(indices_.size());
+    for (std::size_t t = 0; t < indices_.size(); ++t) {
+        d.indices.push_back(static_cast<int32_t>(indices_[t].first));
+        d.indices.push_back(static_cast<int32_t>(indices_[t].second));
+        d.params.insert(d.params.end(), {ks_[t], r0s_[t], capping_ratio_});
+    }
+    return d;
+}
+
+CosineDihedralForceFieldGenerator::CosineDihedralForceFieldGenerator(const std::vector<indices_type>& indices,
+                                                                     const std::vector<double>& ks,
+                                                                     const std::vector<double>& theta0s,
+                                                                     const std::vector<double>& ns, bool use_periodic)
+    : indices_(indices), ks_(ks), theta0s_(theta0s), ns_(ns), use_periodic_(use_periodic) {
+    check_sizes("CosineDihedralForceFieldGenerator", indices.size(), ks, "k");
+    check_sizes("CosineDihedralForceFieldGenerator", indices.size(), theta0s, "theta0");
+    check_sizes("CosineDihedralForceFieldGenerator", indices.size(), ns, "n");
+}
+ForceDescriptor CosineDihedralForceFieldGenerator::generate() const {
+    ForceDescriptor d;
+    d.kind = OCM_COSINE_DIHEDRAL;
+    d.use_periodic = use_periodic_;
+    d.n_terms = static_cast<int>(indices_.size());
+    for (std::size_t t = 0; t < indices_.size(); ++t) {
+        for (auto i : indices_[t]) d.indices.push_back(static_cast<int32_t>(i));
+        d.params.insert(d.params.end(), {ks_[t], theta0s_[t], ns_[t]});
+    }
+    return d;
+}
+
+// (largest + second largest) of the values that are present, both tracked
+// with <= from the smallest positive double, as the WCA and iSoLF generators
+// do (WeeksChandlerAndersenForceFieldGenerator.cpp:46-66).
+static double two_largest_sum(const std::vector<std::optional<double>>& v) {
+    double mx = std::numeric_limits<double>::min(), second = std::numeric_limits<double>::min();
+    for (const auto& x : v) {
+        if (!x) continue;
+        if (mx <= *x) { second = mx; mx = *x; }
+        else if (second <= *x) second = *x;
+    }
+    return mx + second;
+}
+
+static const double kTwoSixth = std::pow(2.0, 1.0 / 6.0);
+
+WeeksChandlerAndersenForceFieldGenerator::WeeksChandlerAndersenForceFieldGenerator(
+    const std::vector<std::optional<double>>& sigmas, const std::vector<std::optional<double>>& epsilons,
+    const index_pairs_type& ignore, bool use_periodic, const group_pairs_type& ignore_group_pairs,
+    const std::vector<std::optional<std::string>>& group_vec)
+    : sigmas_(sigmas), epsilons_(epsilons), ignore_list_(ignore), use_periodic_(use_periodic),
+      ignore_group_pairs_(ignore_group_pairs), group_vec_(group_vec) {
+    if (sigmas_.size() != epsilons_.size())
+        throw std::runtime_error("[error] WCA: sigma and epsilon sizes differ");
+    for (std::size_t i = 0; i < sigmas_.size(); ++i)
+        if (sigmas_[i].has_value() != epsilons_[i].has_value())
+            throw std::runtime_error("[error] WeeksChandlerAndersenForceFieldGenerator : The parameter set is sigma and "
+                                     "epsilon. incomplete parameter set was given for particle idx " +
+                                     std::to_string(i) + ".");
+}
+ForceDescriptor WeeksChandlerAndersenForceFieldGenerator::generate() const {
+    ForceDescriptor d;
+    d.kind = OCM_WCA;
+    d.use_periodic = use_periodic_;
+    std::vector<bool> has(sigmas_.size());
+    for (std::size_t i = 0; i < sigmas_.size(); ++i) {
+        d.particle_params.push_back(sigmas_[i] ? *sigmas_[i] : 0.0);
+        d.particle_params.push_back(epsilons_[i] ? *epsilons_[i] : 0.0);
+        has[i] = sigmas_[i].has_value();
+    }
+    interaction_categories(has, ignore_group_pairs_, group_vec_, d);
+    d.cutoff = two_largest_sum(sigmas_) * 0.5 * kTwoSixth;
+    std::cerr << "    WeeksChandlerAndersen         : cutoff distance is " << d.cutoff << " nm" << std::endl;
+    put_exclusions(ignore_list_, d);
+    return d;
+}
+
+set_pairs_type uniform_interaction_groups(const std::vector<std::size_t>& former, const std::vector<std::size_t>& latter,
+                                          const group_pairs_type& ignore_group_pairs,
+                                          const std::vector<std::optional<std::string>>& group_vec) {
+    set_pairs_type out;
+    if (ignore_group_pairs.empty()) {
+        out.push_back({std::set<int>(former.begin(), former.end()), std::set<int>(latter.begin(), latter.end())});
+        return out;
+    }
+    // beads whose group is named in an ignore pair go to that name's set,
+    // the rest to "others"; names are visited in sorted order
+    std::set<std::string> names;
+    for (const auto& p : ignore_group_pairs) { names.insert(p.first); names.insert(p.second); }
+    auto split = [&](const std::vector<std::size_t>& members, std::map<std::string, std::set<int>>& named,
+                     std::set<int>& others) {
+        for (const auto& nm : names) named[nm];
+        for (std::size_t i : members) {
+            const bool grouped = i < group_vec.size() && group_vec[i] && names.count(*group_vec[i]);
+            if (grouped) named[*group_vec[i]].insert(static_cast<int>(i));
+            else others.insert(static_cast<int>(i));
+        }
+    };
+    std::map<std::string, std::set<int>> f_named, l_named;
+    std::set<int> f_others, l_others;
+    split(former, f_named, f_others);
+    split(latter, l_named, l_others);
+    auto ignored = [&](const std::string& a, const std::string& b) {
+        return std::find(ignore_group_pairs.begin(), ignore_group_pairs.end(), std::make_pair(a, b)) !=
+                   ignore_group_pairs.end() ||
+               std::find(ignore_group_pairs.begin(), ignore_group_pairs.end(), std::make_pair(b, a)) !=
+                   ignore_group_pairs.end();
+    };
+    if (!f_others.empty()) {
+        for (const auto& kv : l_named)
+            if (!kv.second.empty()) out.push_back({f_others, kv.second});
+        if (!l_others.empty()) out.push_back({f_others, l_others});
+    }
+    for (const auto& fk : f_named) {
+        if (fk.second.empty()) continue;
+        if (!l_others.empty()) out.push_back({fk.second, l_others});
+        for (const auto& lk : l_named)
+            if (!ignored(fk.first, lk.first) && !lk.second.empty()) out.push_back({fk.second, lk.second});
+    }
+    return out;
+}
+
+// Set pairs as categories: beads in the same sets share a category (-1: in
+// no set), and categories a, b interact when some pair holds a in its first
+// set and b in its second, or the reverse.  A pair of beads interacts once
+// however many set pairs name it.
+static void categories_from_set_pairs(std::size_t n, const set_pairs_type& groups, ForceDescriptor& d) {
+    const std::size_t ns = 2 * groups.size();
+    std::vector<std::vector<bool>> sig(n, std::vector<bool>(ns, false));
+    for (std::size_t k = 0; k < groups.size(); ++k) {
+        for (int i : groups[k].first) sig[static_cast<std::size_t>(i)][2 * k] = true;
+        for (int i : groups[k].second) sig[static_cast<std::size_t>(i)][2 * k + 1] = true;
+    }
+    std::vector<std::vector<bool>> cats;
+    d.category.assign(n, -1);
+    for (std::size_t i = 0; i < n; ++i) {
+        if (std::none_of(sig[i].begin(), sig[i].end(), [](bool b) { return b; })) continue;
+        auto it = std::find(cats.begin(), cats.end(), sig[i]);
+        if (it == cats.end()) {
+            if (cats.size() == 32) throw std::runtime_error("[error] interaction groups need more than 32 categories");
+            cats.push_back(sig[i]);
+            it = cats.end() - 1;
+        }
+        d.category[i] = static_cast<int32_t>(it - cats.begin());
+    }
+    const std::size_t nc = std::max<std::size_t>(cats.size(), 1);
+    d.n_categories = static_cast<int>(nc);
+    d.allow.assign(nc * nc, 0);
+    for (std::size_t a = 0; a < cats.size(); ++a)
+        for (std::size_t b = 0; b < cats.size(); ++b)
+            for (std::size_t k = 0; k < groups.size(); ++k)
+                if ((cats[a][2 * k] && cats[b][2 * k + 1]) || (cats[a][2 * k + 1] && cats[b][2 * k])) {
+                    d.allow[a * nc + b] = 1;
+                    break;
+                }
+}
+
+UniformWeeksChandlerAndersenForceFieldGenerator::UniformWeeksChandlerAndersenForceFieldGenerator(
+    std::size_t system_size, double eps, double sigma, const std::vector<std::size_t>& former_group,
+    const std::vector<std::size_t>& latter_group, const index_pairs_type& ignore, bool use_periodic,
+    const group_pairs_type& ignore_group_pairs, const std::vector<std::optional<std::string>>& group_vec)
+    : system_size_(system_size), eps_(eps), sigma_(sigma), ignore_list_(ignore), use_periodic_(use_periodic),
+      former_size_(former_group.size()), latter_size_(latter_group.size()),
+      groups_(uniform_interaction_groups(former_group, latter_group, ignore_group_pairs, group_vec)) {}
+ForceDescriptor UniformWeeksChandlerAndersenForceFieldGenerator::generate() const {
+    ForceDescriptor d;
+    d.kind = OCM_UNIFORM_WCA;
+    d.use_periodic = use_periodic_;
+    d.globals = {sigma_, eps_};
+    categories_from_set_pairs(system_size_, groups_, d);
+    d.cutoff = sigma_ * kTwoSixth;
+    std::cerr << "    UniformWeeksChandlerAndersen  : cutoff distance is " << d.cutoff << " nm" << std::endl;
+    put_exclusions(ignore_list_, d);
+    return d;
+}
+
+UniformLennardJonesAttractiveForceFieldGenerator::UniformLennardJonesAttractiveForceFieldGenerator(
+    std::size_t system_size, double eps, double sigma, double cutoff_ratio, const std::vector<std::size_t>& former_group,
+    const std::vector<std::size_t>& latter_group, const index_pairs_type& ignore, bool use_periodic,
+    const group_pairs_type& ignore_group_pairs, const std::vector<std::optional<std::string>>& group_vec)
+    : system_size_(system_size), eps_(eps), sigma_(sigma), abs_cutoff_(sigma * cutoff_ratio), ignore_list_(ignore),
+      use_periodic_(use_periodic), former_size_(former_group.size()), latter_size_(latter_group.size()),
+      groups_(uniform_interaction_groups(former_group, latter_group, ignore_group_pairs, group_vec)) {}
+ForceDescriptor UniformLennardJonesAttractiveForceFieldGenerator::generate() const {
+    ForceDescriptor d;
+    d.kind = OCM_UNIFORM_LJ_ATTRACTIVE;
+    d.use_periodic = use_periodic_;
+    d.globals = {sigma_, eps_};
+    categories_from_set_pairs(system_size_, groups_, d);
+    d.cutoff = abs_cutoff_;
+    d.switch_distance = 0.9 * abs_cutoff_;
+    std::cerr << "    UniformLennardJonesAttractive : cutoff distance is " << d.cutoff << " nm" << std::endl;
+    put_exclusions(ignore_list_, d);
+    return d;
+}
+
+iSoLFAttractiveForceFieldGenerator::iSoLFAttractiveForceFieldGenerator(
+    const std::vector<std::optional<double>>& sigmas, const std::vector<std::optional<double>>& epsilons,
+    const std::vector<std::optional<double>>& omegas, const index_pairs_type& ignore, bool use_periodic,
+    const group_pairs_type& ignore_group_pairs, const std::vector<std::optional<std::string>>& group_vec)
+    : sigmas_(sigmas), epsilons_(epsilons), omegas_(omegas), ignore_list_(ignore), use_periodic_(use_periodic),
+      ignore_group_pairs_(ignore_group_pairs), group_vec_(group_vec) {
+    if (sigmas_.size() != epsilons_.size() || sigmas_.size() != omegas_.size())
+        throw std::runtime_error("[error] iSoLFAttractive: sigma, epsilon and omega sizes differ");
+    for (std::size_t i = 0; i < sigmas_.size(); ++i)
+        if (sigmas_[i].has_value() != epsilons_[i].has_value() || sigmas_[i].has_value() != omegas_[i].has_value())
+            throw std::runtime_error("[error] iSoLFAttractiveForceFieldGenerator : The parameter set is sigma, epsilon "
+                                     "and omega. incomplete parameter set was given for particle index " +
+                                     std::to_string(i) + ".");
+}
+ForceDescriptor iSoLFAttractiveForceFieldGenerator::generate() const {
+    ForceDescriptor d;
+    d.kind = OCM_ISOLF_ATTRACTIVE;
+    d.use_periodic = use_periodic_;
+    std::vector<bool> has(sigmas_.size());
+    for (std::size_t i = 0; i < sigmas_.size(); ++i) {
+        d.particle_params.push_back(sigmas_[i] ? *sigmas_[i] : 0.0);
+        d.particle_params.push_back(epsilons_[i] ? *epsilons_[i] : 0.0);
+        d.particle_params.push_back(omegas_[i] ? *omegas_[i] : 0.0);
+        has[i] = sigmas_[i].has_value();
+    }
+    interaction_categories(has, ignore_group_pairs_, group_vec_, d);
+    d.cutoff = two_largest_sum(sigmas_) * 0.5 * kTwoSixth + two_largest_sum(omegas_) * 0.5;
+    std::cerr << "    iSoLFAttractive               : cutoff distance is " << d.cutoff << " nm" << std::endl;
     put_exclusions(ignore_list_, d);
     return d;
 }

@@ -125,6 +125,22 @@ private:
     bool use_periodic_;
 };
 
+// CappedGoContactForceFieldGenerator.hpp:20-52: Go 12-10 contacts with a
+// linear cap below capping_ratio * r0.
+class CappedGoContactForceFieldGenerator final : public ForceFieldGeneratorBase {
+public:
+    CappedGoContactForceFieldGenerator(const index_pairs_type& indices, const std::vector<double>& ks,
+                                       const std::vector<double>& r0s, double capping_ratio, bool use_periodic);
+    ForceDescriptor generate() const override;
+    std::string name() const override { return "CappedGoContact"; }
+    std::size_t size() const { return indices_.size(); }
+private:
+    index_pairs_type indices_;
+    std::vector<double> ks_, r0s_;
+    double capping_ratio_;
+    bool use_periodic_;
+};
+
 class HarmonicAngleForceFieldGenerator final : public ForceFieldGeneratorBase {
 public:
     using indices_type = std::array<std::size_t, 3>;
@@ -169,6 +185,22 @@ public:
 private:
     std::vector<indices_type> indices_;
     std::vector<double> ks_, theta0s_, sigmas_;
+    bool use_periodic_;
+};
+
+// CosineDihedralForceFieldGenerator.hpp:22-60: k (1 - cos(n (phi - t0))).
+class CosineDihedralForceFieldGenerator final : public ForceFieldGeneratorBase {
+public:
+    using indices_type = std::array<std::size_t, 4>;
+    CosineDihedralForceFieldGenerator(const std::vector<indices_type>& indices, const std::vector<double>& ks,
+                                      const std::vector<double>& theta0s, const std::vector<double>& ns,
+                                      bool use_periodic);
+    ForceDescriptor generate() const override;
+    std::string name() const override { return "CosineDihedral"; }
+    std::size_t size() const { return indices_.size(); }
+private:
+    std::vector<indices_type> indices_;
+    std::vector<double> ks_, theta0s_, ns_;
     bool use_periodic_;
 };
 
@@ -262,6 +294,105 @@ public:
 private:
     double cutoff_ratio_;
     std::vector<std::optional<double>> epsilons_, sigmas_;
+    index_pairs_type ignore_list_;
+    bool use_periodic_;
+    group_pairs_type ignore_group_pairs_;
+    std::vector<std::optional<std::string>> group_vec_;
+};
+
+// WeeksChandlerAndersenForceFieldGenerator.hpp:15-45: per-bead sigma and
+// epsilon, cutoff 2^(1/6) (largest + second largest sigma) / 2.
+class WeeksChandlerAndersenForceFieldGenerator final : public ForceFieldGeneratorBase {
+public:
+    WeeksChandlerAndersenForceFieldGenerator(const std::vector<std::optional<double>>& sigmas,
+                                             const std::vector<std::optional<double>>& epsilons,
+                                             const index_pairs_type& ignore_list, bool use_periodic,
+                                             const group_pairs_type& ignore_group_pairs = {},
+                                             const std::vector<std::optional<std::string>>& group_vec = {});
+    ForceDescriptor generate() const override;
+    std::string name() const override { return "WeeksChandlerAndersen"; }
+private:
+    std::vector<std::optional<double>> sigmas_, epsilons_;
+    index_pairs_type ignore_list_;
+    bool use_periodic_;
+    group_pairs_type ignore_group_pairs_;
+    std::vector<std::optional<std::string>> group_vec_;
+};
+
+// Interaction groups as pairs of bead sets, each pair meaning "every bead of
+// the first set with every bead of the second" (OpenMM addInteractionGroup).
+using set_pairs_type = std::vector<std::pair<std::set<int>, std::set<int>>>;
+
+// The set pairs of a table potential between two named bead kinds, split by
+// the groups `ignore.group` names (UniformWeeksChandlerAndersen-
+// ForceFieldGenerator.cpp:21-131; the same rule in UniformLennardJones-
+// AttractiveForceFieldGenerator.cpp:36-139).
+set_pairs_type uniform_interaction_groups(const std::vector<std::size_t>& former, const std::vector<std::size_t>& latter,
+                                          const group_pairs_type& ignore_group_pairs,
+                                          const std::vector<std::optional<std::string>>& group_vec);
+
+// UniformWeeksChandlerAndersenForceFieldGenerator.hpp:17-52: one sigma and
+// epsilon between the beads of two named kinds, cutoff 2^(1/6) sigma.
+class UniformWeeksChandlerAndersenForceFieldGenerator final : public ForceFieldGeneratorBase {
+public:
+    UniformWeeksChandlerAndersenForceFieldGenerator(std::size_t system_size, double eps, double sigma,
+                                                    const std::vector<std::size_t>& former_group,
+                                                    const std::vector<std::size_t>& latter_group,
+                                                    const index_pairs_type& ignore_list, bool use_periodic,
+                                                    const group_pairs_type& ignore_group_pairs = {},
+                                                    const std::vector<std::optional<std::string>>& group_vec = {});
+    ForceDescriptor generate() const override;
+    // the reference spells this name so (.hpp:36); force groups follow it
+    std::string name() const override { return "UniformWeeksChadlerAndersen"; }
+    std::size_t former_group_size() const { return former_size_; }
+    std::size_t latter_group_size() const { return latter_size_; }
+private:
+    std::size_t system_size_;
+    double eps_, sigma_;
+    index_pairs_type ignore_list_;
+    bool use_periodic_;
+    std::size_t former_size_, latter_size_;
+    set_pairs_type groups_;
+};
+
+// UniformLennardJonesAttractiveForceFieldGenerator.hpp:17-55: switched LJ
+// attraction with one sigma and epsilon between two named kinds; cutoff
+// cutoff_ratio * sigma, switching from 0.9 of it.
+class UniformLennardJonesAttractiveForceFieldGenerator final : public ForceFieldGeneratorBase {
+public:
+    UniformLennardJonesAttractiveForceFieldGenerator(std::size_t system_size, double eps, double sigma,
+                                                     double cutoff_ratio, const std::vector<std::size_t>& former_group,
+                                                     const std::vector<std::size_t>& latter_group,
+                                                     const index_pairs_type& ignore_list, bool use_periodic,
+                                                     const group_pairs_type& ignore_group_pairs = {},
+                                                     const std::vector<std::optional<std::string>>& group_vec = {});
+    ForceDescriptor generate() const override;
+    std::string name() const override { return "UniformLennardJonesAttractive"; }
+    std::size_t former_group_size() const { return former_size_; }
+    std::size_t latter_group_size() const { return latter_size_; }
+private:
+    std::size_t system_size_;
+    double eps_, sigma_, abs_cutoff_;
+    index_pairs_type ignore_list_;
+    bool use_periodic_;
+    std::size_t former_size_, latter_size_;
+    set_pairs_type groups_;
+};
+
+// iSoLFAttractiveForceFieldGenerator.hpp:15-48: per-bead sigma, epsilon,
+// omega; cutoff 2^(1/6) (two largest sigma)/2 + (two largest omega)/2.
+class iSoLFAttractiveForceFieldGenerator final : public ForceFieldGeneratorBase {
+public:
+    iSoLFAttractiveForceFieldGenerator(const std::vector<std::optional<double>>& sigmas,
+                                       const std::vector<std::optional<double>>& epsilons,
+                                       const std::vector<std::optional<double>>& omegas,
+                                       const index_pairs_type& ignore_list, bool use_periodic,
+                                       const group_pairs_type& ignore_group_pairs = {},
+                                       const std::vector<std::optional<std::string>>& group_vec = {});
+    ForceDescriptor generate() const override;
+    std::string name() const override { return "iSoLFAttractive"; }
+private:
+    std::vector<std::optional<double>> sigmas_, epsilons_, omegas_;
     index_pairs_type ignore_list_;
     bool use_periodic_;
     group_pairs_type ignore_group_pairs_;

@@ -274,6 +274,49 @@ std::unique_ptr<GoContactForceFieldGenerator> read_toml_go_contact_ff_generator(
     return std::make_unique<GoContactForceFieldGenerator>(idx, ks, r0s, use_periodic);
 }
 
+// ReadTOMLForceFieldGenerator.cpp:216-275: the Go contact's keys plus
+// capping_ratio (default 0.75).
+std::unique_ptr<CappedGoContactForceFieldGenerator> read_toml_capped_go_contact_ff_generator(const Value& ff,
+                                                                                             Topology& topology,
+                                                                                             bool use_periodic) {
+    check_keys_available(ff, {"interaction", "potential", "topology", "parameters", "capping_ratio", "env"});
+    const Value* env = env_of(ff);
+    const double capping_ratio = ff.contains("capping_ratio") ? ff.at("capping_ratio").as_double() : 0.75;
+    index_pairs_type idx;
+    std::vector<double> ks, r0s;
+    for (const auto& p : ff.at("parameters").as_array()) {
+        const auto i = read_indices<2>(p, env, "read_toml_capped_go_contact_ff_generator", topology.size());
+        idx.emplace_back(i[0], i[1]);
+        ks.push_back(num(p, env, "k") * KJ);
+        r0s.push_back(num(p, env, "v0") * NM);
+    }
+    if (!topology_type(ff).empty()) topology.add_edges(idx, topology_type(ff));
+    std::cerr << "    BondLength    : CappedGoContact (" << idx.size() << " found)" << std::endl;
+    return std::make_unique<CappedGoContactForceFieldGenerator>(idx, ks, r0s, capping_ratio, use_periodic);
+}
+
+// ReadTOMLForceFieldGenerator.cpp:549-622: k in kcal/mol, t0 = v0 + pi (the
+// input's convention is k (1 + cos(n (phi - v0)))), integer n.
+std::unique_ptr<CosineDihedralForceFieldGenerator> read_toml_cosine_dihedral_ff_generator(const Value& ff,
+                                                                                           Topology& topology,
+                                                                                           bool use_periodic) {
+    check_keys_available(ff, {"interaction", "potential", "topology", "parameters", "env"});
+    const Value* env = env_of(ff);
+    std::vector<std::array<std::size_t, 4>> idx;
+    std::vector<double> ks, t0s, ns;
+    for (const auto& p : ff.at("parameters").as_array()) {
+        idx.push_back(read_indices<4>(p, env, "read_toml_cosine_dihedral_ff_generator", topology.size()));
+        ks.push_back(num(p, env, "k") * KJ);
+        t0s.push_back(num(p, env, "v0") + constant::pi);
+        const long long n = find_parameter(p, env, "n").as_int();
+        if (n < 0) throw std::runtime_error("[error] read_toml_cosine_dihedral_ff_generator: n must not be negative" + p.where());
+        ns.push_back(static_cast<double>(n));
+    }
+    if (!topology_type(ff).empty()) topology.add_edges(idx, topology_type(ff));
+    std::cerr << "    DihedralAngle : Cosine (" << idx.size() << " found)" << std::endl;
+    return std::make_unique<CosineDihedralForceFieldGenerator>(idx, ks, t0s, ns, use_periodic);
+}
+
 std::unique_ptr<HarmonicAngleForceFieldGenerator> read_toml_harmonic_angle_ff_generator(const Value& ff, Topology& topology,
                                                                                          bool use_periodic) {
     check_keys_available(ff, {"interaction", "potential", "topology", "parameters", "env"});
@@ -443,6 +486,144 @@ std::unique_ptr<LennardJonesRepulsiveForceFieldGenerator> read_toml_lennard_jone
                                                              "LennardJonesRepulsive");
 }
 
+static void read_ignore(const Value& ff, const Topology& topology, index_pairs_type& ignore, group_pairs_type& gp) {
+    if (const Value* ig = ff.find("ignore")) {
+        ignore = read_ignore_molecule_and_particles_within(*ig, topology);
+        gp = read_ignore_group(*ig);
+    }
+}
+
+// ReadTOMLForceFieldGenerator.cpp:1225-1296: per-bead sigma (or "σ", Å) and
+// epsilon (or "ε", kcal/mol).
+std::unique_ptr<WeeksChandlerAndersenForceFieldGenerator> read_toml_weeks_chandler_andersen_ff_generator(
+    const Value& ff, std::size_t n, const Topology& topology, const std::vector<std::optional<std::string>>& groups,
+    bool use_periodic) {
+    check_keys_available(ff, {"interaction", "potential", "ignore", "parameters", "env"});
+    const Value* env = env_of(ff);
+    std::vector<std::optional<double>> sig(n), eps(n);
+    const auto& params = ff.at("parameters").as_array();
+    for (const auto& p : params) {
+        const std::size_t i = read_index(p, env, "read_toml_weeks_chandler_andersen_ff_generator", topology.size());
+        sig[i] = find_parameter_either(p, env, "sigma", "\xCF\x83").as_double() * NM;
+        eps[i] = find_parameter_either(p, env, "epsilon", "\xCE\xB5").as_double() * KJ;
+    }
+    std::cerr << "    Global        : WeeksChandlerAndersen (" << params.size() << " found)" << std::endl;
+    index_pairs_type ignore;
+    group_pairs_type gp;
+    read_ignore(ff, topology, ignore, gp);
+    return std::make_unique<WeeksChandlerAndersenForceFieldGenerator>(sig, eps, ignore, use_periodic, gp, groups);
+}
+
+// ReadTOMLForceFieldGenerator.cpp:1423-1494: per-bead sigma, omega (Å) and
+// epsilon (kcal/mol).
+std::unique_ptr<iSoLFAttractiveForceFieldGenerator> read_toml_isolf_attractive_ff_generator(
+    const Value& ff, std::size_t n, const Topology& topology, const std::vector<std::optional<std::string>>& groups,
+    bool use_periodic) {
+    check_keys_available(ff, {"interaction", "potential", "ignore", "parameters", "env"});
+    const Value* env = env_of(ff);
+    std::vector<std::optional<double>> sig(n), eps(n), om(n);
+    const auto& params = ff.at("parameters").as_array();
+    for (const auto& p : params) {
+        const std::size_t i = read_index(p, env, "read_toml_isolf_attractive_ff_generator", topology.size());
+        sig[i] = num(p, env, "sigma") * NM;
+        eps[i] = num(p, env, "epsilon") * KJ;
+        om[i] = num(p, env, "omega") * NM;
+    }
+    std::cerr << "    Global        : iSoLFAttractive (" << params.size() << " found)" << std::endl;
+    index_pairs_type ignore;
+    group_pairs_type gp;
+    read_ignore(ff, topology, ignore, gp);
+    return std::make_unique<iSoLFAttractiveForceFieldGenerator>(sig, eps, om, ignore, use_periodic, gp, groups);
+}
+
+// The table potentials: for every entry table.<first>.<second> = {sigma,
+// epsilon}, the beads named <first> and <second> in `parameters`.  A pair
+// listed twice is an error; a pair with no bead on one side is skipped.
+template <typename Gen, typename Make>
+static std::vector<std::unique_ptr<Gen>> read_uniform(const Value& ff, std::size_t n, const Topology& topology,
+                                                      const char* who, const char* label, Make make) {
+    const Value* env = env_of(ff);
+    index_pairs_type ignore;
+    group_pairs_type gp;
+    read_ignore(ff, topology, ignore, gp);
+    std::cerr << "    Global        : " << label << std::endl;
+    std::vector<std::pair<std::string, std::string>> treated;
+    std::vector<std::unique_ptr<Gen>> out;
+    const auto& params = ff.at("parameters").as_array();
+    for (const auto& first : ff.at("table").as_table()) {
+        for (const auto& second : first.second.as_table()) {
+            const auto name_pair = std::make_pair(first.first, second.first);
+            if (std::find(treated.begin(), treated.end(), name_pair) != treated.end())
+                throw std::runtime_error("[error] parameter table for " + name_pair.first + "-" + name_pair.second +
+                                         " is defined in dupulicate.");
+            const double sigma = find_parameter_either(second.second, env, "sigma", "\xCF\x83").as_double() * NM;
+            const double epsilon = find_parameter_either(second.second, env, "epsilon", "\xCE\xB5").as_double() * KJ;
+            std::vector<std::size_t> former, latter;
+            for (const auto& p : params) {
+                const std::string& name = p.at("name").as_string();
+                const std::size_t i = read_index(p, env, who, topology.size());
+                if (name == name_pair.first) former.push_back(i);
+                if (name == name_pair.second) latter.push_back(i);
+            }
+            std::cerr << "        " << name_pair.first << "-" << name_pair.second << " (" << former.size() << "-"
+                      << latter.size() << " found)" << std::endl;
+            if (former.empty() || latter.empty()) {
+                std::cerr << "        \033[33m[warning]\033[m this force field generation will be skipped" << std::endl;
+                continue;
+            }
+            out.push_back(make(n, epsilon, sigma, former, latter, ignore, gp));
+            treated.push_back(name_pair);
+        }
+    }
+    return out;
+}
+
+std::vector<std::unique_ptr<UniformWeeksChandlerAndersenForceFieldGenerator>>
+read_toml_uniform_weeks_chandler_andersen_ff_generators(const Value& ff, std::size_t n, const Topology& topology,
+                                                        const std::vector<std::optional<std::string>>& groups,
+                                                        bool use_periodic) {
+    check_keys_available(ff, {"interaction", "potential", "ignore", "parameters", "table", "env"});
+    return read_uniform<UniformWeeksChandlerAndersenForceFieldGenerator>(
+        ff, n, topology, "read_toml_uniform_weeks_chandler_andersen_ff_generator", "UniformWeeksChandlerAndersen",
+        [&](std::size_t sz, double e, double s, const std::vector<std::size_t>& f, const std::vector<std::size_t>& l,
+            const index_pairs_type& ig, const group_pairs_type& gp) {
+            return std::make_unique<UniformWeeksChandlerAndersenForceFieldGenerator>(sz, e, s, f, l, ig, use_periodic,
+                                                                                    gp, groups);
+        });
+}
+
+// cutoff: a ratio on sigma, default 2.5 (ReadTOMLForceFieldGenerator.cpp:1509-1530)
+std::vector<std::unique_ptr<UniformLennardJonesAttractiveForceFieldGenerator>>
+read_toml_uniform_lennard_jones_attractive_ff_generators(const Value& ff, std::size_t n, const Topology& topology,
+                                                         const std::vector<std::optional<std::string>>& groups,
+                                                         bool use_periodic) {
+    check_keys_available(ff, {"interaction", "potential", "ignore", "table", "cutoff", "parameters", "env"});
+    const double ratio = find_double_or(ff, env_of(ff), "cutoff", 2.5);
+    return read_uniform<UniformLennardJonesAttractiveForceFieldGenerator>(
+        ff, n, topology, "read_toml_uniform_lennard_jones_attractive_ff_generator", "UniformLennardJonesAttractive",
+        [&](std::size_t sz, double e, double s, const std::vector<std::size_t>& f, const std::vector<std::size_t>& l,
+            const index_pairs_type& ig, const group_pairs_type& gp) {
+            return std::make_unique<UniformLennardJonesAttractiveForceFieldGenerator>(sz, e, s, ratio, f, l, ig,
+                                                                                     use_periodic, gp, groups);
+        });
+}
+
+// The Gaussian+Cosine dihedral form: each parameter holds a Gaussian and a
+// Cosine table beside its indices; each half becomes its own generator
+// (ReadTOMLInput.cpp:262-292).
+static Value split_dihedral_half(const Value& ff, const char* which) {
+    Value arr = Value::array();
+    for (const auto& elem : ff.at("parameters").as_array()) {
+        Value v = Value::table();
+        for (const auto& kv : elem.at(which).as_table()) v.insert(kv.first, kv.second);
+        v.insert("indices", elem.at("indices"));
+        arr.a->push_back(v);
+    }
+    Value out = Value::table();
+    out.insert("parameters", arr);
+    return out;
+}
+
 // ------------------------------------------------------------ system
 static const Value& first_system(const Value& data) {
     const auto& systems = data.at("systems").as_array();
@@ -514,7 +695,16 @@ SystemGenerator read_toml_system(const Value& data) {
             else if (inter == "DihedralAngle" && pot == "FlexibleLocalDihedral")
                 for (const auto& kv : constant::fld_fourier_table())
                     gen.add_ff_generator(read_toml_flexible_local_dihedral_ff_generator(lf, kv.first, kv.second, topology, use_periodic));
-            else
+            else if (inter == "BondLength" && pot == "CappedGoContact")
+                gen.add_ff_generator(read_toml_capped_go_contact_ff_generator(lf, topology, use_periodic));
+            else if (inter == "DihedralAngle" && pot == "Cosine")
+                gen.add_ff_generator(read_toml_cosine_dihedral_ff_generator(lf, topology, use_periodic));
+            else if (inter == "DihedralAngle" && (pot == "Gaussian+Cosine" || pot == "Cosine+Gaussian")) {
+                gen.add_ff_generator(read_toml_gaussian_dihedral_ff_generator(split_dihedral_half(lf, "Gaussian"), topology,
+                                                                              use_periodic));
+                gen.add_ff_generator(read_toml_cosine_dihedral_ff_generator(split_dihedral_half(lf, "Cosine"), topology,
+                                                                            use_periodic));
+            } else
                 throw std::runtime_error("[error] local potential " + inter + "/" + pot +
                                          " is outside this build's MD-step scope.");
         }
@@ -535,8 +725,20 @@ SystemGenerator read_toml_system(const Value& data) {
                                                                          group_vec, use_periodic));
             } else if (pot == "LennardJonesAttractive") {
                 if (gf.contains("table"))
-                    throw std::runtime_error("[error] table-form LennardJonesAttractive is outside this build's MD-step scope.");
-                gen.add_ff_generator(read_toml_lennard_jones_attractive_ff_generator(gf, n, topology, group_vec, use_periodic));
+                    for (auto& g : read_toml_uniform_lennard_jones_attractive_ff_generators(gf, n, topology, group_vec,
+                                                                                           use_periodic))
+                        gen.add_ff_generator(std::move(g));
+                else
+                    gen.add_ff_generator(read_toml_lennard_jones_attractive_ff_generator(gf, n, topology, group_vec, use_periodic));
+            } else if (pot == "WCA") {
+                if (gf.contains("table"))
+                    for (auto& g : read_toml_uniform_weeks_chandler_andersen_ff_generators(gf, n, topology, group_vec,
+                                                                                          use_periodic))
+                        gen.add_ff_generator(std::move(g));
+                else
+                    gen.add_ff_generator(read_toml_weeks_chandler_andersen_ff_generator(gf, n, topology, group_vec, use_periodic));
+            } else if (pot == "iSoLFAttractive") {
+                gen.add_ff_generator(read_toml_isolf_attractive_ff_generator(gf, n, topology, group_vec, use_periodic));
             } else if (pot == "LennardJonesRepulsive") {
                 gen.add_ff_generator(read_toml_lennard_jones_repulsive_ff_generator(gf, n, topology, group_vec, use_periodic));
             } else {

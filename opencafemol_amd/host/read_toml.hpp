@@ -29,6 +29,25 @@ group_pairs_type read_ignore_group(const Value& ignore);
 std::unique_ptr<HarmonicBondForceFieldGenerator> read_toml_harmonic_bond_ff_generator(const Value& ff, Topology& topology, bool use_periodic);
 std::unique_ptr<GaussianBondForceFieldGenerator> read_toml_gaussian_bond_ff_generator(const Value& ff, Topology& topology, bool use_periodic);
 std::unique_ptr<GoContactForceFieldGenerator> read_toml_go_contact_ff_generator(const Value& ff, Topology& topology, bool use_periodic);
+std::unique_ptr<CappedGoContactForceFieldGenerator> read_toml_capped_go_contact_ff_generator(const Value& ff, Topology& topology, bool use_periodic);
+std::unique_ptr<CosineDihedralForceFieldGenerator> read_toml_cosine_dihedral_ff_generator(const Value& ff, Topology& topology, bool use_periodic);
+std::unique_ptr<WeeksChandlerAndersenForceFieldGenerator> read_toml_weeks_chandler_andersen_ff_generator(
+    const Value& ff, std::size_t n, const Topology& topology, const std::vector<std::optional<std::string>>& groups,
+    bool use_periodic);
+std::unique_ptr<iSoLFAttractiveForceFieldGenerator> read_toml_isolf_attractive_ff_generator(
+    const Value& ff, std::size_t n, const Topology& topology, const std::vector<std::optional<std::string>>& groups,
+    bool use_periodic);
+// One generator per (first, second) entry of the potential's `table`, in the
+// table's order; pairs with an empty side are skipped with a warning
+// (ReadTOMLInput.cpp:435-500 for WCA, :546-611 for LJ attraction).
+std::vector<std::unique_ptr<UniformWeeksChandlerAndersenForceFieldGenerator>>
+read_toml_uniform_weeks_chandler_andersen_ff_generators(const Value& ff, std::size_t n, const Topology& topology,
+                                                        const std::vector<std::optional<std::string>>& groups,
+                                                        bool use_periodic);
+std::vector<std::unique_ptr<UniformLennardJonesAttractiveForceFieldGenerator>>
+read_toml_uniform_lennard_jones_attractive_ff_generators(const Value& ff, std::size_t n, const Topology& topology,
+                                                         const std::vector<std::optional<std::string>>& groups,
+                                                         bool use_periodic);
 std::unique_ptr<HarmonicAngleForceFieldGenerator> read_toml_harmonic_angle_ff_generator(const Value& ff, Topology& topology, bool use_periodic);
 std::unique_ptr<FlexibleLocalAngleForceFieldGenerator> read_toml_flexible_local_angle_ff_generator(
     const Value& ff, const std::string& aa_type, const std::array<double, 10>& table, Topology& topology, bool use_periodic);

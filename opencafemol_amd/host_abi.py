@@ -78,7 +78,8 @@ def _load(loader, path):
                 fs.params = (_arr(f.params, npar * f.n_terms, np.float64) if f.n_terms else np.zeros(0)).reshape(-1, npar)
             else:
                 npp = abi.PAIR_NPP[f.kind]
-                fs.particle_params = _arr(f.particle_params, npp * n, np.float64).reshape(n, npp)
+                fs.particle_params = (_arr(f.particle_params, npp * n, np.float64).reshape(n, npp) if npp
+                                      else np.zeros((n, 0)))
                 fs.cutoff = f.cutoff
                 fs.switch_distance = f.switch_distance
                 if f.category:
@@ -98,7 +99,9 @@ def _load(loader, path):
         groups = {}
         for name in ("HarmonicBond", "GaussianBond", "GoContact", "HarmonicAngle", "FlexibleLocalAngle",
                      "GaussianDihedral", "FlexibleLocalDihedral", "ExcludedVolume", "DebyeHuckel",
-                     "LennardJonesRepulsive", "LennardJonesAttractive"):
+                     "LennardJonesRepulsive", "LennardJonesAttractive", "CappedGoContact", "CosineDihedral",
+                     "WeeksChandlerAndersen", "UniformWeeksChadlerAndersen", "UniformLennardJonesAttractive",
+                     "iSoLFAttractive"):
             g = L.ocmh_group_of(h, name.encode())
             if g >= 0:
                 groups[name] = g

@@ -164,7 +164,7 @@ def test_hps_input_conversions(tmp_path):
 @pytest.mark.parametrize("mutate,needle", [
     (lambda t: t.replace("{indices = [0, 1], v0", "{indices = [0, 9], v0"), "exceeds the system's largest index"),
     (lambda t: t.replace('attributes.ionic_strength = 0.1\n', ''), "ionic_strength"),
-    (lambda t: t.replace('potential = "DebyeHuckel"', 'potential = "iSoLFAttractive"'), "outside this build"),
+    (lambda t: t.replace('potential = "DebyeHuckel"', 'potential = "3SPN2ExcludedVolume"'), "outside this build"),
     (lambda t: t.replace('ignore.particles_within.bond = 3', 'ignore.molecule = "Others"'), "Others"),
     (lambda t: t.replace('attributes.temperature = 300.0', 'attributes.temperature = 300.0\nattributes.ensemble.type = "NPT"'), "barostat"),
     (lambda t: t.replace('k = "k"}', 'k = "kk"}'), "named variable"),
@@ -212,3 +212,84 @@ def test_cli_rejects_bad_usage():
     exe = os.path.join(os.path.dirname(host_abi.LIB_PATH), "..", "bin", "ocm_run")
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_membrane_and_table_potentials(tmp_path):
+    """WCA, iSoLF, the table (uniform) WCA and LJ attraction, capped Go
+    contacts, cosine and Gaussian+Cosine dihedrals through the C++ readers
+    (ReadTOMLInput.cpp:212-292, :435-611; ReadTOMLForceFieldGenerator.cpp
+    :216-275, :549-622, :1225-1368, :1423-1580)."""
+    from tests import membrane_input
+    n_lip = 27
+    s, integ, x, v = host_abi.load_toml(_write(tmp_path, membrane_input.toml_text(n_lip)))
+    n = 3 * n_lip + 6
+    assert s.n == n
+    kinds = _by_kind(s)
+    A, KJ = 0.1, 4.184
+    two6 = 2.0 ** (1.0 / 6.0)
+    (cg,) = kinds[abi.CAPPED_GO_CONTACT]
+    np.testing.assert_allclose(cg.params, [[0.3 * KJ, 2.0, 0.6], [0.5 * KJ, 1.1, 0.6]], rtol=1e-12)
+    cos = kinds[abi.COSINE_DIHEDRAL]
+    assert len(cos) == 2                                     # Cosine, then the Cosine half of Gaussian+Cosine
+    np.testing.assert_allclose(cos[0].params, [[-0.4 * KJ, -1.0 + np.pi, 1], [0.3 * KJ, 0.5 + np.pi, 3]], rtol=1e-12)
+    np.testing.assert_allclose(cos[1].params, [[0.5 * KJ, -0.2 + np.pi, 2]], rtol=1e-12)
+    (gd,) = kinds[abi.GAUSSIAN_DIHEDRAL]
+    np.testing.assert_allclose(gd.params, [[-1.0 * KJ, 0.6, 0.3]], rtol=1e-12)
+    assert gd.indices.tolist() == cos[1].indices.tolist()
+    (wca,) = kinds[abi.WCA]
+    head = np.arange(n) % 3 == 0
+    lipid = np.arange(n) < 3 * n_lip
+    want_sig = np.where(head, 4.62215, 7.111) * A
+    assert wca.particle_params[lipid, 0] == pytest.approx(want_sig[lipid])
+    assert wca.particle_params[lipid, 1] == pytest.approx(np.full(lipid.sum(), 0.416 * KJ))
+    assert wca.cutoff == pytest.approx(7.111 * A * two6)     # both largest sigmas are tails
+    assert wca.category.tolist() == [0] * (3 * n_lip) + [-1] * 6
+    # bond + angle neighbours within a lipid: (0,1), (0,2), (1,2)
+    assert sorted(map(tuple, wca.exclusions.tolist()))[:3] == [(0, 1), (0, 2), (1, 2)]
+    (iso,) = kinds[abi.ISOLF_ATTRACTIVE]
+    assert iso.cutoff == pytest.approx(7.111 * A * two6 + 9.867 * A)
+    assert iso.category.tolist() == [-1 if (i % 3 == 0 or i >= 3 * n_lip) else 0 for i in range(n)]
+    assert iso.particle_params[1].tolist() == pytest.approx([7.111 * A, 0.416 * KJ, 9.867 * A])
+    # the table WCA: H-T only (C-H names no C bead in `parameters`: skipped)
+    (uw,) = kinds[abi.UNIFORM_WCA]
+    assert uw.globals.tolist() == pytest.approx([5.5 * A, 0.3 * KJ])
+    assert uw.cutoff == pytest.approx(5.5 * A * two6)
+    cat = uw.category
+    assert len(set(cat[head & lipid])) == 1 and len(set(cat[~head & lipid])) == 1 and (cat[~lipid] == -1).all()
+    h, t = cat[0], cat[1]
+    assert uw.allow[h, t] == 1 and uw.allow[t, h] == 1 and uw.allow[h, h] == 0 and uw.allow[t, t] == 0
+    (ul,) = kinds[abi.UNIFORM_LJ_ATTRACTIVE]
+    assert ul.globals.tolist() == pytest.approx([6.0 * A, 0.2 * KJ])
+    assert ul.cutoff == pytest.approx(2.5 * 6.0 * A) and ul.switch_distance == pytest.approx(0.9 * ul.cutoff)
+    assert (ul.category[~head & lipid] == ul.category[1]).all() and (ul.category[head | ~lipid] == -1).all()
+    assert ul.allow[ul.category[1], ul.category[1]] == 1
+    assert s.group_ids["UniformWeeksChadlerAndersen"] != s.group_ids["WeeksChandlerAndersen"]
+    assert s.group_ids["CosineDihedral"] != s.group_ids["GaussianDihedral"]
+
+
+def test_uniform_groups_split_by_ignore_group(tmp_path):
+    """table potentials with ignore.group: beads of named groups form their
+    own sets, and pairs of ignored group pairs do not interact
+    (UniformWeeksChandlerAndersenForceFieldGenerator.cpp:37-131)."""
+    from tests import membrane_input
+    # lipids 0-3 (beads 0-11) in group "up", 4-7 (beads 12-23) in "down"
+    out, k = [], 0
+    for ln in membrane_input.toml_text(8).split("\n"):
+        if ln.startswith("{m = 100.0") and k < 24:
+            grp = "up" if k < 12 else "down"
+            body = ln.rstrip(",")
+            ln = body[:-1] + f', group = "{grp}"}}' + ("," if ln.endswith(",") else "")
+            k += 1
+        out.append(ln)
+    text = "\n".join(out).replace('table.H.T = {sigma = 5.5, epsilon = 0.3}',
+                                   'table.H.T = {sigma = 5.5, epsilon = 0.3}\nignore.group.inter = [["up", "down"]]')
+    s, *_ = host_abi.load_toml(_write(tmp_path, text))
+    (uw,) = [f for f in s.forces if f.kind == abi.UNIFORM_WCA]
+    cat, allow = uw.category, uw.allow
+    def inter(i, j):
+        return cat[i] >= 0 and cat[j] >= 0 and allow[cat[i], cat[j]] == 1
+    # up head (0) with up tail (1) and with down tail (13); down head (12) with up tail (1)
+    assert inter(0, 1) and inter(12, 13)
+    assert not inter(0, 13) and not inter(13, 0)     # up-down ignored
+    assert not inter(12, 1)                          # ignored in either order
+    assert not inter(0, 3) and not inter(1, 2)       # H-H and T-T are not in the H-T table

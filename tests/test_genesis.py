@@ -296,3 +296,27 @@ def test_without_iseed_the_seed_is_drawn(tmp_path):
     inp = sh3_genesis.write(str(tmp_path), iseed=None)
     seeds = {host_abi.load_genesis(inp)[1].seed for _ in range(3)}
     assert 0 not in seeds and len(seeds) == 3
+
+
+def test_reference_sample_fixtures_load_and_match_the_readers():
+    """The committed fixtures of the reference's samples (used by the GPU
+    tests) rebuild the systems the readers give for the sample files."""
+    from tests.golden import reference_inputs
+    for name, (kind, path) in reference_inputs.SAMPLES.items():
+        s, it, x, v = reference_inputs.load(name)
+        assert x.shape == (s.n, 3) and len(s.forces) > 0
+        if not os.path.exists(path):
+            continue
+        loader = host_abi.load_toml if kind == "toml" else host_abi.load_genesis
+        r, rit, rx, rv = loader(path)
+        assert np.array_equal(r.masses, s.masses) and np.array_equal(rx, x)
+        assert (rit.temperature, rit.friction, rit.dt, rit.seed) == (it.temperature, it.friction, it.dt, it.seed)
+        assert len(r.forces) == len(s.forces)
+        for a, b in zip(r.forces, s.forces):
+            assert (a.kind, a.group, a.use_periodic, a.cutoff, a.switch_distance) == \
+                (b.kind, b.group, b.use_periodic, b.cutoff, b.switch_distance)
+            for fld in reference_inputs.FIELDS:
+                fa, fb = getattr(a, fld), getattr(b, fld)
+                assert (fa is None) == (fb is None), fld
+                if fa is not None:
+                    assert np.array_equal(np.asarray(fa), np.asarray(fb)), fld
