@@ -367,6 +367,21 @@ def hps_condensate(n_chains=1000, chain_len=1000, box=(60.0, 60.0, 600.0), seed=
     return s, it, x
 
 
+def hps_open_cluster(n_chains=150, chain_len=140, extent=(12.0, 12.0), seed=0):
+    """HPS chains (the slab's parameter distributions) with open boundaries:
+    serpentine-packed in a column of the given x-y extent (nm), with every
+    force non-periodic and no box -- an open-boundary system large enough
+    to need a cell grid (21,000 beads by default)."""
+    box = (extent[0], extent[1], 1.0e4)
+    x = serpentine_chains(n_chains, chain_len, box)
+    x[:, 2] -= x[:, 2].min()
+    s, it, _ = hps_slab(n_chains=n_chains, chain_len=chain_len, box=box, seed=seed, positions=x)
+    s.box = None
+    for f in s.forces:
+        f.use_periodic = False
+    return s, it, x
+
+
 def served_job_mix(n_jobs=24, seed=0, steps=(10000, 50000), frame_interval=10000, go_fraction=0.6):
     """Job stream: Go chains of 50-500 beads (fraction go_fraction) and HPS
     slabs of 2,000-20,000 beads, each running a whole number of frame

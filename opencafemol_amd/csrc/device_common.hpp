@@ -74,11 +74,14 @@ struct DevPairGroup {
     int n_types;
     int cell_begin;
     int ncell[3];
-    int reach[3];          // 1 when the cell grid has >= 3 cells along an axis
+    int reach[3];          // 1 when the stencil spans neighbour cells along an axis
     int egroup[kPairKinds];   // energy group of the member force of each kind
     int n_cat;
     int stride;            // list entries reserved per row (multiple of 4)
     int lj_shared;         // LJ repulsive and attractive members share sigma for every type
+    int open_grid;         // open boundary with a cell grid over the members' bounding box,
+                           // laid out again by k_open_grid at every build
+    int grid_cap;          // cells reserved for that grid
     int pad1;
     double rc2;            // cutoff^2 (nm^2)
     double rl2;            // (cutoff + skin)^2
@@ -88,7 +91,8 @@ struct DevPairGroup {
                            // with delta a bound on its per-axis coordinate error
     double rc;
     double box[3], inv_box[3];
-    double cell_scale[3];  // ncell / box
+    double cell_scale[3];  // cells per nm (ncell / box when periodic)
+    double cell_lo[3];     // open grid: the lower corner of cell 0
     // potential constants (FP64; the force pass rounds them to FP32)
     double exv_eps, exv_corr;
     double dh_K, dh_inv_lambda, dh_corr;

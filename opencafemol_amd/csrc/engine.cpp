@@ -14,6 +14,9 @@ namespace ocm {
 // Pair-group members above which a system counts as large: 64-row list-build
 // work items, and chunks capped at two list lifetimes.
 constexpr int kLargeMembers = 1 << 17;
+// Members from which an open-boundary pair group gets a cell grid; below it
+// every member pair is a candidate.
+constexpr int kOpenGridMembers = 2048;
 // Estimated list entries (whole batch) below which no inner list is kept.
 constexpr long long kPruneMinEntries = 1 << 14;
 // Systems in a batch from which the pair kernel's table is split into XCD slices.
@@ -359,6 +362,7 @@ void Engine::pack() {
     std::vector<float> types_f;
     std::vector<double> types_d;
     int ncell_total = 0;
+    int open_grids = 0;
     int max_types = 0;
     force_group_of_.assign(nsys, {});
     for (int s = 0; s < nsys; ++s) {
@@ -556,11 +560,18 @@ void Engine::pack() {
                 const double rho = G.member_count / vol;
                 est_neighbors = static_cast<long long>(G.member_count * (rho * 4.18879 * rl * rl * rl * 3.0 + 8.0));
             } else {
-                for (int c = 0; c < 3; ++c) { G.ncell[c] = 1; G.reach[c] = 0; G.cell_scale[c] = 0; }
+                for (int c = 0; c < 3; ++c) { G.ncell[c] = 1; G.reach[c] = 0; G.cell_scale[c] = 0; G.cell_lo[c] = 0; }
+                // a large open group gets a grid over its bounding box, laid
+                // out at every build (k_open_grid); small ones test all pairs
+                if (G.member_count >= kOpenGridMembers) {
+                    G.open_grid = 1;
+                    G.grid_cap = std::min(std::max(G.member_count / 4, 27), 1 << 22);
+                    ++open_grids;
+                }
                 est_neighbors = static_cast<long long>(G.member_count) * std::min(G.member_count, 256);
             }
             G.cell_begin = ncell_total;
-            ncell_total += G.ncell[0] * G.ncell[1] * G.ncell[2];
+            ncell_total += G.open_grid ? G.grid_cap : G.ncell[0] * G.ncell[1] * G.ncell[2];
             cell_group.resize(ncell_total, gid);
             group_list_estimate_.push_back(est_neighbors);
             group_member_begin_.push_back(G.member_begin);
@@ -582,6 +593,7 @@ void Engine::pack() {
     // items so the build spreads over the device
     d_.tile_chunk = d_.nm > kLargeMembers ? 64 : 16;
     d_.ncell = std::max(ncell_total, 1);
+    d_.open_grids = open_grids;
     d_.ngroups = static_cast<int>(groups.size());
     d_.max_types = std::max(max_types, 1);
 

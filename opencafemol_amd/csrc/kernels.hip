@@ -360,6 +360,76 @@ __global__ void k_pack(int nm, const DevPairGroup* __restrict__ groups,
 }
 
 // ------------------------------------------------------------ list build
+// Open-boundary groups with a cell grid: one workgroup per group finds the
+// members' bounding box and lays the grid over it, cells of rl = cutoff +
+// skin (widened while the grid would exceed the cells reserved for it).
+// Beads that leave the box before the next build are clamped into the edge
+// cells, which keeps every pair within rl in neighbouring cells.
+constexpr int kOpenGridThreads = 256;
+__global__ __launch_bounds__(kOpenGridThreads) void k_open_grid(DevPairGroup* __restrict__ groups,
+                                                                const int* __restrict__ member_bead,
+                                                                const double* __restrict__ px,
+                                                                const double* __restrict__ py,
+                                                                const double* __restrict__ pz) {
+    DevPairGroup& G = groups[blockIdx.x];
+    if (!G.open_grid) return;   // uniform over the workgroup
+    __shared__ double red[6][kOpenGridThreads];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int m = G.member_begin + threadIdx.x; m < G.member_begin + G.member_count; m += blockDim.x) {
+        const int b = member_bead[m];
+        const double p[3] = {px[b], py[b], pz[b]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (isfinite(p[d])) { lo[d] = fmin(lo[d], p[d]); hi[d] = fmax(hi[d], p[d]); }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { red[d][threadIdx.x] = lo[d]; red[3 + d][threadIdx.x] = hi[d]; }
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + w]);
+                red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double ext[3], l0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool any = red[d][0] <= red[3 + d][0];
+        l0[d] = any ? red[d][0] : 0.0;
+        ext[d] = any ? red[3 + d][0] - red[d][0] : 0.0;
+    }
+    double h = sqrt(G.rl2);
+    int n[3] = {1, 1, 1};
+    bool fits = false;
+    for (int it = 0; it < 512 && !fits; ++it) {
+        long long total = 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double c = floor(ext[d] / h) + 1.0;
+            n[d] = c < 1.0 ? 1 : (c > 1.0e6 ? 1000000 : static_cast<int>(c));
+            total *= n[d];
+        }
+        fits = total <= G.grid_cap;
+        if (!fits) h *= 1.25;
+    }
+    if (!fits) {   // an extent no finite widening covers: one cell
+        n[0] = n[1] = n[2] = 1;
+        h = 1.0;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        G.ncell[d] = n[d];
+        G.reach[d] = n[d] > 1 ? 1 : 0;
+        G.cell_scale[d] = 1.0 / h;
+        G.cell_lo[d] = l0[d];
+    }
+}
+
 __global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
                               const int* __restrict__ member_group, const int* __restrict__ member_bead,
                               const double* __restrict__ px, const double* __restrict__ py,
@@ -380,6 +450,14 @@ __global__ void k_cell_assign(int nm, const DevPairGroup* __restrict__ groups,
             u -= floor(u);
             int ci = isfinite(u) ? static_cast<int>(u * G.ncell[d]) : 0;
             c[d] = ci < 0 ? 0 : (ci >= G.ncell[d] ? G.ncell[d] - 1 : ci);
+        }
+    } else if (G.open_grid) {
+        const double p[3] = {px[b], py[b], pz[b]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            // clamped into the grid (in FP64, before the conversion)
+            const double u = floor((p[d] - G.cell_lo[d]) * G.cell_scale[d]);
+            c[d] = isfinite(u) ? static_cast<int>(fmin(fmax(u, 0.0), static_cast<double>(G.ncell[d] - 1))) : 0;
         }
     }
     const int cell = G.cell_begin + (c[2] * G.ncell[1] + c[1]) * G.ncell[0] + c[0];
@@ -597,12 +675,18 @@ __global__ __launch_bounds__(256) void k_nl_tiles(
         const int dx = tid % w0 - rch[0], dy = (tid / w0) % w1 - rch[1], dz = tid / (w0 * w1) - rch[2];
         int x = cx + dx, y = cy + dy, z = cz + dz;
         float sx = 0.f, sy = 0.f, sz = 0.f;   // image shift of the cell's beads
-        if (x < 0) { x += nc[0]; sx = -Lf0; } else if (x >= nc[0]) { x -= nc[0]; sx = Lf0; }
-        if (y < 0) { y += nc[1]; sy = -Lf1; } else if (y >= nc[1]) { y -= nc[1]; sy = Lf1; }
-        if (z < 0) { z += nc[2]; sz = -Lf2; } else if (z >= nc[2]) { z -= nc[2]; sz = Lf2; }
+        bool inside = true;
+        if (periodic) {
+            if (x < 0) { x += nc[0]; sx = -Lf0; } else if (x >= nc[0]) { x -= nc[0]; sx = Lf0; }
+            if (y < 0) { y += nc[1]; sy = -Lf1; } else if (y >= nc[1]) { y -= nc[1]; sy = Lf1; }
+            if (z < 0) { z += nc[2]; sz = -Lf2; } else if (z >= nc[2]) { z -= nc[2]; sz = Lf2; }
+        } else {
+            // open grid: stencil cells past its edge hold nothing
+            inside = x >= 0 && x < nc[0] && y >= 0 && y < nc[1] && z >= 0 && z < nc[2];
+        }
         const int c = cbeg + (z * nc[1] + y) * nc[0] + x;
-        st_start[tid] = cell_start[c];
-        st_pre[tid + 1] = cell_start[c + 1] - cell_start[c];
+        st_start[tid] = inside ? cell_start[c] : 0;
+        st_pre[tid + 1] = inside ? cell_start[c + 1] - cell_start[c] : 0;
         st_shift[tid][0] = sx; st_shift[tid][1] = sy; st_shift[tid][2] = sz;
     }
     __syncthreads();
@@ -2024,6 +2108,8 @@ __global__ __launch_bounds__(kSmallScanThreads) void k_scan_cells_small(int ncel
 
 void launch_rebuild(const DevPtrs& d, hipStream_t st) {
     if (d.nm == 0) return;
+    if (d.open_grids > 0)
+        k_open_grid<<<d.ngroups, kOpenGridThreads, 0, st>>>(d.groups, d.member_bead, d.px, d.py, d.pz);
     // cell_count is zero here: the previous build's k_place cleared it
     k_cell_assign<<<nblk(d.nm, 256), 256, 0, st>>>(d.nm, d.groups, d.member_group, d.member_bead, d.px, d.py,
                                                    d.pz, d.member_cell, d.member_rank, d.cell_count,

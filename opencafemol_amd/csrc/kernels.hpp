@@ -17,6 +17,7 @@ struct ForceBuf {
 // nt bonded terms, nm pair-group members (= slots), ncell cells.
 struct DevPtrs {
     int nb = 0, nsys = 0, nt = 0, nm = 0, ncell = 0, ngroups = 0;
+    int open_grids = 0;            // pair groups with an open-boundary cell grid
     DevSys sys0{};                 // host copy of system 0's record (kernel argument when nsys == 1)
     DevSys* sys = nullptr;
     int* sys_of = nullptr;

@@ -10,6 +10,8 @@ the FP64 oracle.
   one batch: step-0 forces and energies of every replica, each replica's
   kinetic temperature over 10,000 steps against its ladder temperature, and
   8 replicas stepped alone against their bits in the batch.
+- Open boundaries at scale: a 21,000-bead HPS cluster with no box, whose
+  pair groups use a cell grid over their bounding box.
 - Served stream: 10 jobs of the 60% Go / 40% HPS mix (HPS slabs of 2,520 to
   17,640 beads), 2,000-6,000 steps each with a frame every 1,000: every frame
   and final state from the server equal the job run alone.
@@ -47,8 +49,8 @@ def _check_pairs_against_tree(c, s, x):
     of the cutoff may fall either way (the tree's distance arithmetic is not
     the kernel's)."""
     from scipy.spatial import cKDTree
-    box = np.array(s.box)
-    xw = np.mod(x, box)
+    box = np.array(s.box) if s.box is not None else None
+    xw = np.mod(x, box) if box is not None else x
     checked = 0
     for fi, f in enumerate(s.forces):
         if f.kind not in (abi.LJ_REPULSIVE, abi.DEBYE_HUCKEL):
@@ -87,6 +89,30 @@ def test_condensate_one_million_beads(oracle):
         Fo, Eo = oracle.evaluate_cells(s, st["positions"])
         assert_forces_close(st["forces"], Fo)
         assert_energies_close(st["energy_by_group"], Eo)
+
+
+def test_open_boundary_cluster_cell_grid(oracle):
+    """21,000 HPS beads with open boundaries: the pair groups get a cell grid
+    over the members' bounding box, laid out again at every build.  Pair
+    sets against a k-d tree and forces against the oracle, at the start and
+    after 400 steps (the grid follows the moving beads)."""
+    s, it, x = configs.hps_open_cluster()
+    assert s.n >= 20_000 and s.box is None
+    with Context(s, it) as c:
+        c.set_positions(x)
+        st = c.get_state(abi.STATE_FORCES | abi.STATE_ENERGY)
+        Fo, Eo = oracle.evaluate_cells(s, x)
+        assert_forces_close(st["forces"], Fo)
+        assert_energies_close(st["energy_by_group"], Eo)
+        assert _check_pairs_against_tree(c, s, x) > 1_000_000
+        c.minimize(tolerance=100.0, max_iterations=50)
+        c.set_velocities_to_temperature(it.temperature, 3)
+        assert c.step(400) == abi.OCM_OK
+        st = c.get_state(abi.STATE_POSITIONS | abi.STATE_FORCES | abi.STATE_ENERGY)
+        Fo, Eo = oracle.evaluate_cells(s, st["positions"])
+        assert_forces_close(st["forces"], Fo)
+        assert_energies_close(st["energy_by_group"], Eo)
+        assert _check_pairs_against_tree(c, s, st["positions"]) > 1_000_000
 
 
 def test_replicas_256_forces_and_ladder_temperatures(oracle):
