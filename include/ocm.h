@@ -320,6 +320,19 @@ int ocm_batch_get_state(ocm_context* ctx, int32_t system, int32_t what,
 int ocm_batch_system_status(ocm_context* ctx, int32_t system);
 int32_t ocm_batch_size(ocm_context* ctx);
 
+/* Checkpoint and resume.  A checkpoint holds one system's positions,
+ * velocities and steps taken; with the integrator's seed these fix the
+ * Philox noise of every later step, so a run resumed from a checkpoint gives
+ * the same bits as one that never stopped.  The file records the seed and
+ * bead count, and reading it into a system with another seed or size is
+ * OCM_ERR_INVALID.  No counterpart in the reference, whose runs cannot be
+ * resumed; the state is what its Context holds between Integrator::step
+ * calls (src/Simulator.cpp:49-76).                                          */
+int ocm_batch_write_checkpoint(ocm_context* ctx, int32_t system, const char* path);
+int ocm_batch_read_checkpoint(ocm_context* ctx, int32_t system, const char* path);
+int ocm_write_checkpoint(ocm_context* ctx, const char* path);   /* system 0 */
+int ocm_read_checkpoint(ocm_context* ctx, const char* path);
+
 /* ----------------------------------------------------------------- server */
 /* A request loop over batches: jobs (one system each) are admitted as job
  * slots and the device-memory budget allow, packed into one batch with the

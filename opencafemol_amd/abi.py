@@ -53,6 +53,7 @@ EXPORTED = [
     "ocm_get_inner_list_stats", "ocm_get_list_counts",
     "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
     "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
+    "ocm_batch_write_checkpoint", "ocm_batch_read_checkpoint", "ocm_write_checkpoint", "ocm_read_checkpoint",
     "ocm_version", "ocm_minimize", "ocm_server_options_default", "ocm_server_create",
     "ocm_server_destroy", "ocm_server_submit", "ocm_server_next_frame", "ocm_server_wait",
     "ocm_server_get_stats",
@@ -151,6 +152,10 @@ def load_library(path=LIB_PATH):
         "ocm_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(State)]),
         "ocm_batch_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, P(State)]),
         "ocm_get_neighbor_pairs": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), C.c_int64, P(C.c_int64)]),
+        "ocm_batch_write_checkpoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
+        "ocm_batch_read_checkpoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
+        "ocm_write_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "ocm_read_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
         "ocm_get_neighbor_pairs_noalloc": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), C.c_int64,
                                                      P(C.c_int64)]),
         "ocm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -75,6 +75,12 @@ class Batch:
     def status(self, s):
         return self.lib.ocm_batch_system_status(self.h, s)
 
+    def write_checkpoint(self, s, path):
+        check(self.lib.ocm_batch_write_checkpoint(self.h, s, str(path).encode()))
+
+    def read_checkpoint(self, s, path):
+        check(self.lib.ocm_batch_read_checkpoint(self.h, s, str(path).encode()))
+
     def state(self, s, what=15, groups=0xFFFFFFFF):
         n = self.systems[s].n
         st = abi.State()

@@ -6,8 +6,12 @@
 #include "engine.hpp"
 #include "server.hpp"
 
+#include <cstdio>
+#include <cstring>
+#include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 struct ocm_context {
     ocm::Engine* eng;
@@ -266,6 +270,83 @@ int ocm_batch_system_status(ocm_context* ctx, int32_t s) {
         return err.code;
     }
 }
+
+// ------------------------------------------------------------ checkpoints
+// File layout (native byte order): "OCMCKPT" + version byte 1, int32 bead
+// count, int32 0, int64 steps taken, uint64 integrator seed, then positions
+// and velocities as n*3 doubles each.
+namespace {
+constexpr char kCkptMagic[8] = {'O', 'C', 'M', 'C', 'K', 'P', 'T', 1};
+
+struct FileCloser {
+    void operator()(std::FILE* f) const { if (f) std::fclose(f); }
+};
+}  // namespace
+
+int ocm_batch_write_checkpoint(ocm_context* ctx, int32_t s, const char* path) {
+    return guarded([&] {
+        auto& e = eng(ctx);
+        check_sys(e, s);
+        need(path, "path");
+        const int n = e.n_particles(s);
+        std::vector<double> x(3 * static_cast<size_t>(n)), v(3 * static_cast<size_t>(n));
+        ocm_state st{};
+        st.positions = x.data();
+        st.velocities = v.data();
+        e.get_state(s, OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0xFFFFFFFFu, &st);
+        std::unique_ptr<std::FILE, FileCloser> f(std::fopen(path, "wb"));
+        if (!f) throw ocm::Error(OCM_ERR_INVALID, std::string("cannot open checkpoint file for writing: ") + path);
+        const int32_t head[2] = {n, 0};
+        const int64_t step = st.step;
+        const uint64_t seed = e.seed(s);
+        bool ok = std::fwrite(kCkptMagic, 1, 8, f.get()) == 8 && std::fwrite(head, 4, 2, f.get()) == 2 &&
+                  std::fwrite(&step, 8, 1, f.get()) == 1 && std::fwrite(&seed, 8, 1, f.get()) == 1 &&
+                  std::fwrite(x.data(), 8, x.size(), f.get()) == x.size() &&
+                  std::fwrite(v.data(), 8, v.size(), f.get()) == v.size();
+        ok = std::fflush(f.get()) == 0 && ok;
+        if (!ok) throw ocm::Error(OCM_ERR_INVALID, std::string("writing checkpoint file failed: ") + path);
+        return OCM_OK;
+    });
+}
+
+int ocm_batch_read_checkpoint(ocm_context* ctx, int32_t s, const char* path) {
+    return guarded([&] {
+        auto& e = eng(ctx);
+        check_sys(e, s);
+        need(path, "path");
+        std::unique_ptr<std::FILE, FileCloser> f(std::fopen(path, "rb"));
+        if (!f) throw ocm::Error(OCM_ERR_INVALID, std::string("cannot open checkpoint file: ") + path);
+        char magic[8];
+        int32_t head[2];
+        int64_t step = 0;
+        uint64_t seed = 0;
+        if (std::fread(magic, 1, 8, f.get()) != 8 || std::memcmp(magic, kCkptMagic, 8) != 0)
+            throw ocm::Error(OCM_ERR_INVALID, std::string("not a checkpoint file of this version: ") + path);
+        if (std::fread(head, 4, 2, f.get()) != 2 || std::fread(&step, 8, 1, f.get()) != 1 ||
+            std::fread(&seed, 8, 1, f.get()) != 1)
+            throw ocm::Error(OCM_ERR_INVALID, std::string("truncated checkpoint file: ") + path);
+        const int n = e.n_particles(s);
+        if (head[0] != n)
+            throw ocm::Error(OCM_ERR_INVALID, "checkpoint holds " + std::to_string(head[0]) + " beads, the system " +
+                                                  std::to_string(n));
+        if (seed != e.seed(s))
+            throw ocm::Error(OCM_ERR_INVALID, "checkpoint seed " + std::to_string(seed) +
+                                                  " differs from the integrator's " + std::to_string(e.seed(s)) +
+                                                  ": the noise stream would not continue");
+        if (step < 0) throw ocm::Error(OCM_ERR_INVALID, "checkpoint step count is negative");
+        std::vector<double> x(3 * static_cast<size_t>(n)), v(3 * static_cast<size_t>(n));
+        if (std::fread(x.data(), 8, x.size(), f.get()) != x.size() ||
+            std::fread(v.data(), 8, v.size(), f.get()) != v.size())
+            throw ocm::Error(OCM_ERR_INVALID, std::string("truncated checkpoint file: ") + path);
+        e.set_positions(s, x.data());
+        e.set_velocities(s, v.data());
+        e.set_steps_done(s, step);
+        return OCM_OK;
+    });
+}
+
+int ocm_write_checkpoint(ocm_context* ctx, const char* path) { return ocm_batch_write_checkpoint(ctx, 0, path); }
+int ocm_read_checkpoint(ocm_context* ctx, const char* path) { return ocm_batch_read_checkpoint(ctx, 0, path); }
 
 int32_t ocm_batch_size(ocm_context* ctx) {
     if (!ctx || !ctx->eng) return 0;

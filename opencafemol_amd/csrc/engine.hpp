@@ -65,6 +65,7 @@ public:
 
     int n_systems() const { return static_cast<int>(sys_.size()); }
     int n_particles(int s) const { return sys_.at(s).n; }
+    uint64_t seed(int s) const { return sys_.at(s).integ.seed; }
 
     void set_positions(int s, const double* xyz);
     void set_velocities(int s, const double* xyz);
