@@ -1367,14 +1367,20 @@ __device__ inline RowType row_type(const float* __restrict__ t) {
 // constants in the specialized row loops, which then hold no branches; the
 // switch polynomial is evaluated at x = max(0, (r - rsw)/w), where x = 0 gives
 // S = 1 and dS = 0, the unswitched value exactly.
+// (1/r is left in *ir_out when the kinds need it, else 0.)
+__device__ __forceinline__ bool pair_force_has_ir(int kinds, bool lsw) {
+    return (kinds & (kDh | kIsolf)) || ((kinds & (kLja | kUlja)) && lsw);
+}
 __device__ __forceinline__ float pair_force(int kinds, bool shared, bool lsw, const PairK<float>& G,
-                                            const RowType& ti, const float* __restrict__ tj, float r2) {
+                                            const RowType& ti, const float* __restrict__ tj, float r2,
+                                            float* ir_out = nullptr) {
     const float ir2 = __builtin_amdgcn_rcpf(r2);
     float f = 0.f, r = 0.f, ir = 0.f;
-    if ((kinds & (kDh | kIsolf)) || ((kinds & (kLja | kUlja)) && lsw)) {
+    if (pair_force_has_ir(kinds, lsw)) {
         ir = __builtin_amdgcn_rsqf(r2);
         r = r2 * ir;
     }
+    if (ir_out) *ir_out = ir;
     if (kinds & kExv) {
         const float s = ti.exv + tj[kTyExv];
         const float q2 = s * s * ir2;
@@ -1620,20 +1626,27 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
             const bool in = s >= 0 && r2 < g.rc2;
             const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
             const float r2f = in ? static_cast<float>(r2) : 1.0f;
-            float f;
+            float f, ir = 0.f;
             if (SPEC == kSpecExvPeriodic || SPEC == kSpecExvOpen) f = pair_force(kExv, false, false, Kf, rt, tfj, r2f);
-            else if (SPEC == kSpecDh) f = pair_force(kDh, false, false, Kf, rt, tfj, r2f);
-            else if (SPEC == kSpecLjShared) f = pair_force(kLjr | kLja, true, true, Kf, rt, tfj, r2f);
+            else if (SPEC == kSpecDh) f = pair_force(kDh, false, false, Kf, rt, tfj, r2f, &ir);
+            else if (SPEC == kSpecLjShared) f = pair_force(kLjr | kLja, true, true, Kf, rt, tfj, r2f, &ir);
             else if (SPEC == kSpecSolo) f = pair_force(kinds & ~(kExv | kDh | kLjr | kLja), false, Kf.lja_switch, Kf, rt,
                                                        tfj, r2f);
             else f = pair_force(kinds & (kExv | kDh | kLjr | kLja), Kf.lj_shared, Kf.lja_switch, Kf, rt, tfj, r2f);
             // |f d| <= |f| r: one range test covers the three components.  A
             // pair beyond the limit is flagged and clamped to it along its
             // direction (so relaxation still pushes the beads apart); NaN
-            // contributes nothing.
+            // contributes nothing.  Where the kinds already took 1/r the clamp
+            // is formed every time and selected, with no branch in the loop.
             const bool ok = f * f * r2f < kPairLimit * kPairLimit;
             acc.bad |= in && !ok;
-            float gg = ok ? f : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
+            float gg;
+            if (SPEC == kSpecDh || SPEC == kSpecLjShared) {
+                const float cl = copysignf(0.999f * kPairLimit, f) * ir;
+                gg = ok ? f : (f == f ? cl : 0.0f);
+            } else {
+                gg = ok ? f : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
+            }
             gg = in ? gg : 0.0f;   // a masked entry adds exactly the bias constant
             // scaling by 2^32 is exact in FP32 (the result stays far below
             // FP32's largest value; a subnormal gg, flushed or not, rounds to

@@ -135,8 +135,10 @@ bool Server::admit_locked() {
         auto& j = queue_.front();
         if (!active_.empty() && static_cast<long long>(active_bytes_ + j->est_bytes) > opt_.hbm_budget_bytes) break;
         active_bytes_ += j->est_bytes;
-        j->t_start = now_ms();
-        waits_.push_back(j->t_start - j->t_submit);
+        if (j->t_start < 0) {   // a job handed back after a failed repack keeps its first admission
+            j->t_start = now_ms();
+            waits_.push_back(j->t_start - j->t_submit);
+        }
         active_.push_back(j);
         queue_.pop_front();
         any = true;
@@ -203,7 +205,7 @@ void Server::repack() {
             engine_->get_state(static_cast<int>(s), OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
             j->v_set = true;
         }
-        release_slots();
+        drain_slots();   // snapshots in flight index the old batch's beads
         engine_.reset();
         engine_bytes_ = 0;
     }
@@ -230,14 +232,20 @@ void Server::repack() {
             engine_->set_steps_done(static_cast<int>(s), j->done);
         }
         const long long nb = engine_->n_beads_total();
-        slots_.resize(kSlots);
-        for (FrameSlot& sl : slots_) {
-            sl.nb = nb;
-            hip_ok(hipMalloc(&sl.dev, 3 * nb * sizeof(double)), "hipMalloc");
-            hip_ok(hipHostMalloc(&sl.host, 3 * nb * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-            hip_ok(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming), "event");
-            hip_ok(hipEventCreateWithFlags(&sl.staged, hipEventDisableTiming), "event");
+        if (slots_.empty() || slots_[0].cap < nb) {
+            // frame slots outlive repacks; they grow (by half again) only when a batch outgrows them
+            release_slots();
+            const long long cap = std::max(nb, nb + nb / 2);
+            slots_.resize(kSlots);
+            for (FrameSlot& sl : slots_) {
+                sl.cap = cap;
+                hip_ok(hipMalloc(&sl.dev, 3 * cap * sizeof(double)), "hipMalloc");
+                hip_ok(hipHostMalloc(&sl.host, 3 * cap * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+                hip_ok(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming), "event");
+                hip_ok(hipEventCreateWithFlags(&sl.staged, hipEventDisableTiming), "event");
+            }
         }
+        for (FrameSlot& sl : slots_) sl.nb = nb;
     } catch (const Error& e) {
         engine_.reset();
         release_slots();

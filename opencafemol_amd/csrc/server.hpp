@@ -69,7 +69,8 @@ private:
         double* host = nullptr;        // pinned
         hipEvent_t copied = nullptr, staged = nullptr;
         bool busy = false;
-        long long nb = 0;
+        long long nb = 0;              // beads of the batch the slot serves
+        long long cap = 0;             // beads the buffers hold
         std::vector<std::pair<long long, long long>> jobs;   // (job, step)
         std::vector<std::pair<int, int>> ranges;              // (bead begin, count)
     };
