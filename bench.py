@@ -163,11 +163,12 @@ def replicas_run(args, steps, warmup, serial_steps):
         el = time.perf_counter() - t0
         if code != abi.OCM_OK:
             raise SystemExit(f"replica batch failed with status {code}")
+        shape = b.launch_shape()
     dt_ps = integs[0].dt
     agg = ns_per_day(len(systems) * steps / el, dt_ps)
     log(f"replicas batched: {steps} steps of {len(systems)} replicas in {el:.3f} s -> {agg:.0f} ns/day aggregate")
     out = {"aggregate_ns_day": agg, "steps": steps, "warmup": warmup, "ms_per_step": el / steps * 1000.0,
-           "timed_s": el, "config": cfg}
+           "timed_s": el, "launch_shape": shape, "config": cfg}
     if serial_steps <= 0:
         return out
     tot = 0.0
@@ -410,6 +411,7 @@ def single_run(args, workload):
     # pair kernel duration over the timed region: first block start to last
     # block end of every launch, on the device wall clock
     entries, members, _ = b.list_stats()
+    shape = b.launch_shape()
     inner_padded, prune = b.inner_list_stats()
     counts = b.list_counts()
     avg_us = stamp_ms / max(stamp_n, 1) * 1000.0
@@ -448,7 +450,8 @@ def single_run(args, workload):
                "pairs_per_s": listed_pairs / (avg_us * 1e-6),
                "directed_entry_bytes_per_launch": directed_bytes,
                "directed_entry_GBs": directed_bytes / (avg_us * 1e-6) / 1e9,
-               "list_rebuilds_timed": rebuilds_timed, "steps_per_rebuild": args.steps / max(rebuilds_timed, 1)}
+               "list_rebuilds_timed": rebuilds_timed, "steps_per_rebuild": args.steps / max(rebuilds_timed, 1),
+               "launch_shape": shape}
 
     traffic = None
     tf = TRAFFIC_FILES.get(cfg["workload"])

@@ -303,6 +303,11 @@ int ocm_get_inner_list_stats(ocm_context* ctx, int64_t* inner_entries,
  * inner list (0 when it is off).  Copies the lists to the host; a diagnostic
  * with no counterpart in the reference.                                    */
 int ocm_get_list_counts(ocm_context* ctx, int64_t* counts);
+/* The pair kernel's autotuned launch shape: lanes per member row, list
+ * entries per lane per round, workgroups of its fixed grid, and the prune's
+ * lanes per row.  A diagnostic with no counterpart in the reference.       */
+int ocm_get_launch_shape(ocm_context* ctx, int32_t* team, int32_t* entries_per_lane,
+                         int32_t* grid, int32_t* prune_team);
 
 /* ------------------------------------------------------------------ batch */
 /* Many independent systems packed into one set of device arrays; every

@@ -75,6 +75,11 @@ class Batch:
     def status(self, s):
         return self.lib.ocm_batch_system_status(self.h, s)
 
+    def launch_shape(self):
+        v = [C.c_int32() for _ in range(4)]
+        check(self.lib.ocm_get_launch_shape(self.h, *[C.byref(x) for x in v]))
+        return {"team": v[0].value, "entries_per_lane": v[1].value, "grid": v[2].value, "prune_team": v[3].value}
+
     def write_checkpoint(self, s, path):
         check(self.lib.ocm_batch_write_checkpoint(self.h, s, str(path).encode()))
 

@@ -211,6 +211,23 @@ int ocm_get_neighbor_pairs_noalloc(ocm_context* ctx, int32_t force_index, int32_
     });
 }
 
+int ocm_get_launch_shape(ocm_context* ctx, int32_t* team, int32_t* entries_per_lane, int32_t* grid,
+                         int32_t* prune_team) {
+    return guarded([&] {
+        need(team, "team");
+        need(entries_per_lane, "entries_per_lane");
+        need(grid, "grid");
+        need(prune_team, "prune_team");
+        int t = 0, u = 0, g = 0, p = 0;
+        eng(ctx).launch_shape(&t, &u, &g, &p);
+        *team = t;
+        *entries_per_lane = u;
+        *grid = g;
+        *prune_team = p;
+        return OCM_OK;
+    });
+}
+
 int ocm_set_timing(ocm_context* ctx, int32_t enable) {
     return guarded([&] {
         eng(ctx).set_timing(enable);

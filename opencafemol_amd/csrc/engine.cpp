@@ -1010,11 +1010,14 @@ void Engine::autotune_team() {
     check_hip(hipEventCreate(&a), "event");
     check_hip(hipEventCreate(&b), "event");
     float best = 1e30f;
-    int best_team = d_.team, best_grid = d_.pair_grid;
-    const int teams[4] = {8, 16, 32, 64};
+    int best_team = d_.team, best_grid = d_.pair_grid, best_u = d_.pair_u;
+    // (lanes per row, entries per lane per round)
+    const int shapes[7][2] = {{8, 4}, {16, 4}, {32, 4}, {64, 4}, {8, 2}, {16, 2}, {32, 2}};
     const int grids[4] = {1024, 2048, 4096, 1 << 30};
-    for (int t : teams) {
+    for (const auto& sh : shapes) {
+        const int t = sh[0];
         build_pair_blocks(t);
+        d_.pair_u = sh[1];
         for (int g : grids) {
             d_.pair_grid = g;
             // the inner list (pruned by the evaluation before) is what most steps read
@@ -1026,7 +1029,7 @@ void Engine::autotune_team() {
             check_hip(hipEventSynchronize(b), "event");
             float ms = 0;
             check_hip(hipEventElapsedTime(&ms, a, b), "event");
-            if (ms < best) { best = ms; best_team = t; best_grid = g; }
+            if (ms < best) { best = ms; best_team = t; best_grid = g; best_u = sh[1]; }
             if (g >= d_.n_pair_blocks) break;
         }
     }
@@ -1052,6 +1055,7 @@ void Engine::autotune_team() {
     (void)hipEventDestroy(b);
     build_pair_blocks(best_team);
     d_.pair_grid = best_grid;
+    d_.pair_u = best_u;
     // the tuning launches may have flagged numeric trouble in the scratch pass only
     check_hip(hipMemsetAsync(d_.err, 0, sizeof(int) * d_.nsys, stream_), "memset");
     sync();
@@ -1593,6 +1597,14 @@ void Engine::list_counts(long long out[4]) {
             for (int e = 0; e < cnt[r]; ++e) out[2 * inner] += col[static_cast<size_t>(start[r]) + e] >= 0;
         }
     }
+}
+
+void Engine::launch_shape(int* team, int* u, int* grid, int* prune_team) {
+    ensure_forces();   // the shape is tuned at the first force evaluation
+    *team = d_.team;
+    *u = d_.pair_u;
+    *grid = pair_grid_size(d_);
+    *prune_team = d_.col_in ? d_.prune_team : 0;
 }
 
 void Engine::list_stats(long long* entries, long long* members, long long* rebuilds) {

@@ -104,6 +104,7 @@ public:
     void list_stats(long long* entries, long long* members, long long* rebuilds);
     void inner_stats(long long* entries, int* prune);
     void list_counts(long long out[4]);
+    void launch_shape(int* team, int* u, int* grid, int* prune_team);
 
 private:
     void pack();

@@ -1574,8 +1574,10 @@ struct RowAcc {
 };
 
 // One member row for lanes `lane` of a team of TEAM: forces of its entries
-// in fixed point (see kFixMagic).
-template <int TEAM, int SPEC>
+// in fixed point (see kFixMagic).  U entries per lane per round (4, or 2:
+// rounds of 2*TEAM entries waste less of a row's last round, with fewer
+// gathers in flight; the autotuner picks).
+template <int TEAM, int SPEC, int U>
 __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const PairK<float>& Kf, const RowType& rt,
                                           const PairRec& ri, int beg, int cnt, int lane,
                                           const PairRec* __restrict__ rec, int nrec, const int* __restrict__ col,
@@ -1592,26 +1594,28 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
     // loads, as in the prune, measured 4-6% slower in this loop)
     const int* crow = col + beg;
     const __amdgpu_buffer_rsrc_t rs = rec_rsrc(rec, nrec);
-    int cn[4];
+    int cn[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
-    for (int base = 0; base < cnt; base += 4 * TEAM) {
-        const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
-        PairRec rjs[4];
+    for (int u = 0; u < U; ++u) cn[u] = lane + u * TEAM < cnt ? crow[lane + u * TEAM] : -1;
+    for (int base = 0; base < cnt; base += U * TEAM) {
+        int cs[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) rjs[u] = load_rec(rs, cs[u]);
+        for (int u = 0; u < U; ++u) cs[u] = cn[u];
+        PairRec rjs[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = base + 4 * TEAM + lane + u * TEAM;
+        for (int u = 0; u < U; ++u) rjs[u] = load_rec(rs, cs[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + U * TEAM + lane + u * TEAM;
             cn[u] = e < cnt ? crow[e] : -1;
         }
         __builtin_amdgcn_sched_barrier(0);
-        acc.nadd += 4;
-        // the four entries are evaluated without branches (an entry that is
+        acc.nadd += U;
+        // the round's entries are evaluated without branches (an entry that is
         // padding or beyond the cutoff is computed at r2 = 1 and masked to
         // zero) so their dependency chains interleave
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             const int s = cs[u];
             const PairRec rj = rjs[u];
             double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
@@ -1776,7 +1780,7 @@ __global__ __launch_bounds__(kPairBlock) void k_prune(int nm, const DevPairGroup
 
 // At least 5 waves per SIMD (<= 96 VGPRs): the record gathers need the latency hiding.
 // SLICED: walk the table in XCD slices (xcd_walk); otherwise a plain stride.
-template <bool ENERGY, int TEAM, bool SLICED>
+template <bool ENERGY, int TEAM, bool SLICED, int U>
 __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_pair(
     const DevPairGroup* __restrict__ groups, const int2* __restrict__ blocks,
     const PairRec* __restrict__ rec, const int* __restrict__ row_start,
@@ -1850,27 +1854,27 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
             else if (kinds & ~(kExv | kDh | kLjr | kLja)) spec = kSpecSolo;
             switch (spec) {
             case kSpecExvPeriodic:
-                force_row<TEAM, kSpecExvPeriodic>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                force_row<TEAM, kSpecExvPeriodic, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                          lds_types, acc);
                 break;
             case kSpecExvOpen:
-                force_row<TEAM, kSpecExvOpen>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                force_row<TEAM, kSpecExvOpen, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                      lds_types, acc);
                 break;
             case kSpecDh:
-                force_row<TEAM, kSpecDh>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                force_row<TEAM, kSpecDh, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                 lds_types, acc);
                 break;
             case kSpecLjShared:
-                force_row<TEAM, kSpecLjShared>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                force_row<TEAM, kSpecLjShared, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                       lds_types, acc);
                 break;
             case kSpecSolo:
-                force_row<TEAM, kSpecSolo>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                force_row<TEAM, kSpecSolo, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                   lds_types, acc);
                 break;
             default:
-                force_row<TEAM, kSpecGeneric>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
+                force_row<TEAM, kSpecGeneric, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                      lds_types, acc);
                 break;
             }
@@ -2163,21 +2167,30 @@ static void pair_dispatch(const DevPtrs& d, hipStream_t st, const ForceBuf& f, u
     const size_t lds = E ? 0 : static_cast<size_t>(d.max_types) * kTypeStride * sizeof(float);
     const BondedArgs bd{d.nt, d.terms, d.sys, d.sys_of, d.px, d.py, d.pz, d.fla, d.fld};
     const dim3 grid(std::min(d.n_pair_blocks, d.pair_grid)), block(kPairBlock);
-#define OCM_PAIR(T)                                                                                       \
+#define OCM_PAIR(T, U)                                                                                    \
     if (d.xcd_sliced)                                                                                     \
-        k_pair<E, T, true><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col, \
-                                                     d.types_f, d.types_d, f, d.energy, groups, d.err,  \
-                                                     d.n_pair_blocks, d.nm, stamp, bd);                   \
+        k_pair<E, T, true, U><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, \
+                                                        col, d.types_f, d.types_d, f, d.energy, groups,   \
+                                                        d.err, d.n_pair_blocks, d.nm, stamp, bd);         \
     else                                                                                                  \
-        k_pair<E, T, false><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, col, \
-                                                      d.types_f, d.types_d, f, d.energy, groups, d.err, \
-                                                      d.n_pair_blocks, d.nm, stamp, bd)
-    switch (d.team) {
-    case 4: OCM_PAIR(4); break;
-    case 8: OCM_PAIR(8); break;
-    case 32: OCM_PAIR(32); break;
-    case 64: OCM_PAIR(64); break;
-    default: OCM_PAIR(16); break;
+        k_pair<E, T, false, U><<<grid, block, lds, st>>>(d.groups, d.pair_blocks, d.rec, d.row_start, count, \
+                                                         col, d.types_f, d.types_d, f, d.energy, groups,  \
+                                                         d.err, d.n_pair_blocks, d.nm, stamp, bd)
+    // the energy pass has its own loop and always runs the 4-entry instance
+    if (!E && d.pair_u == 2) {
+        switch (d.team) {
+        case 8: OCM_PAIR(8, 2); break;
+        case 32: OCM_PAIR(32, 2); break;
+        default: OCM_PAIR(16, 2); break;
+        }
+    } else {
+        switch (d.team) {
+        case 4: OCM_PAIR(4, 4); break;
+        case 8: OCM_PAIR(8, 4); break;
+        case 32: OCM_PAIR(32, 4); break;
+        case 64: OCM_PAIR(64, 4); break;
+        default: OCM_PAIR(16, 4); break;
+        }
     }
 #undef OCM_PAIR
 }

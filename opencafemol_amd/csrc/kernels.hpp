@@ -72,6 +72,7 @@ struct DevPtrs {
     int n_pair_blocks = 0;
     int xcd_sliced = 0;            // pair_blocks holds 8 contiguous XCD slices of equal work mix (batches of many systems)
     int team = 16;
+    int pair_u = 4;                // force pass: entries per lane per round (4 or 2)
     int prune_team = 32;           // lanes per row of the inner-list prune
     long long* energy = nullptr;   // nsys * 32 fixed point
     long long* ke = nullptr;       // nsys fixed point
