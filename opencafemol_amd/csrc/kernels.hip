@@ -1,15 +1,21 @@
 // kernels.hip -- gfx950 kernels of the MD step.
 //
 // Step stages (one launch each, covering every system of a batch):
-//   k_integrate      BAOAB: kick (B), drift (A), Ornstein-Uhlenbeck (O), drift (A)
-//   k_pack           refresh the cell-sorted pair records, check list validity
-//   list build       k_cell_assign -> scan -> k_place -> k_nl_count -> scan -> k_nl_fill
-//   k_bonded         all bonded kinds, one thread per term
-//   k_pair           neighbor-list pair forces, a team of lanes per member row
+//   k_integrate      BAOAB: kick (B), drift (A), Ornstein-Uhlenbeck (O), drift
+//                    (A); refreshes the bead's pair records and checks list age
+//   list build       [k_open_grid] -> k_cell_assign -> cell scan -> k_place ->
+//                    k_tile_items -> k_nl_tiles (every rebuild interval)
+//   k_prune          inner list: the outer entries within cutoff + margin
+//                    (every prune interval)
+//   k_pair           list pair forces, a team of lanes per member row, with the
+//                    bonded terms as leading work items of the same launch
 //   k_half_kick      closing half kick at the end of a chunk
+//   k_pack           pair-record refresh outside a step (after set_positions,
+//                    a restore, the minimizer's moves)
+//   k_snapshot / k_status_out   chunk start snapshot and end-of-chunk flags
 // Forces are computed in FP32 from FP64 displacements and accumulated as
 // 64-bit fixed point (2^-32 kJ/mol/nm); positions/velocities are FP64.
-// The energy pass (template ENERGY=true) evaluates terms in FP64.
+// The energy pass (k_pair / k_bonded with ENERGY=true) evaluates terms in FP64.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
