@@ -14,7 +14,8 @@ dt 10 fs, 300 K), with two more keys for the metric's other parts:
     job latency under Poisson arrivals, against serial plain execution.
 
 Slab protocol: build the seeded slab, relax it on the device (ocm_minimize),
-draw Maxwell-Boltzmann velocities, run W untimed steps, capture the step
+draw Maxwell-Boltzmann velocities, equilibrate (2,000 untimed steps, stated
+in the line's config), run W untimed steps, capture the step
 graphs the timed call will launch (ocm_prepare_steps, no stepping), then time
 exactly K steps (one ocm_step call; it returns after the device finished).
 The pair kernel times its own launches on the device clock inside the graphs;
@@ -381,6 +382,12 @@ def single_run(args, workload):
     else:
         for k, v in enumerate(mb_velocities(systems, integs)):
             b.set_velocities(k, v)
+    # equilibration from the packed start (the slab's list shrinks by ~13% over
+    # the first thousand steps), untimed, before the warm-up
+    eq = args.equilibrate if args.equilibrate is not None else (200 if workload == "condensate" else 2000)
+    if eq > 0 and b.step(eq) != abi.OCM_OK:
+        raise SystemExit("equilibration failed")
+    cfg = dict(cfg, equilibration_steps=eq)
     b.set_timing(2)   # pair-kernel duration stamps on the device clock, inside the step graphs
     code = b.step(args.warmup)
     if code != abi.OCM_OK:
@@ -513,6 +520,8 @@ def main():
     ap.add_argument("--no-serial", action="store_true",
                     help="replicas workload: skip the one-at-a-time comparison (the batched timing is unchanged)")
     ap.add_argument("--minimize-iters", type=int, default=3000)
+    ap.add_argument("--equilibrate", type=int, default=None,
+                    help="untimed steps after relaxation, before the warm-up (default 2000; 200 for the condensate)")
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for profilers)")
     ap.add_argument("--skin", type=float, default=None, help="neighbor-list skin (nm); default: library's")
     ap.add_argument("--rebuild-interval", type=int, default=None, help="steps between list rebuilds")
