@@ -161,3 +161,39 @@ def baoab_cells(system, integrator, x, v, step0, nsteps, nthreads=1):
     L.oracle_baoab_cells(C.byref(sd), C.byref(it), _dp(x), _dp(v), step0, nsteps, nthreads)
     del keep
     return x, v
+
+
+def fast_forces(system, x, skin=0.3, nthreads=None):
+    """Forces through the fast CPU step's Verlet lists (built at x)."""
+    L = lib()
+    L.oracle_fast_forces.argtypes = [C.POINTER(abi.SystemDesc), C.POINTER(C.c_double), C.c_double, C.c_int32,
+                                     C.POINTER(C.c_double)]
+    L.oracle_fast_forces.restype = C.c_int
+    sd, keep = system.descriptor()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    F = np.zeros((system.n, 3))
+    code = L.oracle_fast_forces(C.byref(sd), _dp(x), skin, nthreads or min(16, os.cpu_count() or 1), _dp(F))
+    del keep
+    if code != 0:
+        raise ValueError(f"oracle_fast_forces: unsupported system ({code})")
+    return F
+
+
+def baoab_fast(system, integrator, x, v, step0, nsteps, nthreads=1, skin=0.3):
+    """oracle_baoab with the fast CPU force evaluation (Verlet lists with a
+    skin, rebuilt on displacement); returns (x, v, list builds)."""
+    L = lib()
+    L.oracle_baoab_fast.argtypes = [C.POINTER(abi.SystemDesc), C.POINTER(abi.IntegratorDesc), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_int64, C.c_int64, C.c_int32, C.c_double,
+                                    C.POINTER(C.c_int64)]
+    L.oracle_baoab_fast.restype = C.c_int
+    sd, keep = system.descriptor()
+    it = integrator.descriptor()
+    x = np.array(x, dtype=np.float64, order="C")
+    v = np.array(v, dtype=np.float64, order="C")
+    nb = C.c_int64(0)
+    code = L.oracle_baoab_fast(C.byref(sd), C.byref(it), _dp(x), _dp(v), step0, nsteps, nthreads, skin, C.byref(nb))
+    del keep
+    if code != 0:
+        raise ValueError(f"oracle_baoab_fast: unsupported system ({code})")
+    return x, v, nb.value

@@ -501,3 +501,34 @@ def test_finite_differences_membrane_and_capped(oracle):
     s, it, x = configs.capped_go_chain(50, seed=2, capping_ratio=1.0)
     x = x + np.random.default_rng(4).normal(scale=0.02, size=x.shape)
     _fd_check(oracle, s, x, range(0, 50, 3))
+
+
+@pytest.mark.parametrize("which", ["hps", "go", "exv_gas"])
+def test_fast_cpu_step_matches_oracle(oracle, which):
+    """The CPU step timed as the benchmark's baseline (Verlet lists with a
+    skin, both directions per row, no pow) against the oracle: forces equal
+    up to summation order, and the same BAOAB trajectory over 30 steps
+    (with list rebuilds inside) within FP64 rounding."""
+    if which == "hps":
+        s, it, x = configs.hps_slab(n_chains=12, chain_len=30, box=(8.0, 8.0, 30.0), slab_z=6.0, seed=6)
+    elif which == "go":
+        s, it, x = configs.go_chain(70, seed=4)
+        x = x + np.random.default_rng(4).normal(scale=0.02, size=x.shape)
+    else:
+        rng = np.random.default_rng(5)
+        n = 300
+        x = rng.uniform(0, 5.0, size=(n, 3))
+        s = SystemSpec(masses=np.full(n, 80.0), box=(5.0, 5.0, 5.0))
+        s.add(ForceSpec(abi.EXCLUDED_VOLUME, "ExcludedVolume", use_periodic=True,
+                        particle_params=np.full((n, 1), 0.12), globals=np.array([1.0]), cutoff=0.5))
+        from opencafemol_amd.model import IntegratorSpec
+        it = IntegratorSpec(300.0, 0.5, 0.005, 2)
+    Fb, _ = oracle.evaluate(s, x)
+    Ff = oracle.fast_forces(s, x, skin=0.3, nthreads=4)
+    assert np.abs(Ff - Fb).max() <= 1e-9 * np.abs(Fb).max()
+    v = np.random.default_rng(1).normal(scale=0.1, size=x.shape)
+    x1, v1 = oracle.baoab(s, it, x, v, 7, 30)
+    x2, v2, builds = oracle.baoab_fast(s, it, x, v, 7, 30, nthreads=4, skin=0.05)
+    assert builds >= 2          # the small skin forces rebuilds inside the run
+    assert np.abs(x2 - x1).max() <= 1e-9
+    assert np.abs(v2 - v1).max() <= 1e-7 * np.abs(v1).max()
