@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # VALU issue ceiling: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
 # SIMD every 4 cycles (16 lanes wide); x 64 lanes x 2 flops per FMA this is
 # the published 78.6 TFLOP/s of non-packed vector math.
-VALU_PEAK_GINST = 256 * 4 * 2.4 / 4 * 1000.0
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4   # G wave-instructions/s (614.4)
 VALU_CLOCK_HZ = 2.4e9
 VALU_SIMDS = 256 * 4
 KB = 1.380649e-23 * 6.02214076e23 / 1000.0   # kJ/mol/K
