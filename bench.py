@@ -20,8 +20,11 @@ graphs the timed call will launch (ocm_prepare_steps, no stepping), then time
 exactly K steps (one ocm_step call; it returns after the device finished).
 The pair kernel times its own launches on the device clock inside the graphs;
 a second phase with HIP events around every launch gives the other kernels.
-The CPU baseline is the cell-list FP64 CPU step (oracle/liboracle.so, OpenMP)
-on a bounded number of steps of the same relaxed configuration.
+The CPU baseline is the FP64 Verlet-list CPU step (oracle/fast_step.cpp in
+oracle/liboracle.so: SIMD pair rows, OpenMP) on a bounded number of steps of
+the same relaxed configuration, on all usable host threads and on one.
+Batches whose every system has at most 1,024 beads (the Go chain, the
+replicas) step resident: one workgroup per system runs whole chunks.
 """
 import argparse
 import json
@@ -146,6 +149,8 @@ def list_options(args, base=None):
         lopt["rebuild_interval"] = args.rebuild_interval
     if args.prune_interval is not None:
         lopt["prune_interval"] = args.prune_interval
+    if args.resident is not None:
+        lopt["resident"] = args.resident
     return lopt
 
 
@@ -178,11 +183,12 @@ def replicas_run(args, steps, warmup, serial_steps):
         if code != abi.OCM_OK:
             raise SystemExit(f"replica batch failed with status {code}")
         shape = b.launch_shape()
+        resident = b.resident_threads()
     dt_ps = integs[0].dt
     agg = ns_per_day(len(systems) * steps / el, dt_ps)
     log(f"replicas batched: {steps} steps of {len(systems)} replicas in {el:.3f} s -> {agg:.0f} ns/day aggregate")
     out = {"aggregate_ns_day": agg, "steps": steps, "warmup": warmup, "ms_per_step": el / steps * 1000.0,
-           "timed_s": el, "launch_shape": shape, "config": cfg}
+           "timed_s": el, "launch_shape": shape, "resident_threads": resident, "config": cfg}
     if serial_steps <= 0:
         return out
     tot = 0.0
@@ -428,6 +434,23 @@ def single_run(args, workload):
     ns_day = ns_per_day(args.steps / el, dt_ps) * len(systems)
     log(f"timed {args.steps} steps in {el:.4f} s: {ms_per_step * 1000:.2f} us/step, {ns_day:.1f} ns/day")
 
+    resident = b.resident_threads()
+    if resident:
+        # one workgroup per system steps whole chunks (k_resident): no pair
+        # kernel to stamp or break down
+        cpu = None
+        if not args.no_cpu and len(systems) == 1:
+            st = b.state(0, abi.STATE_POSITIONS | abi.STATE_VELOCITIES)
+            cpu = cpu_baseline(systems[0], integs[0], st["positions"], st["velocities"], dt_ps)
+            log(f"cpu baseline {cpu['value']:.2f} ns/day on {cpu['cores']} threads")
+        b.close()
+        return {"metric": METRIC, "value": ns_day, "unit": "ns/day", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic", "config": cfg,
+                "roofline": None, "cpu_baseline": cpu, "timed_s": el,
+                "kernels": {"resident_threads": resident, "us_per_step": ms_per_step * 1000.0,
+                            "note": "resident stepping: k_resident runs each chunk in one workgroup per system"}}
+
     # pair kernel duration over the timed region: first block start to last
     # block end of every launch, on the device wall clock
     entries, members, _ = b.list_stats()
@@ -540,6 +563,8 @@ def main():
     ap.add_argument("--rebuild-interval", type=int, default=None, help="steps between list rebuilds")
     ap.add_argument("--prune-interval", type=int, default=None,
                     help="steps between inner-list prunes (<= 1: off); default: library's")
+    ap.add_argument("--resident", type=int, default=None,
+                    help="resident stepping of small systems: -1 where the batch qualifies (library default), 0 never")
     ap.add_argument("--replica-steps", type=int, default=4000, help="timed steps of the replica batch")
     ap.add_argument("--replica-serial-steps", type=int, default=1000, help="timed steps per replica alone")
     ap.add_argument("--serve-jobs", type=int, default=8)

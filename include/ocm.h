@@ -177,6 +177,13 @@ typedef struct ocm_options {
                                   <= 1: every pair pass reads the full list,
                                   as does a batch whose list is estimated
                                   under 16,384 entries                      */
+    int32_t resident;          /* batches whose every system has <= 1024
+                                  beads (<= 8 nonbonded lists, <= 64
+                                  parameter sets): one workgroup per system
+                                  steps whole chunks on chip, with the same
+                                  bits as the standard path.  -1 (default):
+                                  when the batch qualifies; 0: never;
+                                  1: the same as -1                         */
 } ocm_options;
 
 /* getState selection bits */
@@ -308,6 +315,10 @@ int ocm_get_list_counts(ocm_context* ctx, int64_t* counts);
  * lanes per row.  A diagnostic with no counterpart in the reference.       */
 int ocm_get_launch_shape(ocm_context* ctx, int32_t* team, int32_t* entries_per_lane,
                          int32_t* grid, int32_t* prune_team);
+/* Resident stepping: *threads receives the workgroup size each system steps
+ * in (256, 512 or 1024), or 0 when the batch steps through the standard
+ * path.  A diagnostic with no counterpart in the reference.               */
+int ocm_get_resident(ocm_context* ctx, int32_t* threads);
 
 /* ------------------------------------------------------------------ batch */
 /* Many independent systems packed into one set of device arrays; every

@@ -50,7 +50,7 @@ EXPORTED = [
     "ocm_options_default", "ocm_context_create", "ocm_context_destroy", "ocm_set_positions",
     "ocm_set_velocities", "ocm_set_velocities_to_temperature", "ocm_step", "ocm_prepare_steps", "ocm_get_state",
     "ocm_get_neighbor_pairs", "ocm_get_neighbor_pairs_noalloc", "ocm_set_timing", "ocm_get_timing", "ocm_get_list_stats",
-    "ocm_get_inner_list_stats", "ocm_get_list_counts", "ocm_get_launch_shape",
+    "ocm_get_inner_list_stats", "ocm_get_list_counts", "ocm_get_launch_shape", "ocm_get_resident",
     "ocm_batch_create", "ocm_batch_set_positions", "ocm_batch_set_velocities",
     "ocm_batch_get_state", "ocm_batch_system_status", "ocm_batch_size", "ocm_last_error",
     "ocm_batch_write_checkpoint", "ocm_batch_read_checkpoint", "ocm_write_checkpoint", "ocm_read_checkpoint",
@@ -89,7 +89,7 @@ class IntegratorDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("rebuild_interval", C.c_int32), ("skin", C.c_double),
                 ("team_size", C.c_int32), ("use_graph", C.c_int32), ("chunk_steps", C.c_int32),
-                ("prune_interval", C.c_int32)]
+                ("prune_interval", C.c_int32), ("resident", C.c_int32)]
 
 
 class State(C.Structure):
@@ -164,6 +164,7 @@ def load_library(path=LIB_PATH):
         "ocm_get_inner_list_stats": (C.c_int, [C.c_void_p, P(C.c_int64), P(C.c_int32)]),
         "ocm_get_list_counts": (C.c_int, [C.c_void_p, P(C.c_int64)]),
         "ocm_get_launch_shape": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+        "ocm_get_resident": (C.c_int, [C.c_void_p, P(C.c_int32)]),
         "ocm_batch_system_status": (C.c_int, [C.c_void_p, C.c_int32]),
         "ocm_batch_size": (C.c_int32, [C.c_void_p]),
         "ocm_last_error": (C.c_char_p, []),

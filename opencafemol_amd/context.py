@@ -80,6 +80,12 @@ class Batch:
         check(self.lib.ocm_get_launch_shape(self.h, *[C.byref(x) for x in v]))
         return {"team": v[0].value, "entries_per_lane": v[1].value, "grid": v[2].value, "prune_team": v[3].value}
 
+    def resident_threads(self):
+        """Workgroup size of resident stepping, 0 on the standard path."""
+        v = C.c_int32()
+        check(self.lib.ocm_get_resident(self.h, C.byref(v)))
+        return v.value
+
     def write_checkpoint(self, s, path):
         check(self.lib.ocm_batch_write_checkpoint(self.h, s, str(path).encode()))
 

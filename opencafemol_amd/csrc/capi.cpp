@@ -73,6 +73,7 @@ void ocm_options_default(ocm_options* o) {
     o->use_graph = 1;
     o->chunk_steps = 200;
     o->prune_interval = 5;
+    o->resident = -1;
 }
 
 int ocm_batch_create(const ocm_system_desc* systems, const ocm_integrator_desc* integrators, int32_t n,
@@ -224,6 +225,14 @@ int ocm_get_launch_shape(ocm_context* ctx, int32_t* team, int32_t* entries_per_l
         *entries_per_lane = u;
         *grid = g;
         *prune_team = p;
+        return OCM_OK;
+    });
+}
+
+int ocm_get_resident(ocm_context* ctx, int32_t* threads) {
+    return guarded([&] {
+        need(threads, "threads");
+        *threads = eng(ctx).resident_threads();
         return OCM_OK;
     });
 }

@@ -102,6 +102,33 @@ struct DevPairGroup {
     uint32_t allow[32];    // category allow masks
 };
 
+// Resident stepping (small systems, one workgroup per system, whole chunks
+// on chip): per system, its bonded terms (bead indices local to the system)
+// and its pair groups, whose FP32 type rows are staged together in LDS.
+constexpr int kResMaxBeads = 1024;      // beads per system (local indices fit 10 bits)
+constexpr int kResMaxGroups = 8;        // pair groups per system
+constexpr int kResMaxTypeRows = 64;     // staged type rows per system (6 bits)
+// The in-kernel build tests pairs on FP32 coordinates: periodic systems'
+// wrapped into boxes of at most kResWide nm, open systems' relative to bead 0
+// (a build that finds one beyond kResWide tests that build's pairs in FP64).
+constexpr double kResWide = 1000.0;
+struct ResSys {
+    int sys;          // DevSys index
+    int nb;           // beads
+    int b0;           // first global bead
+    int t0, nt;       // bonded terms [t0, t0 + nt) of the resident term array
+    int g0, ng;       // resident pair groups [g0, g0 + ng)
+    int row0, nrow;   // staged type rows [row0, row0 + nrow) of the resident row array
+};
+struct ResGroup {
+    int gid;          // pair group (DevPairGroup index)
+    int m0, nm;       // members [m0, m0 + nm)
+    int trow;         // the group's first type row within its system's staged rows
+    long long l0;     // list region: nm (nm - 1) / 2 entries from l0
+    float rl2w;       // (cutoff + skin + FP32 margin)^2: the in-kernel build's threshold
+    int pad;
+};
+
 // Per-type parameters: {exv_radius, dh_charge, 0, 0, ljr_eps, ljr_sigma, lja_eps, lja_sigma};
 // the four LJ values sit 16-byte aligned for one vector read.  A group of
 // WCA or iSoLF (which holds no other kind) uses {sigma, epsilon, omega}.

@@ -717,7 +717,127 @@ void Engine::pack() {
     if (team != 4 && team != 8 && team != 16 && team != 32 && team != 64) team = 16;
     build_pair_blocks(team);
     team_tuned_ = opt_.team_size != 0;
+    if (opt_.resident != 0) pack_resident(terms, groups, types_f);
+    // resident batches step outside the pair kernel: no launch shape to tune
+    if (d_.res_n > 0) team_tuned_ = true;
     check_hip(hipDeviceSynchronize(), "upload sync");
+}
+
+// Resident stepping, when every system fits one workgroup: each system's
+// bonded terms with local bead indices, its groups' FP32 type rows staged
+// together, and a list region of nm (nm - 1) / 2 entries per group (every
+// member pair fits, so the in-kernel build never overflows).  Leaves
+// d_.res_n at 0 when some system does not qualify.
+void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<DevPairGroup>& groups,
+                           const std::vector<float>& types_f) {
+    const int nsys = n_systems();
+    int max_n = 1;
+    std::vector<std::vector<int>> sys_groups(nsys);
+    for (int g = 0; g < static_cast<int>(groups.size()); ++g) sys_groups[groups[g].sys].push_back(g);
+    for (int s = 0; s < nsys; ++s) {
+        const HostSystem& H = sys_[s];
+        if (H.n > kResMaxBeads || static_cast<int>(sys_groups[s].size()) > kResMaxGroups) return;
+        int rows = 0;
+        for (int g : sys_groups[s]) {
+            const DevPairGroup& G = groups[g];
+            // the build's FP32 coordinates: wrapped into a box of at most
+            // kResWide nm, or open positions relative to bead 0
+            if ((G.periodic != 0) != H.periodic) return;
+            if (G.periodic && std::max({H.box[0], H.box[1], H.box[2]}) > kResWide) return;
+            rows += G.n_types;
+        }
+        if (rows > kResMaxTypeRows) return;
+        max_n = std::max(max_n, H.n);
+    }
+    std::vector<ResSys> tab(nsys);
+    std::vector<ResGroup> grp;
+    std::vector<BTerm> rterms;
+    std::vector<float> rows;
+    std::vector<std::vector<int>> sys_terms(nsys);
+    for (int t = 0; t < static_cast<int>(terms.size()); ++t) {
+        const int b = terms[t].idx[0];
+        const int s = static_cast<int>(std::upper_bound(bead_begin_.begin(), bead_begin_.end(), b) -
+                                       bead_begin_.begin()) - 1;
+        sys_terms[s].push_back(t);
+    }
+    long long l0 = 0;
+    for (int s = 0; s < nsys; ++s) {
+        ResSys& R = tab[s];
+        R.sys = s;
+        R.nb = sys_[s].n;
+        R.b0 = bead_begin_[s];
+        R.t0 = static_cast<int>(rterms.size());
+        R.nt = static_cast<int>(sys_terms[s].size());
+        for (int t : sys_terms[s]) {   // kind order is kept within the system
+            BTerm T = terms[t];
+            for (int k = 0; k < 4; ++k)
+                if (T.idx[k] >= 0) T.idx[k] -= R.b0;
+            rterms.push_back(T);
+        }
+        R.g0 = static_cast<int>(grp.size());
+        R.ng = static_cast<int>(sys_groups[s].size());
+        R.row0 = static_cast<int>(rows.size() / kTypeStride);
+        int trow = 0;
+        for (int g : sys_groups[s]) {
+            const DevPairGroup& G = groups[g];
+            ResGroup Q{};
+            Q.gid = g;
+            Q.m0 = G.member_begin;
+            Q.nm = G.member_count;
+            Q.trow = trow;
+            Q.l0 = l0;
+            // the FP32 test's threshold: coordinates below kResWide carry an
+            // error under 2.2e-4 nm in a distance, well inside 1e-3 nm
+            const double rlw = std::sqrt(G.rl2) + 1e-3;
+            Q.rl2w = static_cast<float>(rlw * rlw);
+            l0 += std::max<long long>(1, static_cast<long long>(Q.nm) * (Q.nm - 1) / 2);
+            for (int k = 0; k < G.n_types * kTypeStride; ++k)
+                rows.push_back(types_f[static_cast<size_t>(G.type_begin) * kTypeStride + k]);
+            trow += G.n_types;
+            grp.push_back(Q);
+        }
+        R.nrow = trow;
+    }
+    // a step's work (terms, list entries, noise) is dealt over all threads:
+    // 1024 of them keep every thread's share short
+    d_.res_threads = 1024;
+    d_.res_nbs = (max_n + 63) & ~63;
+    // LDS: what one system's step reads, as far as it fits.  Up to one
+    // workgroup per CU the whole 160 KB may go to one; beyond that half, so
+    // that two workgroups share a CU and hide each other's latency.
+    int max_nt = 0;
+    long long max_pairs = 0;
+    for (int s = 0; s < nsys; ++s) {
+        max_nt = std::max(max_nt, tab[s].nt);
+        long long p = 0;
+        for (int g = tab[s].g0; g < tab[s].g0 + tab[s].ng; ++g)
+            p += static_cast<long long>(grp[g].nm) * (grp[g].nm - 1) / 2;
+        max_pairs = std::max(max_pairs, p);
+    }
+    int cus = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, opt_.device) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    const size_t budget = (nsys <= cus ? kResLdsMax : kResLdsMax / 2) - 1024;
+    const size_t fixed = resident_lds_bytes(d_.res_nbs, 0, 0);
+    const size_t room = budget > fixed ? budget - fixed : 0;
+    d_.res_list_cap = static_cast<int>(std::min<long long>({max_pairs, kResListLdsMax,
+                                                            static_cast<long long>(room / sizeof(uint32_t))}));
+    d_.res_term_cap = static_cast<int>(std::min<size_t>(static_cast<size_t>(max_nt),
+                                                        (room - d_.res_list_cap * sizeof(uint32_t)) / sizeof(BTerm)));
+    const size_t lds = resident_lds_bytes(d_.res_nbs, d_.res_term_cap, d_.res_list_cap);
+    check_hip(resident_prepare(d_.res_threads, lds), "hipFuncSetAttribute (resident LDS)");
+    d_.res_tab = dalloc<ResSys>(tab.size()); upload(d_.res_tab, tab);
+    d_.res_grp = dalloc<ResGroup>(std::max<size_t>(grp.size(), 1));
+    if (!grp.empty()) upload(d_.res_grp, grp);
+    d_.res_terms = dalloc<BTerm>(std::max<size_t>(rterms.size(), 1));
+    if (!rterms.empty()) upload(d_.res_terms, rterms);
+    d_.res_rows = dalloc<float>(std::max<size_t>(rows.size(), 1));
+    if (!rows.empty()) upload(d_.res_rows, rows);
+    d_.res_list = dalloc<uint32_t>(static_cast<size_t>(std::max<long long>(l0, 1)));
+    d_.res_n = nsys;
 }
 
 void Engine::build_pair_blocks(int team) {
@@ -828,6 +948,8 @@ void Engine::sync() {
 
 // ------------------------------------------------------------- stepping
 void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
+    rebuild = rebuild || lists_stale_;
+    lists_stale_ = false;
     if (d_.nm > 0) {
         if (rebuild) { timed(3, [&] { launch_rebuild(d_, stream_); }); ++rebuilds_; }
         else timed(4, [&] { launch_pack(d_, stream_); });
@@ -1097,6 +1219,7 @@ void Engine::prepare(long long n) {
     if (n < 0) invalid("negative step count");
     if (n == 0) return;
     ensure_forces();
+    if (d_.res_n > 0) return;   // resident chunks capture nothing
     if (timing_mode_ == 2) ensure_stamps();
     // walk the chunks step(n) would run, with the ages they would leave
     const int age = age_, page = page_, fresh = fresh_;
@@ -1188,10 +1311,38 @@ int Engine::run_chunk(int k) {
     throw Error(OCM_ERR_HIP, "step chunk could not complete");
 }
 
+// Resident stepping: one launch per chunk runs every system's steps on chip;
+// nothing rolls back (each workgroup rebuilds its own list when a bead has
+// moved more than skin/2), so a chunk only reports numeric errors.
+int Engine::step_resident(long long n) {
+    int status = OCM_OK;
+    long long remaining = n;
+    while (remaining > 0) {
+        const int k = static_cast<int>(std::min<long long>(remaining, kResChunk));
+        launch_resident(d_, stream_, k);
+        launch_status_out(d_, stream_, status_host_);
+        sync();
+        const int* err = status_host_ + 1;
+        for (int s = 0; s < d_.nsys; ++s) {
+            if ((err[s] | err_carry_[s]) & kErrNumeric) {
+                sys_status_[s] = OCM_ERR_NUMERIC;
+                status = OCM_ERR_NUMERIC;
+            }
+            err_carry_[s] = 0;
+        }
+        remaining -= k;
+    }
+    // the standard path's lists and records no longer match the positions
+    stale_lists();
+    lists_stale_ = true;
+    return status;
+}
+
 int Engine::step(long long n) {
     if (n < 0) invalid("negative step count");
     if (n == 0) return OCM_OK;
     ensure_forces();
+    if (d_.res_n > 0) return step_resident(n);
     int status = OCM_OK;
     long long remaining = n;
     while (remaining > 0) {
@@ -1206,6 +1357,12 @@ int Engine::step(long long n) {
 // Potential energy of every system at the current positions, by the FP64
 // energy pass over the list the last force evaluation built.
 std::vector<double> Engine::potentials() {
+    if (lists_stale_ && d_.nm > 0) {
+        launch_rebuild(d_, stream_);
+        ++rebuilds_;
+        stale_lists();
+    }
+    lists_stale_ = false;
     launch_zero_energy(d_, stream_);
     launch_bonded(d_, stream_, true, d_.f2, 0xFFFFFFFFu);
     launch_pair(d_, stream_, true, d_.f2, 0xFFFFFFFFu);
@@ -1478,6 +1635,7 @@ int Engine::pair_group_of(int s, int force_index) const {
 
 // Rebuilds the outer list at the current positions; returns the flag word.
 int Engine::rebuild_now() {
+    lists_stale_ = false;
     launch_fill_i32(stream_, d_.flags, 0, 1);
     launch_rebuild(d_, stream_);
     ++rebuilds_;

@@ -105,9 +105,13 @@ public:
     void inner_stats(long long* entries, int* prune);
     void list_counts(long long out[4]);
     void launch_shape(int* team, int* u, int* grid, int* prune_team);
+    int resident_threads() const { return d_.res_n > 0 ? d_.res_threads : 0; }
 
 private:
     void pack();
+    void pack_resident(const std::vector<BTerm>& terms, const std::vector<DevPairGroup>& groups,
+                       const std::vector<float>& types_f);
+    int step_resident(long long n);
     void alloc_device();
     void build_pair_blocks(int team);
     void ensure_forces();
@@ -159,6 +163,11 @@ private:
     std::vector<int> err_carry_;     // flags of evaluations outside a chunk, reported with the next one
     std::vector<int64_t> sys_step_origin_;
     bool forces_dirty_ = true;
+    // resident steps leave the standard path's lists behind the positions:
+    // the next evaluation that reads them rebuilds first
+    bool lists_stale_ = false;
+    static constexpr int kResChunk = 2000;   // resident steps per launch
+    static constexpr long long kResListLdsMax = 8192;   // list entries per system kept in LDS
     int interval_ = 20;
     int chunk_ = 200;           // steps per chunk (chunk_steps, capped for large systems)
     int clean_chunks_ = 0;      // chunks since the interval was last shortened

@@ -246,6 +246,54 @@ __device__ __forceinline__ double half_kick(double v, double h, double f, double
     return fma(h * f, im, v);
 }
 
+// One bead's BAOAB step from its force words F (fixed point), position X and
+// velocity V: the closing half kick of the previous step first when pending,
+// the opening half kick, half a drift, the Ornstein-Uhlenbeck update with the
+// step's noise, half a drift.  Fixed (massless) beads keep X and V.  Returns
+// false when a mobile bead's new position is not finite.
+// g holds the step's three N(0,1) samples (gauss3_fast).
+__device__ __forceinline__ bool baoab_bead_g(const DevSys& S, double im, long long F0, long long F1, long long F2,
+                                             const double (&X)[3], const double (&V)[3], int pending,
+                                             const double (&g)[3], double (&x)[3], double (&v)[3]) {
+    const double f0 = F0 * kFixInv, f1 = F1 * kFixInv, f2 = F2 * kFixInv;
+    double v0 = V[0], v1 = V[1], v2 = V[2];
+    if (pending) {
+        v0 = half_kick(v0, S.half_dt, f0, im);
+        v1 = half_kick(v1, S.half_dt, f1, im);
+        v2 = half_kick(v2, S.half_dt, f2, im);
+    }
+    v0 = half_kick(v0, S.half_dt, f0, im);
+    v1 = half_kick(v1, S.half_dt, f1, im);
+    v2 = half_kick(v2, S.half_dt, f2, im);
+    // every product-sum is an explicit fma, so the bits do not depend on
+    // how the compiler contracts the expression in the kernel it is inlined in
+    double x0 = fma(S.half_dt, v0, X[0]);
+    double x1 = fma(S.half_dt, v1, X[1]);
+    double x2 = fma(S.half_dt, v2, X[2]);
+    const double sig = sqrt(S.noise_kT * im);
+    v0 = fma(S.c1, v0, sig * g[0]);
+    v1 = fma(S.c1, v1, sig * g[1]);
+    v2 = fma(S.c1, v2, sig * g[2]);
+    x0 = fma(S.half_dt, v0, x0);
+    x1 = fma(S.half_dt, v1, x1);
+    x2 = fma(S.half_dt, v2, x2);
+    if (im == 0.0) {
+        x0 = X[0]; x1 = X[1]; x2 = X[2];
+        v0 = V[0]; v1 = V[1]; v2 = V[2];
+    }
+    x[0] = x0; x[1] = x1; x[2] = x2;
+    v[0] = v0; v[1] = v1; v[2] = v2;
+    return im == 0.0 || (isfinite(x0) && isfinite(x1) && isfinite(x2));
+}
+
+__device__ __forceinline__ bool baoab_bead(const DevSys& S, double im, long long F0, long long F1, long long F2,
+                                           const double (&X)[3], const double (&V)[3], int pending, long long step,
+                                           uint32_t lid, double (&x)[3], double (&v)[3]) {
+    double g[3];
+    gauss3_fast(S.seed, static_cast<uint64_t>(step - S.step_origin), lid, g);
+    return baoab_bead_g(S, im, F0, F1, F2, X, V, pending, g, x, v);
+}
+
 // pending != 0: the closing half kick of the previous step is still owed and
 // is applied first.  gstep[pin] is this step's global step; thread 0 writes
 // gstep[pout] = gstep[pin] + 1 for the next step.
@@ -278,35 +326,11 @@ __global__ void k_integrate(int nb, const DevSys* __restrict__ sys, const int* _
     // mobile-bead inputs behind a wait on the mass and a branch)
     asm volatile("" ::"v"(im), "v"(F0), "v"(F1), "v"(F2), "v"(X0), "v"(X1), "v"(X2), "v"(V0), "v"(V1), "v"(V2),
                  "v"(lid));
-    const double f0 = F0 * kFixInv, f1 = F1 * kFixInv, f2 = F2 * kFixInv;
-    double v0 = V0, v1 = V1, v2 = V2;
-    if (pending) {
-        v0 = half_kick(v0, S.half_dt, f0, im);
-        v1 = half_kick(v1, S.half_dt, f1, im);
-        v2 = half_kick(v2, S.half_dt, f2, im);
-    }
-    v0 = half_kick(v0, S.half_dt, f0, im);
-    v1 = half_kick(v1, S.half_dt, f1, im);
-    v2 = half_kick(v2, S.half_dt, f2, im);
-    double x0 = X0 + S.half_dt * v0;
-    double x1 = X1 + S.half_dt * v1;
-    double x2 = X2 + S.half_dt * v2;
-    double g[3];
-    gauss3_fast(S.seed, static_cast<uint64_t>(step - S.step_origin), lid, g);
-    const double sig = sqrt(S.noise_kT * im);
-    v0 = S.c1 * v0 + sig * g[0];
-    v1 = S.c1 * v1 + sig * g[1];
-    v2 = S.c1 * v2 + sig * g[2];
-    x0 += S.half_dt * v0;
-    x1 += S.half_dt * v1;
-    x2 += S.half_dt * v2;
+    double xn[3], vn[3];
+    const bool finite = baoab_bead(S, im, F0, F1, F2, {X0, X1, X2}, {V0, V1, V2}, pending, step, lid, xn, vn);
+    if (!finite) atomicOr(&err[s], kErrNumeric);
     const bool fixed = im == 0.0;
-    if (fixed) {
-        x0 = X0; x1 = X1; x2 = X2;
-        v0 = V0; v1 = V1; v2 = V2;
-    } else if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) {
-        atomicOr(&err[s], kErrNumeric);
-    }
+    const double x0 = xn[0], x1 = xn[1], x2 = xn[2], v0 = vn[0], v1 = vn[1], v2 = vn[2];
     fx[b] = 0; fy[b] = 0; fz[b] = 0;
     px[b] = x0; py[b] = x1; pz[b] = x2;
     vx[b] = v0; vy[b] = v1; vz[b] = v2;
@@ -1579,6 +1603,64 @@ struct RowAcc {
     int bad = 0;
 };
 
+// One list entry of a row: the displacement d = x_j - x_i (minimum image
+// applied in place), and g * 2^32 (g = (dE/dr)/r, so that g d is the force on
+// i) in FP64, or 0 when the entry is padding (valid false) or beyond the
+// cutoff.  *bad is set when a pair inside the cutoff exceeded the per-pair
+// limit.  The expression is symmetric in i and j, so the entry (j, i) gives
+// the same g and the negated d.
+template <int SPEC>
+__device__ __forceinline__ double pair_entry(const RowGeom& g, int kinds, const PairK<float>& Kf, const RowType& rt,
+                                             const float* __restrict__ tfj, bool valid, double& dx, double& dy,
+                                             double& dz, int& bad) {
+    // a group without a box has inverse box 0, for which the minimum
+    // image is the identity
+    if (SPEC == kSpecGeneric || SPEC == kSpecSolo ? g.periodic : SPEC != kSpecExvOpen) {
+        dx = min_image(dx, g.bx, g.ibx);
+        dy = min_image(dy, g.by, g.iby);
+        dz = min_image(dz, g.bz, g.ibz);
+    }
+    const double r2 = dist2(dx, dy, dz);
+    const bool in = valid && r2 < g.rc2;
+    const float r2f = in ? static_cast<float>(r2) : 1.0f;
+    float f, ir = 0.f;
+    if (SPEC == kSpecExvPeriodic || SPEC == kSpecExvOpen) f = pair_force(kExv, false, false, Kf, rt, tfj, r2f);
+    else if (SPEC == kSpecDh) f = pair_force(kDh, false, false, Kf, rt, tfj, r2f, &ir);
+    else if (SPEC == kSpecLjShared) f = pair_force(kLjr | kLja, true, true, Kf, rt, tfj, r2f, &ir);
+    else if (SPEC == kSpecSolo) f = pair_force(kinds & ~(kExv | kDh | kLjr | kLja), false, Kf.lja_switch, Kf, rt,
+                                               tfj, r2f);
+    else f = pair_force(kinds & (kExv | kDh | kLjr | kLja), Kf.lj_shared, Kf.lja_switch, Kf, rt, tfj, r2f);
+    // |f d| <= |f| r: one range test covers the three components.  A
+    // pair beyond the limit is flagged and clamped to it along its
+    // direction (so relaxation still pushes the beads apart); NaN
+    // contributes nothing.  Where the kinds already took 1/r the clamp
+    // is formed every time and selected, with no branch in the loop.
+    const bool ok = f * f * r2f < kPairLimit * kPairLimit;
+    bad |= in && !ok;
+    float gg;
+    if (SPEC == kSpecDh || SPEC == kSpecLjShared) {
+        const float cl = copysignf(0.999f * kPairLimit, f) * ir;
+        gg = ok ? f : (f == f ? cl : 0.0f);
+    } else {
+        gg = ok ? f : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
+    }
+    gg = in ? gg : 0.0f;   // a masked entry adds exactly the bias constant
+    // scaling by 2^32 is exact in FP32 (the result stays far below
+    // FP32's largest value; a subnormal gg, flushed or not, rounds to
+    // a zero contribution either way), so one FP32 multiply replaces
+    // an FP64 one with the same bits
+    return static_cast<double>(gg * 4294967296.0f);
+}
+
+// The pair kernel's spec (specialized row loop) of a group with these kinds.
+__device__ __forceinline__ int pair_spec(int kinds, bool periodic, const PairK<float>& Kf) {
+    if (kinds == kExv) return periodic ? kSpecExvPeriodic : kSpecExvOpen;
+    if (kinds == kDh && periodic) return kSpecDh;
+    if (kinds == (kLjr | kLja) && Kf.lj_shared && Kf.lja_switch && periodic) return kSpecLjShared;
+    if (kinds & ~(kExv | kDh | kLjr | kLja)) return kSpecSolo;
+    return kSpecGeneric;
+}
+
 // One member row for lanes `lane` of a team of TEAM: forces of its entries
 // in fixed point (see kFixMagic).  U entries per lane per round (4, or 2:
 // rounds of 2*TEAM entries waste less of a row's last round, with fewer
@@ -1622,47 +1704,10 @@ __device__ __forceinline__ void force_row(const RowGeom& g, int kinds, const Pai
         // zero) so their dependency chains interleave
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int s = cs[u];
             const PairRec rj = rjs[u];
             double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-            // a group without a box has inverse box 0, for which the minimum
-            // image is the identity
-            if (SPEC == kSpecGeneric || SPEC == kSpecSolo ? g.periodic : SPEC != kSpecExvOpen) {
-                dx = min_image(dx, g.bx, g.ibx);
-                dy = min_image(dy, g.by, g.iby);
-                dz = min_image(dz, g.bz, g.ibz);
-            }
-            const double r2 = dist2(dx, dy, dz);
-            const bool in = s >= 0 && r2 < g.rc2;
             const float* tfj = lds_types + (rj.type & 0xFFFF) * kTypeStride;
-            const float r2f = in ? static_cast<float>(r2) : 1.0f;
-            float f, ir = 0.f;
-            if (SPEC == kSpecExvPeriodic || SPEC == kSpecExvOpen) f = pair_force(kExv, false, false, Kf, rt, tfj, r2f);
-            else if (SPEC == kSpecDh) f = pair_force(kDh, false, false, Kf, rt, tfj, r2f, &ir);
-            else if (SPEC == kSpecLjShared) f = pair_force(kLjr | kLja, true, true, Kf, rt, tfj, r2f, &ir);
-            else if (SPEC == kSpecSolo) f = pair_force(kinds & ~(kExv | kDh | kLjr | kLja), false, Kf.lja_switch, Kf, rt,
-                                                       tfj, r2f);
-            else f = pair_force(kinds & (kExv | kDh | kLjr | kLja), Kf.lj_shared, Kf.lja_switch, Kf, rt, tfj, r2f);
-            // |f d| <= |f| r: one range test covers the three components.  A
-            // pair beyond the limit is flagged and clamped to it along its
-            // direction (so relaxation still pushes the beads apart); NaN
-            // contributes nothing.  Where the kinds already took 1/r the clamp
-            // is formed every time and selected, with no branch in the loop.
-            const bool ok = f * f * r2f < kPairLimit * kPairLimit;
-            acc.bad |= in && !ok;
-            float gg;
-            if (SPEC == kSpecDh || SPEC == kSpecLjShared) {
-                const float cl = copysignf(0.999f * kPairLimit, f) * ir;
-                gg = ok ? f : (f == f ? cl : 0.0f);
-            } else {
-                gg = ok ? f : (f == f ? copysignf(0.999f * kPairLimit, f) * __builtin_amdgcn_rsqf(r2f) : 0.0f);
-            }
-            gg = in ? gg : 0.0f;   // a masked entry adds exactly the bias constant
-            // scaling by 2^32 is exact in FP32 (the result stays far below
-            // FP32's largest value; a subnormal gg, flushed or not, rounds to
-            // a zero contribution either way), so one FP32 multiply replaces
-            // an FP64 one with the same bits
-            const double gs = static_cast<double>(gg * 4294967296.0f);
+            const double gs = pair_entry<SPEC>(g, kinds, Kf, rt, tfj, cs[u] >= 0, dx, dy, dz, acc.bad);
             acc.ux += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic)));
             acc.uy += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic)));
             acc.uz += static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic)));
@@ -1853,12 +1898,7 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
         if (!ENERGY) {
             const RowType rt = row_type(lds_types + ti * kTypeStride);
             // the group's kind selects the loop (uniform over the workgroup)
-            int spec = kSpecGeneric;
-            if (kinds == kExv) spec = geo.periodic ? kSpecExvPeriodic : kSpecExvOpen;
-            else if (kinds == kDh && geo.periodic) spec = kSpecDh;
-            else if (kinds == (kLjr | kLja) && Kf.lj_shared && Kf.lja_switch && geo.periodic) spec = kSpecLjShared;
-            else if (kinds & ~(kExv | kDh | kLjr | kLja)) spec = kSpecSolo;
-            switch (spec) {
+            switch (pair_spec(kinds, geo.periodic, Kf)) {
             case kSpecExvPeriodic:
                 force_row<TEAM, kSpecExvPeriodic, U>(geo, kinds, Kf, rt, ri, beg, cnt, lane, rec, nrec, col,
                                                          lds_types, acc);
@@ -1951,6 +1991,415 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
     if (stamp) {
         __syncthreads();
         if (threadIdx.x == 0) stamp[2 * blockIdx.x + 1] = static_cast<unsigned long long>(wall_clock64());
+    }
+}
+
+// ------------------------------------------------------------ resident step
+// Small systems (up to kResMaxBeads beads): one workgroup per system runs a
+// whole chunk of steps on chip.  Positions and the fixed-point force words
+// live in LDS, the forces are summed with LDS atomics, and the neighbor list
+// (each pair once, i < j in member order, packed as
+// {bead i, bead j, type row i, type row j} in 10 + 10 + 6 + 6 bits) is built
+// inside the kernel by testing every member pair of the group, again whenever
+// a bead has moved more than skin/2 since the last build.  The per-entry
+// force (pair_entry), the bonded terms (bonded_term) and the per-bead step
+// (baoab_bead_g with gauss3_fast noise) are the standard path's own
+// functions, and a pair's contribution to j is the exact negation of its
+// contribution to i, so every force word, and with it every step, has the
+// same bits as the standard path.
+// Everything a step reads sits in LDS where it fits (gfx950 gives a
+// workgroup up to 160 KB): the system's bonded terms up to term_cap, its list
+// up to list_cap entries (the rest stays in global memory), the groups'
+// constants and type rows.  A step is two barriers: the integrator (thread b
+// steps bead b), then the force pass, whose work -- bonded terms, list
+// entries, and the next step's noise of every bead -- is dealt out as one
+// sequence over all threads, so that no thread holds more than its share and
+// the step's critical path is one item long when there are enough threads.
+struct ResArgs {
+    const ResSys* tab;
+    const ResGroup* grp;
+    const BTerm* terms;        // bead indices local to their system
+    const float* rows;         // FP32 type rows, kTypeStride floats each
+    const DevSys* sys;
+    const DevPairGroup* groups;
+    const int* member_bead;
+    const int* member_type;    // type | category << 16
+    const int* excl_start;
+    const int* excl_count;
+    const int* excl;
+    const int* local_id;
+    const double* inv_m;
+    const double* bead_hs2;
+    double *px, *py, *pz, *vx, *vy, *vz;
+    long long *fx, *fy, *fz;
+    uint32_t* list;
+    const FlaTable* fla;
+    const FldTable* fld;
+    const long long* gstep;
+    int* err;
+    int nsteps;
+    int nbs;                   // LDS entries per bead array (>= every system's beads, multiple of 64)
+    int term_cap;              // bonded terms staged in LDS per system
+    int list_cap;              // list entries kept in LDS per system
+};
+
+// A pair group's constants, staged in LDS for the whole chunk.
+struct ResGroupK {
+    RowGeom geo;
+    PairK<float> Kf;
+    long long l0;      // list region in global memory
+    int kinds, spec, m0, nm, trow, gid;
+    int lds_off, lds_cnt;   // the group's entries kept in LDS: [lds_off, lds_off + lds_cnt)
+    int cnt;                // entries of the last build
+    float rl2w;
+};
+
+__host__ __device__ constexpr size_t res_align16(size_t v) { return (v + 15) & ~static_cast<size_t>(15); }
+
+// LDS layout: x, y, z (doubles), f (3 nbs force words; the build's FP32
+// positions lie over them), the next step's noise (3 nbs floats: gauss3_fast
+// rounds its samples through FP32, so a float holds them exactly), mem and
+// lid (nbs ints each), type rows, group constants, flags, terms, list.
+__host__ __device__ constexpr size_t res_lds_fixed(int nbs) {
+    return res_align16(static_cast<size_t>(nbs) * (6 * 8 + 3 * 4 + 2 * 4) +
+                       kResMaxTypeRows * kTypeStride * sizeof(float) + kResMaxGroups * sizeof(ResGroupK) +
+                       8 * sizeof(int));
+}
+__host__ __device__ constexpr size_t res_lds_bytes(int nbs, int term_cap, int list_cap) {
+    return res_lds_fixed(nbs) + static_cast<size_t>(term_cap) * sizeof(BTerm) +
+           static_cast<size_t>(list_cap) * sizeof(uint32_t);
+}
+
+struct ResLds {
+    int nbs;
+    double *x, *y, *z;
+    long long* f;      // fx [0, nbs), fy [nbs, 2 nbs), fz [2 nbs, 3 nbs)
+    float4* p32;       // the build's FP32 positions, over f (dead while the build runs)
+    float* g;          // noise: g0 [0, nbs), g1 [nbs, 2 nbs), g2 [2 nbs, 3 nbs)
+    int* mem;          // packed member info of the group being built
+    int* lid;          // the beads' indices within their system (noise counters)
+    float* rows;
+    ResGroupK* gk;
+    int* flag;         // [0] rebuild, [1] FP64 build, [2] entries kept in LDS so far
+    BTerm* terms;
+    uint32_t* list;
+};
+
+__device__ __forceinline__ ResLds res_lds(unsigned char* base, int nbs, int term_cap) {
+    ResLds L;
+    L.nbs = nbs;
+    L.x = reinterpret_cast<double*>(base);
+    L.y = L.x + nbs;
+    L.z = L.y + nbs;
+    L.f = reinterpret_cast<long long*>(L.z + nbs);
+    L.p32 = reinterpret_cast<float4*>(L.f);
+    L.g = reinterpret_cast<float*>(L.f + 3 * nbs);
+    L.mem = reinterpret_cast<int*>(L.g + 3 * nbs);
+    L.lid = L.mem + nbs;
+    L.rows = reinterpret_cast<float*>(L.lid + nbs);
+    L.gk = reinterpret_cast<ResGroupK*>(L.rows + kResMaxTypeRows * kTypeStride);
+    L.flag = reinterpret_cast<int*>(L.gk + kResMaxGroups);
+    L.terms = reinterpret_cast<BTerm*>(base + res_lds_fixed(nbs));
+    L.list = reinterpret_cast<uint32_t*>(L.terms + term_cap);
+    return L;
+}
+
+// Lists of every group of the system at the positions in LDS.  All threads
+// call it; it overwrites the force words (p32 lies over them) and ends with a
+// barrier, after which they may be written again.  Entries go to LDS while
+// the system's list_cap lasts, then to the group's global region.
+template <int T>
+__device__ void res_build(const ResArgs& a, const ResSys& R, const DevSys& S, const ResLds& L) {
+    const int tid = threadIdx.x;
+    if (R.ng == 0) return;   // uniform; nothing is written
+    if (tid == 0) { L.flag[1] = 0; L.flag[2] = 0; }
+    __syncthreads();
+    if (tid < R.nb) {
+        double x = L.x[tid], y = L.y[tid], z = L.z[tid];
+        if (S.periodic) {
+            x = wrap_coord(x, S.box[0], S.inv_box[0]);
+            y = wrap_coord(y, S.box[1], S.inv_box[1]);
+            z = wrap_coord(z, S.box[2], S.inv_box[2]);
+        } else {
+            x -= L.x[0];
+            y -= L.y[0];
+            z -= L.z[0];
+        }
+        L.p32[tid] = make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z), 0.f);
+        if (!(fabs(x) <= kResWide && fabs(y) <= kResWide && fabs(z) <= kResWide)) L.flag[1] = 1;
+    }
+    for (int g = 0; g < R.ng; ++g) {
+        ResGroupK& K = L.gk[g];
+        const DevPairGroup& G = a.groups[K.gid];
+        if (tid == 0) K.cnt = 0;
+        if (tid < K.nm) {
+            const int m = K.m0 + tid;
+            const int mt = a.member_type[m];
+            L.mem[tid] = (a.member_bead[m] - R.b0) | ((K.trow + (mt & 0xFFFF)) << 10) | ((mt >> 16) << 16);
+        }
+        __syncthreads();
+        const bool wide = L.flag[1] != 0;
+        const bool per = K.geo.periodic;
+        const int lds_base = L.flag[2];
+        const int lds_room = a.list_cap - lds_base;
+        uint32_t* glist = a.list + K.l0;
+        const float Lx = static_cast<float>(G.box[0]), Ly = static_cast<float>(G.box[1]);
+        const float Lz = static_cast<float>(G.box[2]);
+        const float iLx = static_cast<float>(G.inv_box[0]), iLy = static_cast<float>(G.inv_box[1]);
+        const float iLz = static_cast<float>(G.inv_box[2]);
+        // thread r takes rows r and nm - 1 - r: nm - 1 candidates together
+        for (int r = tid; 2 * r < K.nm; r += T) {
+            for (int side = 0; side < 2; ++side) {
+                const int i = side ? K.nm - 1 - r : r;
+                if (side && i == r) break;
+                const int mi = L.mem[i];
+                const int bi = mi & 1023, ti = (mi >> 10) & 63;
+                const uint32_t allow = G.allow[mi >> 16];
+                const int eb = a.excl_start[K.m0 + i], ec = a.excl_count[K.m0 + i];
+                const float4 pi = L.p32[bi];
+                for (int j = i + 1; j < K.nm; ++j) {
+                    const int mj = L.mem[j];
+                    const int bj = mj & 1023;
+                    bool near;
+                    if (!wide) {
+                        const float4 pj = L.p32[bj];
+                        float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                        if (per) {
+                            dx = fmaf(-Lx, rintf(dx * iLx), dx);
+                            dy = fmaf(-Ly, rintf(dy * iLy), dy);
+                            dz = fmaf(-Lz, rintf(dz * iLz), dz);
+                        }
+                        near = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < K.rl2w;
+                    } else {
+                        double dx = L.x[bj] - L.x[bi], dy = L.y[bj] - L.y[bi], dz = L.z[bj] - L.z[bi];
+                        if (per) {
+                            dx = min_image(dx, G.box[0], G.inv_box[0]);
+                            dy = min_image(dy, G.box[1], G.inv_box[1]);
+                            dz = min_image(dz, G.box[2], G.inv_box[2]);
+                        }
+                        near = dist2(dx, dy, dz) < G.rl2;
+                    }
+                    if (!near || !((allow >> (mj >> 16)) & 1u)) continue;
+                    if (ec > 0 && excluded(a.excl, eb, ec, R.b0 + bj)) continue;
+                    const uint32_t w = static_cast<uint32_t>(bi) | (static_cast<uint32_t>(bj) << 10) |
+                                       (static_cast<uint32_t>(ti) << 20) |
+                                       (static_cast<uint32_t>((mj >> 10) & 63) << 26);
+                    const int k = atomicAdd(&K.cnt, 1);
+                    if (k < lds_room) L.list[lds_base + k] = w;
+                    else glist[k] = w;
+                }
+            }
+        }
+        __syncthreads();   // the count is final; the next group reuses mem
+        if (tid == 0) {
+            K.lds_off = lds_base;
+            K.lds_cnt = min(K.cnt, max(lds_room, 0));
+            L.flag[2] = lds_base + K.lds_cnt;
+        }
+        __syncthreads();   // p32 is no longer read; the LDS offset is published
+    }
+}
+
+// The first item of the range [off, off + n) of the step's work sequence that
+// thread tid takes (items u with u % T == tid), as an index into the range.
+template <int T>
+__device__ __forceinline__ int res_first(int off) {
+    return static_cast<int>((static_cast<unsigned>(threadIdx.x) - static_cast<unsigned>(off)) % T);
+}
+
+template <int SPEC, int T>
+__device__ __forceinline__ void res_pairs(const ResArgs& a, const ResGroupK& K, int off, const ResLds& L, int& bad) {
+    const uint32_t* glist = a.list + K.l0;
+    const uint32_t* llist = L.list + K.lds_off;
+    const int cnt = K.cnt, lcnt = K.lds_cnt;
+    const RowGeom geo = K.geo;
+    const PairK<float> Kf = K.Kf;
+    const int kinds = K.kinds;
+    const int nbs = L.nbs;
+    unsigned long long* F = reinterpret_cast<unsigned long long*>(L.f);
+    auto entry = [&](uint32_t w) {
+        const int bi = w & 1023, bj = (w >> 10) & 1023, ti = (w >> 20) & 63, tj = w >> 26;
+        double dx = L.x[bj] - L.x[bi], dy = L.y[bj] - L.y[bi], dz = L.z[bj] - L.z[bi];
+        const RowType rt = row_type(L.rows + ti * kTypeStride);
+        const double gs = pair_entry<SPEC>(geo, kinds, Kf, rt, L.rows + tj * kTypeStride, true, dx, dy, dz, bad);
+        const unsigned long long cx =
+            static_cast<unsigned long long>(__double_as_longlong(fma(gs, dx, kFixMagic))) - kFixMagicBits;
+        const unsigned long long cy =
+            static_cast<unsigned long long>(__double_as_longlong(fma(gs, dy, kFixMagic))) - kFixMagicBits;
+        const unsigned long long cz =
+            static_cast<unsigned long long>(__double_as_longlong(fma(gs, dz, kFixMagic))) - kFixMagicBits;
+        atomicAdd(F + bi, cx);
+        atomicAdd(F + nbs + bi, cy);
+        atomicAdd(F + 2 * nbs + bi, cz);
+        atomicAdd(F + bj, 0ull - cx);
+        atomicAdd(F + nbs + bj, 0ull - cy);
+        atomicAdd(F + 2 * nbs + bj, 0ull - cz);
+    };
+    // entries kept in LDS, then the rest from global memory (two loops, so
+    // that each load has one address space)
+    int e = res_first<T>(off);
+    for (const int end = min(lcnt, cnt); e < end; e += T) entry(llist[e]);
+    for (; e < cnt; e += T) entry(glist[e]);
+}
+
+// The force pass, and (noise != 0) every bead's noise for global step
+// `nstep`: the sequence terms, list entries group by group, beads.
+template <int T>
+__device__ __forceinline__ void res_forces(const ResArgs& a, const ResSys& R, const DevSys& S, const ResLds& L,
+                                           int noise, long long nstep, int& bad) {
+    const ForceOut out{L.f, L.f + L.nbs, L.f + 2 * L.nbs};
+    const int nl = min(R.nt, a.term_cap);
+    int t = threadIdx.x;
+    for (; t < nl; t += T) bonded_term<false, float>(L.terms[t], S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
+    for (; t < R.nt; t += T) {
+        const BTerm tm = a.terms[R.t0 + t];
+        bonded_term<false, float>(tm, S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
+    }
+    int off = R.nt;
+    for (int g = 0; g < R.ng; ++g) {
+        const ResGroupK& K = L.gk[g];
+        switch (K.spec) {
+        case kSpecExvPeriodic: res_pairs<kSpecExvPeriodic, T>(a, K, off, L, bad); break;
+        case kSpecExvOpen: res_pairs<kSpecExvOpen, T>(a, K, off, L, bad); break;
+        case kSpecDh: res_pairs<kSpecDh, T>(a, K, off, L, bad); break;
+        case kSpecLjShared: res_pairs<kSpecLjShared, T>(a, K, off, L, bad); break;
+        case kSpecSolo: res_pairs<kSpecSolo, T>(a, K, off, L, bad); break;
+        default: res_pairs<kSpecGeneric, T>(a, K, off, L, bad); break;
+        }
+        off += K.cnt;
+    }
+    if (noise) {
+        for (int b = res_first<T>(off); b < R.nb; b += T) {
+            double g[3];
+            gauss3_fast(S.seed, static_cast<uint64_t>(nstep - S.step_origin), static_cast<uint32_t>(L.lid[b]), g);
+            L.g[b] = static_cast<float>(g[0]);
+            L.g[L.nbs + b] = static_cast<float>(g[1]);
+            L.g[2 * L.nbs + b] = static_cast<float>(g[2]);
+        }
+    }
+}
+
+// One workgroup of T threads per system; thread b owns bead b (nb <= T).
+// On entry the force words hold F(x) of the current positions and the
+// velocities are synchronized (no half kick owed); on exit the same holds
+// nsteps later.  Every barrier is reached by every thread: the step count
+// and the rebuild decision (read from LDS after a barrier) are uniform.
+template <int T>
+__global__ __launch_bounds__(T) void k_resident(ResArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char res_base[];
+    const ResLds L = res_lds(res_base, a.nbs, a.term_cap);
+    const ResSys R = a.tab[blockIdx.x];
+    const DevSys S = a.sys[R.sys];
+    const int tid = threadIdx.x;
+    const int nbs = L.nbs;
+    const bool own = tid < R.nb;
+    const int gb = R.b0 + tid;
+    double X[3] = {0, 0, 0}, V[3] = {0, 0, 0}, ref[3] = {0, 0, 0};
+    double im = 0.0, hs2 = 0.0;
+    int bad = 0;
+    const long long step0 = a.gstep[0];
+    if (own) {
+        X[0] = a.px[gb]; X[1] = a.py[gb]; X[2] = a.pz[gb];
+        V[0] = a.vx[gb]; V[1] = a.vy[gb]; V[2] = a.vz[gb];
+        im = a.inv_m[gb];
+        hs2 = a.bead_hs2[gb];
+        L.x[tid] = X[0]; L.y[tid] = X[1]; L.z[tid] = X[2];
+        const int lid = a.local_id[gb];
+        L.lid[tid] = lid;
+        double g[3];
+        gauss3_fast(S.seed, static_cast<uint64_t>(step0 - S.step_origin), static_cast<uint32_t>(lid), g);
+        L.g[tid] = static_cast<float>(g[0]);
+        L.g[nbs + tid] = static_cast<float>(g[1]);
+        L.g[2 * nbs + tid] = static_cast<float>(g[2]);
+    }
+    for (int k = tid; k < R.nrow * kTypeStride; k += T) L.rows[k] = a.rows[static_cast<size_t>(R.row0) * kTypeStride + k];
+    {
+        // the system's bonded terms, 12 words each, as far as term_cap holds
+        const int nw = min(R.nt, a.term_cap) * static_cast<int>(sizeof(BTerm) / 4);
+        const int* src = reinterpret_cast<const int*>(a.terms + R.t0);
+        int* dst = reinterpret_cast<int*>(L.terms);
+        for (int k = tid; k < nw; k += T) dst[k] = src[k];
+    }
+    if (tid < R.ng) {
+        const ResGroup Q = a.grp[R.g0 + tid];
+        const DevPairGroup& G = a.groups[Q.gid];
+        ResGroupK& K = L.gk[tid];
+        K.geo = RowGeom{G.box[0], G.box[1], G.box[2], G.inv_box[0], G.inv_box[1], G.inv_box[2], G.rc2, G.rin2,
+                        G.periodic != 0};
+        K.Kf = pair_constants<float>(G);
+        K.l0 = Q.l0;
+        K.kinds = G.kinds;
+        K.spec = pair_spec(G.kinds, G.periodic != 0, K.Kf);
+        K.m0 = Q.m0;
+        K.nm = Q.nm;
+        K.trow = Q.trow;
+        K.gid = Q.gid;
+        K.lds_off = 0;
+        K.lds_cnt = 0;
+        K.cnt = 0;
+        K.rl2w = Q.rl2w;
+    }
+    if (tid == 0) L.flag[0] = 0;
+    __syncthreads();
+    res_build<T>(a, R, S, L);
+    ref[0] = X[0]; ref[1] = X[1]; ref[2] = X[2];
+    if (own) {
+        L.f[tid] = a.fx[gb];
+        L.f[nbs + tid] = a.fy[gb];
+        L.f[2 * nbs + tid] = a.fz[gb];
+    }
+    __syncthreads();
+    for (int s = 0; s < a.nsteps; ++s) {
+        if (own) {
+            const long long F0 = L.f[tid], F1 = L.f[nbs + tid], F2 = L.f[2 * nbs + tid];
+            L.f[tid] = 0;   // read: the force pass after the barrier refills them
+            L.f[nbs + tid] = 0;
+            L.f[2 * nbs + tid] = 0;
+            const double g[3] = {L.g[tid], L.g[nbs + tid], L.g[2 * nbs + tid]};
+            double xn[3], vn[3];
+            if (!baoab_bead_g(S, im, F0, F1, F2, X, V, s > 0 ? 1 : 0, g, xn, vn)) bad = 1;
+            X[0] = xn[0]; X[1] = xn[1]; X[2] = xn[2];
+            V[0] = vn[0]; V[1] = vn[1]; V[2] = vn[2];
+            L.x[tid] = X[0]; L.y[tid] = X[1]; L.z[tid] = X[2];
+            const double dx = X[0] - ref[0], dy = X[1] - ref[1], dz = X[2] - ref[2];
+            if (!(dist2(dx, dy, dz) <= hs2)) L.flag[0] = 1;
+        }
+        __syncthreads();
+        if (L.flag[0] != 0) {   // uniform: every thread reads it after the barrier
+            res_build<T>(a, R, S, L);
+            ref[0] = X[0]; ref[1] = X[1]; ref[2] = X[2];
+            if (tid == 0) L.flag[0] = 0;   // read by all before the build's barriers
+            if (own) {   // the build's FP32 positions lay over the force words
+                L.f[tid] = 0;
+                L.f[nbs + tid] = 0;
+                L.f[2 * nbs + tid] = 0;
+            }
+            __syncthreads();
+        }
+        // the noise slots were read by the integrator before the barrier
+        res_forces<T>(a, R, S, L, s + 1 < a.nsteps ? 1 : 0, step0 + s + 1, bad);
+        __syncthreads();
+    }
+    if (own) {
+        const long long F0 = L.f[tid], F1 = L.f[nbs + tid], F2 = L.f[2 * nbs + tid];
+        if (im != 0.0) {   // the closing half kick, as k_half_kick
+            V[0] = half_kick(V[0], S.half_dt, F0 * kFixInv, im);
+            V[1] = half_kick(V[1], S.half_dt, F1 * kFixInv, im);
+            V[2] = half_kick(V[2], S.half_dt, F2 * kFixInv, im);
+        }
+        a.px[gb] = X[0]; a.py[gb] = X[1]; a.pz[gb] = X[2];
+        a.vx[gb] = V[0]; a.vy[gb] = V[1]; a.vz[gb] = V[2];
+        a.fx[gb] = F0; a.fy[gb] = F1; a.fz[gb] = F2;
+    }
+    if (bad) atomicOr(&a.err[R.sys], kErrNumeric);
+}
+
+// Both step slots advance by n after a resident chunk.
+__global__ void k_gstep_advance(long long* gstep, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const long long s = gstep[0] + n;
+        gstep[0] = s;
+        gstep[1] = s;
     }
 }
 
@@ -2234,6 +2683,30 @@ void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& 
 }
 
 int pair_grid_size(const DevPtrs& d) { return std::min(d.n_pair_blocks, d.pair_grid); }
+
+size_t resident_lds_bytes(int nbs, int term_cap, int list_cap) { return res_lds_bytes(nbs, term_cap, list_cap); }
+
+hipError_t resident_prepare(int threads, size_t lds) {
+    const void* f = threads == 256 ? reinterpret_cast<const void*>(&k_resident<256>)
+                  : threads == 512 ? reinterpret_cast<const void*>(&k_resident<512>)
+                                   : reinterpret_cast<const void*>(&k_resident<1024>);
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+}
+
+void launch_resident(const DevPtrs& d, hipStream_t st, int nsteps) {
+    if (d.res_n == 0 || nsteps <= 0) return;
+    const ResArgs a{d.res_tab, d.res_grp, d.res_terms, d.res_rows, d.sys, d.groups, d.member_bead, d.member_type,
+                    d.excl_start, d.excl_count, d.excl, d.local_id, d.inv_m, d.bead_hs2, d.px, d.py, d.pz,
+                    d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.res_list, d.fla, d.fld, d.gstep, d.err, nsteps,
+                    d.res_nbs, d.res_term_cap, d.res_list_cap};
+    const size_t lds = res_lds_bytes(d.res_nbs, d.res_term_cap, d.res_list_cap);
+    switch (d.res_threads) {
+    case 256: k_resident<256><<<d.res_n, 256, lds, st>>>(a); break;
+    case 512: k_resident<512><<<d.res_n, 512, lds, st>>>(a); break;
+    default: k_resident<1024><<<d.res_n, 1024, lds, st>>>(a); break;
+    }
+    k_gstep_advance<<<1, 64, 0, st>>>(d.gstep, nsteps);
+}
 
 void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int nlaunch, int nblk,
                           unsigned long long* out) {

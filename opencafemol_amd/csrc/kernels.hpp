@@ -74,6 +74,16 @@ struct DevPtrs {
     int team = 16;
     int pair_u = 4;                // force pass: entries per lane per round (4 or 2)
     int prune_team = 32;           // lanes per row of the inner-list prune
+    // resident stepping (every system small): one workgroup of res_threads
+    // threads per system, res_n systems
+    int res_n = 0, res_threads = 0;
+    int res_term_cap = 0, res_list_cap = 0;   // bonded terms and list entries held in LDS per system
+    int res_nbs = 0;                          // LDS entries per bead array (multiple of 64)
+    ResSys* res_tab = nullptr;
+    ResGroup* res_grp = nullptr;
+    BTerm* res_terms = nullptr;
+    float* res_rows = nullptr;
+    uint32_t* res_list = nullptr;
     long long* energy = nullptr;   // nsys * 32 fixed point
     long long* ke = nullptr;       // nsys fixed point
     int* dump_count = nullptr;
@@ -110,6 +120,15 @@ void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& 
 // Rewrites the inner list from the outer one at the current positions.
 void launch_prune(const DevPtrs& d, hipStream_t st);
 int pair_grid_size(const DevPtrs& d);
+// Resident stepping: nsteps steps of every system, each in one workgroup, and
+// both step slots advanced by nsteps.  Forces at the current positions must be
+// in d.f and the velocities synchronized; both hold again afterwards.
+void launch_resident(const DevPtrs& d, hipStream_t st, int nsteps);
+// LDS of one resident workgroup (bead arrays of nbs entries), and the
+// attribute that allows it (gfx950 gives a workgroup up to 160 KB).
+size_t resident_lds_bytes(int nbs, int term_cap, int list_cap);
+hipError_t resident_prepare(int threads, size_t lds);
+constexpr size_t kResLdsMax = 160 * 1024;
 void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int nlaunch, int nblk,
                           unsigned long long* out);
 void launch_zero_energy(const DevPtrs& d, hipStream_t st);

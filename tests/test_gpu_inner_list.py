@@ -32,7 +32,8 @@ def _relaxed_hps(seed=3):
 
 
 def _run(s, it, x, nsteps, **opt):
-    with Context(s, it, **opt) as c:
+    # the inner list belongs to the standard path (resident stepping keeps its own list)
+    with Context(s, it, resident=0, **opt) as c:
         c.set_positions(x)
         c.set_velocities_to_temperature(it.temperature, 9)
         assert c.step(nsteps) == abi.OCM_OK
@@ -75,7 +76,7 @@ def test_batch_and_go_chain_with_inner_list():
     reps = configs.go_replicas(4, n_min=60, n_max=200, seed=2)
     out = []
     for p in (0, 4):
-        with Batch([r[0] for r in reps], [r[1] for r in reps], prune_interval=p, chunk_steps=50) as b:
+        with Batch([r[0] for r in reps], [r[1] for r in reps], prune_interval=p, chunk_steps=50, resident=0) as b:
             for k, (s, it, x) in enumerate(reps):
                 b.set_positions(k, x)
             b.step(300)
@@ -89,7 +90,7 @@ def test_small_list_keeps_no_inner_list():
     # a 62-bead chain's whole list is read within the force pass's launch
     # latency: no inner list is kept, whatever prune_interval asks
     s, it, x = configs.go_chain(62, seed=2)
-    with Context(s, it) as c:
+    with Context(s, it, resident=0) as c:
         c.set_positions(x)
         assert c.step(100) == abi.OCM_OK
         assert c.inner_list_stats() == (0, 0)

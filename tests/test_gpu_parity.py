@@ -154,12 +154,14 @@ def test_hps_slab_full_size_forces(oracle):
     _check_forces_energies(s, it, x, oracle)
 
 
-def test_baoab_matches_oracle(oracle):
+@pytest.mark.parametrize("resident", [0, -1])
+def test_baoab_matches_oracle(oracle, resident):
     s, it, x = configs.go_chain(62, seed=2)
     x = thermal(x, 2)
     rng = np.random.default_rng(0)
     v = rng.normal(scale=0.1, size=x.shape)
-    with _ctx(s, it, x) as c:
+    with _ctx(s, it, x, resident=resident) as c:
+        assert (c.resident_threads() > 0) == (resident != 0)
         c.set_velocities(v)
         c.step(7)
         st = c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES)
@@ -172,10 +174,11 @@ def test_baoab_matches_oracle(oracle):
 
 
 def test_step_deterministic_and_graph_equals_eager(oracle):
+    # the standard path's step graphs (resident stepping captures none)
     s, it, x = configs.go_chain(90, seed=4)
     outs = []
     for graph in (1, 1, 0):
-        with _ctx(s, it, x, use_graph=graph, chunk_steps=50, rebuild_interval=10) as c:
+        with _ctx(s, it, x, use_graph=graph, chunk_steps=50, rebuild_interval=10, resident=0) as c:
             c.step(230)
             outs.append(c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES))
     for o in outs[1:]:
@@ -183,14 +186,15 @@ def test_step_deterministic_and_graph_equals_eager(oracle):
         assert np.array_equal(o["velocities"], outs[0]["velocities"])
 
 
-def test_batch_alone_equals_in_mix(oracle):
+@pytest.mark.parametrize("resident", [0, -1])
+def test_batch_alone_equals_in_mix(oracle, resident):
     reps = configs.go_replicas(6, n_min=50, n_max=200, seed=1)
     alone = []
     for s, it, x in reps:
-        with _ctx(s, it, x) as c:
+        with _ctx(s, it, x, resident=resident) as c:
             c.step(150)
             alone.append(c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES | abi.STATE_ENERGY))
-    with Batch([r[0] for r in reps], [r[1] for r in reps]) as b:
+    with Batch([r[0] for r in reps], [r[1] for r in reps], resident=resident) as b:
         for k, (s, it, x) in enumerate(reps):
             b.set_positions(k, x)
         b.step(150)
@@ -201,11 +205,13 @@ def test_batch_alone_equals_in_mix(oracle):
             assert np.array_equal(st["energy_by_group"], alone[k]["energy_by_group"])
 
 
-def test_batch_of_many_systems_xcd_slices():
+@pytest.mark.parametrize("resident", [0, -1])
+def test_batch_of_many_systems_xcd_slices(resident):
     """From 16 systems a batch lays its pair-kernel table out as 8 XCD slices
     (each XCD's workgroups walk one contiguous eighth).  Every system must
     still step to the bits it reaches alone: 20 Go replicas and 2 HPS slabs
-    (periodic, with the inner list) in one batch."""
+    (periodic, with the inner list) in one batch; the same batch stepped
+    resident, one workgroup per system, too."""
     reps = configs.go_replicas(20, n_min=50, n_max=160, seed=3)
     for seed in (5, 6):
         s, it, x = _small_hps(seed)
@@ -213,10 +219,10 @@ def test_batch_of_many_systems_xcd_slices():
     what = abi.STATE_POSITIONS | abi.STATE_VELOCITIES | abi.STATE_FORCES | abi.STATE_ENERGY
     alone = []
     for s, it, x in reps:
-        with _ctx(s, it, x, chunk_steps=60) as c:
+        with _ctx(s, it, x, chunk_steps=60, resident=resident) as c:
             c.step(150)
             alone.append(c.get_state(what))
-    with Batch([r[0] for r in reps], [r[1] for r in reps], chunk_steps=60) as b:
+    with Batch([r[0] for r in reps], [r[1] for r in reps], chunk_steps=60, resident=resident) as b:
         for k, (s, it, x) in enumerate(reps):
             b.set_positions(k, x)
         b.step(150)
@@ -230,7 +236,7 @@ def test_rebuild_interval_does_not_change_bits(oracle):
     s, it, x = _small_hps(4)
     res = []
     for interval in (1, 7):
-        with _ctx(s, it, x, rebuild_interval=interval, use_graph=0, skin=0.4) as c:
+        with _ctx(s, it, x, rebuild_interval=interval, use_graph=0, skin=0.4, resident=0) as c:
             c.step(20)
             res.append(c.get_state(abi.STATE_POSITIONS)["positions"])
     assert np.array_equal(res[0], res[1])
@@ -247,13 +253,14 @@ def test_team_sizes_give_identical_forces(oracle):
         assert np.array_equal(F, ref)
 
 
-def test_numeric_overflow_is_reported(oracle):
+@pytest.mark.parametrize("resident", [0, -1])
+def test_numeric_overflow_is_reported(oracle, resident):
     # two beads of a Go chain placed on top of each other: the r^-12 terms
     # overflow the fixed-point range and the step reports it
     s, it, x = configs.go_chain(30, seed=1)
     x = x.copy()
     x[10] = x[20] + 1e-4
-    with _ctx(s, it, x) as c:
+    with _ctx(s, it, x, resident=resident) as c:
         code = c.step(3)
         assert code == abi.OCM_ERR_NUMERIC
         assert c.status(0) == abi.OCM_ERR_NUMERIC

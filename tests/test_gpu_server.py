@@ -107,11 +107,12 @@ def _assert_same(jobs, ids, frames, results):
             assert np.array_equal(got[step], pos), f"job {jid}: frame at step {step} differs"
 
 
-def test_step_grouping_does_not_change_bits():
+@pytest.mark.parametrize("resident", [0, -1])
+def test_step_grouping_does_not_change_bits(resident):
     s, it, x = configs.go_chain(120, seed=11)
     out = []
     for split in ([300], [7, 150, 143], [1] * 5 + [295], [200, 100]):
-        with Context(s, it, chunk_steps=64, rebuild_interval=10) as c:
+        with Context(s, it, chunk_steps=64, rebuild_interval=10, resident=resident) as c:
             c.set_positions(x)
             c.set_velocities_to_temperature(300.0, 5)
             for n in split:
