@@ -119,14 +119,23 @@ struct ResSys {
     int t0, nt;       // bonded terms [t0, t0 + nt) of the resident term array
     int g0, ng;       // resident pair groups [g0, g0 + ng)
     int row0, nrow;   // staged type rows [row0, row0 + nrow) of the resident row array
+    int ex0, exn;     // the groups' exclusion tables: words [ex0, ex0 + exn) of the resident exclusion array
 };
+// A group's exclusions in member order (members follow bead order), for the
+// in-kernel build, which tests member pairs (i, j > i): per member a mask
+// whose bit k says member i + 1 + k is excluded (k < 32), then nm + 1 starts
+// and the partners further away (member indices, sorted) -- chains exclude
+// their near neighbours, so most exclusions are mask bits.
+constexpr int kResNearExcl = 32;
 struct ResGroup {
     int gid;          // pair group (DevPairGroup index)
     int m0, nm;       // members [m0, m0 + nm)
     int trow;         // the group's first type row within its system's staged rows
     long long l0;     // list region: nm (nm - 1) / 2 entries from l0
+    double rl2;       // (cutoff + skin)^2 with the resident skin
     float rl2w;       // (cutoff + skin + FP32 margin)^2: the in-kernel build's threshold
-    int pad;
+    int exo;          // its exclusion table within the system's: mask [exo, exo + nm),
+                      // starts [exo + nm, exo + 2 nm + 1), far partners after them
 };
 
 // Per-type parameters: {exv_radius, dh_charge, 0, 0, ljr_eps, ljr_sigma, lja_eps, lja_sigma};

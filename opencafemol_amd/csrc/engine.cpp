@@ -677,6 +677,10 @@ void Engine::pack() {
         d_.bead_hs2 = dalloc<double>(hs2.size()); upload(d_.bead_hs2, hs2);
         d_.member_kpos = dalloc<int>(kpos.size()); upload(d_.member_kpos, kpos);
     }
+    excl_bead_ = member_bead;
+    excl_start_h_ = excl_start;
+    excl_count_h_ = excl_count;
+    excl_h_ = excl;
     d_.excl_start = dalloc<int>(nm); upload(d_.excl_start, excl_start);
     d_.excl_count = dalloc<int>(nm); upload(d_.excl_count, excl_count);
     d_.excl = dalloc<int>(excl.size()); upload(d_.excl, excl);
@@ -718,6 +722,7 @@ void Engine::pack() {
     build_pair_blocks(team);
     team_tuned_ = opt_.team_size != 0;
     if (opt_.resident != 0) pack_resident(terms, groups, types_f);
+    excl_bead_.clear(); excl_start_h_.clear(); excl_count_h_.clear(); excl_h_.clear();
     // resident batches step outside the pair kernel: no launch shape to tune
     if (d_.res_n > 0) team_tuned_ = true;
     check_hip(hipDeviceSynchronize(), "upload sync");
@@ -753,6 +758,9 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
     std::vector<ResGroup> grp;
     std::vector<BTerm> rterms;
     std::vector<float> rows;
+    std::vector<int> rexcl;
+    // per bead: the smallest (skin/2)^2 of its groups' resident lists
+    std::vector<double> res_hs2(static_cast<size_t>(d_.nb), std::numeric_limits<double>::max());
     std::vector<std::vector<int>> sys_terms(nsys);
     for (int t = 0; t < static_cast<int>(terms.size()); ++t) {
         const int b = terms[t].idx[0];
@@ -777,6 +785,7 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
         R.g0 = static_cast<int>(grp.size());
         R.ng = static_cast<int>(sys_groups[s].size());
         R.row0 = static_cast<int>(rows.size() / kTypeStride);
+        R.ex0 = static_cast<int>(rexcl.size());
         int trow = 0;
         for (int g : sys_groups[s]) {
             const DevPairGroup& G = groups[g];
@@ -786,10 +795,41 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
             Q.nm = G.member_count;
             Q.trow = trow;
             Q.l0 = l0;
-            // the FP32 test's threshold: coordinates below kResWide carry an
-            // error under 2.2e-4 nm in a distance, well inside 1e-3 nm
-            const double rlw = std::sqrt(G.rl2) + 1e-3;
-            Q.rl2w = static_cast<float>(rlw * rlw);
+            // exclusions by member index (the members of a group are in bead order)
+            Q.exo = static_cast<int>(rexcl.size()) - R.ex0;
+            {
+                std::vector<int> member_of(sys_[s].n, -1);
+                for (int i = 0; i < Q.nm; ++i) member_of[excl_bead_[Q.m0 + i] - R.b0] = i;
+                std::vector<uint32_t> mask(Q.nm, 0u);
+                std::vector<std::vector<int>> far(Q.nm);
+                for (int i = 0; i < Q.nm; ++i) {
+                    const int m = Q.m0 + i;
+                    for (int e = excl_start_h_[m]; e < excl_start_h_[m] + excl_count_h_[m]; ++e) {
+                        const int j = member_of[excl_h_[e] - R.b0];
+                        if (j <= i) continue;   // the pair is tested from its lower member
+                        if (j - i <= kResNearExcl) mask[i] |= 1u << (j - i - 1);
+                        else far[i].push_back(j);
+                    }
+                    std::sort(far[i].begin(), far[i].end());
+                }
+                for (int i = 0; i < Q.nm; ++i) rexcl.push_back(static_cast<int>(mask[i]));
+                int at = 0;
+                for (int i = 0; i < Q.nm; ++i) { rexcl.push_back(at); at += static_cast<int>(far[i].size()); }
+                rexcl.push_back(at);
+                for (int i = 0; i < Q.nm; ++i) rexcl.insert(rexcl.end(), far[i].begin(), far[i].end());
+            }
+            // the resident list's skin: at most kResSkin (a list read every
+            // step by one workgroup pays for its width at every step, a
+            // rebuild is cheap); the FP32 test's threshold adds 1e-3 nm, well
+            // above the 2.2e-4 nm error of coordinates below kResWide
+            const double skin_r = std::min(2.0 * std::sqrt(G.half_skin2), kResSkin);
+            const double rl = G.rc + skin_r;
+            Q.rl2 = rl * rl;
+            Q.rl2w = static_cast<float>((rl + 1e-3) * (rl + 1e-3));
+            for (int i = 0; i < G.member_count; ++i) {
+                double& h = res_hs2[excl_bead_[G.member_begin + i]];
+                h = std::min(h, 0.25 * skin_r * skin_r);
+            }
             l0 += std::max<long long>(1, static_cast<long long>(Q.nm) * (Q.nm - 1) / 2);
             for (int k = 0; k < G.n_types * kTypeStride; ++k)
                 rows.push_back(types_f[static_cast<size_t>(G.type_begin) * kTypeStride + k]);
@@ -797,6 +837,7 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
             grp.push_back(Q);
         }
         R.nrow = trow;
+        R.exn = static_cast<int>(rexcl.size()) - R.ex0;
     }
     // a step's work (terms, list entries, noise) is dealt over all threads:
     // 1024 of them keep every thread's share short
@@ -805,10 +846,11 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
     // LDS: what one system's step reads, as far as it fits.  Up to one
     // workgroup per CU the whole 160 KB may go to one; beyond that half, so
     // that two workgroups share a CU and hide each other's latency.
-    int max_nt = 0;
+    int max_nt = 0, max_ex = 0;
     long long max_pairs = 0;
     for (int s = 0; s < nsys; ++s) {
         max_nt = std::max(max_nt, tab[s].nt);
+        max_ex = std::max(max_ex, tab[s].exn);
         long long p = 0;
         for (int g = tab[s].g0; g < tab[s].g0 + tab[s].ng; ++g)
             p += static_cast<long long>(grp[g].nm) * (grp[g].nm - 1) / 2;
@@ -821,13 +863,17 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
             cus = prop.multiProcessorCount;
     }
     const size_t budget = (nsys <= cus ? kResLdsMax : kResLdsMax / 2) - 1024;
-    const size_t fixed = resident_lds_bytes(d_.res_nbs, 0, 0);
-    const size_t room = budget > fixed ? budget - fixed : 0;
+    // exclusion tables are staged whole; they are small (chains exclude near
+    // neighbours, which are mask bits)
+    d_.res_ex_cap = max_ex;
+    const size_t fixed = resident_lds_bytes(d_.res_nbs, 0, 0, d_.res_ex_cap);
+    if (fixed > budget) { d_.res_n = 0; return; }
+    const size_t room = budget - fixed;
     d_.res_list_cap = static_cast<int>(std::min<long long>({max_pairs, kResListLdsMax,
                                                             static_cast<long long>(room / sizeof(uint32_t))}));
     d_.res_term_cap = static_cast<int>(std::min<size_t>(static_cast<size_t>(max_nt),
                                                         (room - d_.res_list_cap * sizeof(uint32_t)) / sizeof(BTerm)));
-    const size_t lds = resident_lds_bytes(d_.res_nbs, d_.res_term_cap, d_.res_list_cap);
+    const size_t lds = resident_lds_bytes(d_.res_nbs, d_.res_term_cap, d_.res_list_cap, d_.res_ex_cap);
     check_hip(resident_prepare(d_.res_threads, lds), "hipFuncSetAttribute (resident LDS)");
     d_.res_tab = dalloc<ResSys>(tab.size()); upload(d_.res_tab, tab);
     d_.res_grp = dalloc<ResGroup>(std::max<size_t>(grp.size(), 1));
@@ -837,6 +883,10 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
     d_.res_rows = dalloc<float>(std::max<size_t>(rows.size(), 1));
     if (!rows.empty()) upload(d_.res_rows, rows);
     d_.res_list = dalloc<uint32_t>(static_cast<size_t>(std::max<long long>(l0, 1)));
+    d_.res_hs2 = dalloc<double>(std::max<size_t>(res_hs2.size(), 1));
+    upload(d_.res_hs2, res_hs2);
+    d_.res_excl = dalloc<int>(std::max<size_t>(rexcl.size(), 1));
+    if (!rexcl.empty()) upload(d_.res_excl, rexcl);
     d_.res_n = nsys;
 }
 

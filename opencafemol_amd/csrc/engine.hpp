@@ -168,6 +168,9 @@ private:
     bool lists_stale_ = false;
     static constexpr int kResChunk = 2000;   // resident steps per launch
     static constexpr long long kResListLdsMax = 8192;   // list entries per system kept in LDS
+    static constexpr double kResSkin = 0.3;             // nm, at most, for resident lists
+    // host copies of the members' beads and exclusion rows, held while packing
+    std::vector<int> excl_bead_, excl_start_h_, excl_count_h_, excl_h_;
     int interval_ = 20;
     int chunk_ = 200;           // steps per chunk (chunk_steps, capped for large systems)
     int clean_chunks_ = 0;      // chunks since the interval was last shortened

@@ -2024,12 +2024,9 @@ struct ResArgs {
     const DevPairGroup* groups;
     const int* member_bead;
     const int* member_type;    // type | category << 16
-    const int* excl_start;
-    const int* excl_count;
-    const int* excl;
     const int* local_id;
     const double* inv_m;
-    const double* bead_hs2;
+    const double* bead_hs2;    // per bead: (resident skin / 2)^2
     double *px, *py, *pz, *vx, *vy, *vz;
     long long *fx, *fy, *fz;
     uint32_t* list;
@@ -2041,6 +2038,8 @@ struct ResArgs {
     int nbs;                   // LDS entries per bead array (>= every system's beads, multiple of 64)
     int term_cap;              // bonded terms staged in LDS per system
     int list_cap;              // list entries kept in LDS per system
+    const int* excl_tab;       // the groups' exclusion tables (ResGroup::exo), by system
+    int ex_cap;                // their LDS words per system
 };
 
 // A pair group's constants, staged in LDS for the whole chunk.
@@ -2048,7 +2047,8 @@ struct ResGroupK {
     RowGeom geo;
     PairK<float> Kf;
     long long l0;      // list region in global memory
-    int kinds, spec, m0, nm, trow, gid;
+    double rl2;             // (cutoff + resident skin)^2
+    int kinds, spec, m0, nm, trow, gid, exo;
     int lds_off, lds_cnt;   // the group's entries kept in LDS: [lds_off, lds_off + lds_cnt)
     int cnt;                // entries of the last build
     float rl2w;
@@ -2065,9 +2065,9 @@ __host__ __device__ constexpr size_t res_lds_fixed(int nbs) {
                        kResMaxTypeRows * kTypeStride * sizeof(float) + kResMaxGroups * sizeof(ResGroupK) +
                        8 * sizeof(int));
 }
-__host__ __device__ constexpr size_t res_lds_bytes(int nbs, int term_cap, int list_cap) {
-    return res_lds_fixed(nbs) + static_cast<size_t>(term_cap) * sizeof(BTerm) +
-           static_cast<size_t>(list_cap) * sizeof(uint32_t);
+__host__ __device__ constexpr size_t res_lds_bytes(int nbs, int term_cap, int list_cap, int ex_cap) {
+    return res_lds_fixed(nbs) + res_align16(static_cast<size_t>(ex_cap) * sizeof(int)) +
+           static_cast<size_t>(term_cap) * sizeof(BTerm) + static_cast<size_t>(list_cap) * sizeof(uint32_t);
 }
 
 struct ResLds {
@@ -2081,11 +2081,12 @@ struct ResLds {
     float* rows;
     ResGroupK* gk;
     int* flag;         // [0] rebuild, [1] FP64 build, [2] entries kept in LDS so far
+    int* ex;           // the system's exclusion tables
     BTerm* terms;
     uint32_t* list;
 };
 
-__device__ __forceinline__ ResLds res_lds(unsigned char* base, int nbs, int term_cap) {
+__device__ __forceinline__ ResLds res_lds(unsigned char* base, int nbs, int term_cap, int ex_cap) {
     ResLds L;
     L.nbs = nbs;
     L.x = reinterpret_cast<double*>(base);
@@ -2099,7 +2100,8 @@ __device__ __forceinline__ ResLds res_lds(unsigned char* base, int nbs, int term
     L.rows = reinterpret_cast<float*>(L.lid + nbs);
     L.gk = reinterpret_cast<ResGroupK*>(L.rows + kResMaxTypeRows * kTypeStride);
     L.flag = reinterpret_cast<int*>(L.gk + kResMaxGroups);
-    L.terms = reinterpret_cast<BTerm*>(base + res_lds_fixed(nbs));
+    L.ex = reinterpret_cast<int*>(base + res_lds_fixed(nbs));
+    L.terms = reinterpret_cast<BTerm*>(base + res_lds_fixed(nbs) + res_align16(static_cast<size_t>(ex_cap) * sizeof(int)));
     L.list = reinterpret_cast<uint32_t*>(L.terms + term_cap);
     return L;
 }
@@ -2108,6 +2110,12 @@ __device__ __forceinline__ ResLds res_lds(unsigned char* base, int nbs, int term
 // call it; it overwrites the force words (p32 lies over them) and ends with a
 // barrier, after which they may be written again.  Entries go to LDS while
 // the system's list_cap lasts, then to the group's global region.
+// The member pairs (i, j > i) are dealt over all threads: rows r and
+// nm - 1 - r together hold nm - 1 candidates, and each such row pair is cut
+// into S segments so that about 2T segments cover the group; a thread tests
+// its segments' candidates one after another against the row's exclusions (a
+// mask bit for the next kResNearExcl members, a search of the short sorted
+// list of farther ones, both in LDS).
 template <int T>
 __device__ void res_build(const ResArgs& a, const ResSys& R, const DevSys& S, const ResLds& L) {
     const int tid = threadIdx.x;
@@ -2140,6 +2148,7 @@ __device__ void res_build(const ResArgs& a, const ResSys& R, const DevSys& S, co
         __syncthreads();
         const bool wide = L.flag[1] != 0;
         const bool per = K.geo.periodic;
+        const int nm = K.nm;
         const int lds_base = L.flag[2];
         const int lds_room = a.list_cap - lds_base;
         uint32_t* glist = a.list + K.l0;
@@ -2147,47 +2156,68 @@ __device__ void res_build(const ResArgs& a, const ResSys& R, const DevSys& S, co
         const float Lz = static_cast<float>(G.box[2]);
         const float iLx = static_cast<float>(G.inv_box[0]), iLy = static_cast<float>(G.inv_box[1]);
         const float iLz = static_cast<float>(G.inv_box[2]);
-        // thread r takes rows r and nm - 1 - r: nm - 1 candidates together
-        for (int r = tid; 2 * r < K.nm; r += T) {
-            for (int side = 0; side < 2; ++side) {
-                const int i = side ? K.nm - 1 - r : r;
-                if (side && i == r) break;
-                const int mi = L.mem[i];
-                const int bi = mi & 1023, ti = (mi >> 10) & 63;
-                const uint32_t allow = G.allow[mi >> 16];
-                const int eb = a.excl_start[K.m0 + i], ec = a.excl_count[K.m0 + i];
-                const float4 pi = L.p32[bi];
-                for (int j = i + 1; j < K.nm; ++j) {
-                    const int mj = L.mem[j];
-                    const int bj = mj & 1023;
-                    bool near;
-                    if (!wide) {
-                        const float4 pj = L.p32[bj];
-                        float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                        if (per) {
-                            dx = fmaf(-Lx, rintf(dx * iLx), dx);
-                            dy = fmaf(-Ly, rintf(dy * iLy), dy);
-                            dz = fmaf(-Lz, rintf(dz * iLz), dz);
-                        }
-                        near = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < K.rl2w;
-                    } else {
-                        double dx = L.x[bj] - L.x[bi], dy = L.y[bj] - L.y[bi], dz = L.z[bj] - L.z[bi];
-                        if (per) {
-                            dx = min_image(dx, G.box[0], G.inv_box[0]);
-                            dy = min_image(dy, G.box[1], G.inv_box[1]);
-                            dz = min_image(dz, G.box[2], G.inv_box[2]);
-                        }
-                        near = dist2(dx, dy, dz) < G.rl2;
-                    }
-                    if (!near || !((allow >> (mj >> 16)) & 1u)) continue;
-                    if (ec > 0 && excluded(a.excl, eb, ec, R.b0 + bj)) continue;
-                    const uint32_t w = static_cast<uint32_t>(bi) | (static_cast<uint32_t>(bj) << 10) |
-                                       (static_cast<uint32_t>(ti) << 20) |
-                                       (static_cast<uint32_t>((mj >> 10) & 63) << 26);
-                    const int k = atomicAdd(&K.cnt, 1);
-                    if (k < lds_room) L.list[lds_base + k] = w;
-                    else glist[k] = w;
+        const int nrp = (nm + 1) / 2;                       // row pairs
+        const int nseg = max(1, (2 * T + nrp - 1) / max(nrp, 1));   // segments per row pair
+        const int seglen = (nm - 1 + nseg - 1) / nseg;
+        const int* exm = L.ex + K.exo;
+        const int* far = exm + 2 * nm + 1;
+        // consecutive threads take different row pairs: the entries one wave
+        // appends together (consecutive list slots, read together by a wave of
+        // the force pass) then hit different force words
+        for (int u = tid; u < nrp * nseg; u += T) {
+            const int r = u % nrp, sg = u / nrp;
+            const int r2 = nm - 1 - r;
+            const int len = r2 == r ? nm - 1 - r : nm - 1;   // the middle row of an odd count stands alone
+            const int q0 = sg * seglen, q1 = min(q0 + seglen, len);
+            int i = -1, bi = 0, ti = 0, f0 = 0, f1 = 0;
+            uint32_t allow = 0u, near_mask = 0u;
+            float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int q = q0; q < q1; ++q) {
+                const int ii = q < nm - 1 - r ? r : r2;
+                const int j = q < nm - 1 - r ? r + 1 + q : q + 1;   // row r2's candidates are r2+1 .. nm-1
+                if (ii != i) {
+                    i = ii;
+                    const int mi = L.mem[i];
+                    bi = mi & 1023;
+                    ti = (mi >> 10) & 63;
+                    allow = G.allow[mi >> 16];
+                    near_mask = static_cast<uint32_t>(exm[i]);
+                    f0 = exm[nm + i];
+                    f1 = exm[nm + i + 1];
+                    pi = L.p32[bi];
                 }
+                const int mj = L.mem[j];
+                const int bj = mj & 1023;
+                bool keep;
+                if (!wide) {
+                    const float4 pj = L.p32[bj];
+                    float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                    if (per) {
+                        dx = fmaf(-Lx, rintf(dx * iLx), dx);
+                        dy = fmaf(-Ly, rintf(dy * iLy), dy);
+                        dz = fmaf(-Lz, rintf(dz * iLz), dz);
+                    }
+                    keep = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < K.rl2w;
+                } else {
+                    double dx = L.x[bj] - L.x[bi], dy = L.y[bj] - L.y[bi], dz = L.z[bj] - L.z[bi];
+                    if (per) {
+                        dx = min_image(dx, G.box[0], G.inv_box[0]);
+                        dy = min_image(dy, G.box[1], G.inv_box[1]);
+                        dz = min_image(dz, G.box[2], G.inv_box[2]);
+                    }
+                    keep = dist2(dx, dy, dz) < K.rl2;
+                }
+                if (!keep || !((allow >> (mj >> 16)) & 1u)) continue;
+                const int d = j - i;
+                bool hit = false;
+                if (d <= kResNearExcl) hit = (near_mask >> (d - 1)) & 1u;
+                else for (int k = f0; k < f1; ++k) hit |= far[k] == j;
+                if (hit) continue;
+                const uint32_t w = static_cast<uint32_t>(bi) | (static_cast<uint32_t>(bj) << 10) |
+                                   (static_cast<uint32_t>(ti) << 20) | (static_cast<uint32_t>((mj >> 10) & 63) << 26);
+                const int k = atomicAdd(&K.cnt, 1);
+                if (k < lds_room) L.list[lds_base + k] = w;
+                else glist[k] = w;
             }
         }
         __syncthreads();   // the count is final; the next group reuses mem
@@ -2235,11 +2265,17 @@ __device__ __forceinline__ void res_pairs(const ResArgs& a, const ResGroupK& K, 
         atomicAdd(F + nbs + bj, 0ull - cy);
         atomicAdd(F + 2 * nbs + bj, 0ull - cz);
     };
-    // entries kept in LDS, then the rest from global memory (two loops, so
-    // that each load has one address space)
-    int e = res_first<T>(off);
-    for (const int end = min(lcnt, cnt); e < end; e += T) entry(llist[e]);
-    for (; e < cnt; e += T) entry(glist[e]);
+    // entries kept in LDS, the rest read through a buffer resource over the
+    // group's global region (a different load than LDS's, so neither becomes
+    // a generic load, and the entry body is inlined once)
+    const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(glist), 0, cnt * static_cast<int>(sizeof(uint32_t)), 0x00020000);
+    for (int e = res_first<T>(off); e < cnt; e += T) {
+        uint32_t w;
+        if (e < lcnt) w = llist[e];
+        else w = static_cast<uint32_t>(__builtin_amdgcn_raw_buffer_load_b32(grs, e * 4, 0, 0));
+        entry(w);
+    }
 }
 
 // The force pass, and (noise != 0) every bead's noise for global step
@@ -2249,10 +2285,23 @@ __device__ __forceinline__ void res_forces(const ResArgs& a, const ResSys& R, co
                                            int noise, long long nstep, int& bad) {
     const ForceOut out{L.f, L.f + L.nbs, L.f + 2 * L.nbs};
     const int nl = min(R.nt, a.term_cap);
-    int t = threadIdx.x;
-    for (; t < nl; t += T) bonded_term<false, float>(L.terms[t], S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
-    for (; t < R.nt; t += T) {
-        const BTerm tm = a.terms[R.t0 + t];
+    // terms past term_cap through a buffer resource (as the list entries)
+    static_assert(sizeof(BTerm) == 48, "a term is three 16-byte loads");
+    const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<BTerm*>(a.terms + R.t0), 0, R.nt * static_cast<int>(sizeof(BTerm)), 0x00020000);
+    for (int t = threadIdx.x; t < R.nt; t += T) {
+        BTerm tm;
+        if (t < nl) {
+            tm = L.terms[t];
+        } else {
+            uint32_t w[12];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(trs, t * 48 + 16 * k, 0, 0);
+                w[4 * k] = v[0]; w[4 * k + 1] = v[1]; w[4 * k + 2] = v[2]; w[4 * k + 3] = v[3];
+            }
+            __builtin_memcpy(&tm, w, sizeof(tm));
+        }
         bonded_term<false, float>(tm, S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
     }
     int off = R.nt;
@@ -2287,7 +2336,7 @@ __device__ __forceinline__ void res_forces(const ResArgs& a, const ResSys& R, co
 template <int T>
 __global__ __launch_bounds__(T) void k_resident(ResArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char res_base[];
-    const ResLds L = res_lds(res_base, a.nbs, a.term_cap);
+    const ResLds L = res_lds(res_base, a.nbs, a.term_cap, a.ex_cap);
     const ResSys R = a.tab[blockIdx.x];
     const DevSys S = a.sys[R.sys];
     const int tid = threadIdx.x;
@@ -2320,6 +2369,7 @@ __global__ __launch_bounds__(T) void k_resident(ResArgs a) {
         int* dst = reinterpret_cast<int*>(L.terms);
         for (int k = tid; k < nw; k += T) dst[k] = src[k];
     }
+    for (int k = tid; k < R.exn; k += T) L.ex[k] = a.excl_tab[R.ex0 + k];
     if (tid < R.ng) {
         const ResGroup Q = a.grp[R.g0 + tid];
         const DevPairGroup& G = a.groups[Q.gid];
@@ -2334,6 +2384,8 @@ __global__ __launch_bounds__(T) void k_resident(ResArgs a) {
         K.nm = Q.nm;
         K.trow = Q.trow;
         K.gid = Q.gid;
+        K.exo = Q.exo;
+        K.rl2 = Q.rl2;
         K.lds_off = 0;
         K.lds_cnt = 0;
         K.cnt = 0;
@@ -2684,7 +2736,9 @@ void launch_pair(const DevPtrs& d, hipStream_t st, bool energy, const ForceBuf& 
 
 int pair_grid_size(const DevPtrs& d) { return std::min(d.n_pair_blocks, d.pair_grid); }
 
-size_t resident_lds_bytes(int nbs, int term_cap, int list_cap) { return res_lds_bytes(nbs, term_cap, list_cap); }
+size_t resident_lds_bytes(int nbs, int term_cap, int list_cap, int ex_cap) {
+    return res_lds_bytes(nbs, term_cap, list_cap, ex_cap);
+}
 
 hipError_t resident_prepare(int threads, size_t lds) {
     const void* f = threads == 256 ? reinterpret_cast<const void*>(&k_resident<256>)
@@ -2696,10 +2750,10 @@ hipError_t resident_prepare(int threads, size_t lds) {
 void launch_resident(const DevPtrs& d, hipStream_t st, int nsteps) {
     if (d.res_n == 0 || nsteps <= 0) return;
     const ResArgs a{d.res_tab, d.res_grp, d.res_terms, d.res_rows, d.sys, d.groups, d.member_bead, d.member_type,
-                    d.excl_start, d.excl_count, d.excl, d.local_id, d.inv_m, d.bead_hs2, d.px, d.py, d.pz,
+                    d.local_id, d.inv_m, d.res_hs2, d.px, d.py, d.pz,
                     d.vx, d.vy, d.vz, d.f.fx, d.f.fy, d.f.fz, d.res_list, d.fla, d.fld, d.gstep, d.err, nsteps,
-                    d.res_nbs, d.res_term_cap, d.res_list_cap};
-    const size_t lds = res_lds_bytes(d.res_nbs, d.res_term_cap, d.res_list_cap);
+                    d.res_nbs, d.res_term_cap, d.res_list_cap, d.res_excl, d.res_ex_cap};
+    const size_t lds = res_lds_bytes(d.res_nbs, d.res_term_cap, d.res_list_cap, d.res_ex_cap);
     switch (d.res_threads) {
     case 256: k_resident<256><<<d.res_n, 256, lds, st>>>(a); break;
     case 512: k_resident<512><<<d.res_n, 512, lds, st>>>(a); break;

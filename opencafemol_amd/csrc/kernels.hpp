@@ -79,6 +79,9 @@ struct DevPtrs {
     int res_n = 0, res_threads = 0;
     int res_term_cap = 0, res_list_cap = 0;   // bonded terms and list entries held in LDS per system
     int res_nbs = 0;                          // LDS entries per bead array (multiple of 64)
+    int res_ex_cap = 0;                       // exclusion-table words per system (staged in LDS)
+    int* res_excl = nullptr;
+    double* res_hs2 = nullptr;                // per bead: (resident skin / 2)^2
     ResSys* res_tab = nullptr;
     ResGroup* res_grp = nullptr;
     BTerm* res_terms = nullptr;
@@ -126,7 +129,7 @@ int pair_grid_size(const DevPtrs& d);
 void launch_resident(const DevPtrs& d, hipStream_t st, int nsteps);
 // LDS of one resident workgroup (bead arrays of nbs entries), and the
 // attribute that allows it (gfx950 gives a workgroup up to 160 KB).
-size_t resident_lds_bytes(int nbs, int term_cap, int list_cap);
+size_t resident_lds_bytes(int nbs, int term_cap, int list_cap, int ex_cap);
 hipError_t resident_prepare(int threads, size_t lds);
 constexpr size_t kResLdsMax = 160 * 1024;
 void launch_reduce_stamps(hipStream_t st, const unsigned long long* stamp, int nlaunch, int nblk,
