@@ -382,7 +382,8 @@ typedef struct ocm_server_stats {
     double  wait_p50_ms;         /* submit -> admission                       */
     double  wait_p99_ms;
     double  simulated_ns;        /* sum over ended jobs of steps * dt         */
-    double  busy_ms;             /* worker time spent stepping or repacking   */
+    double  busy_ms;             /* worker time spent stepping or repacking,
+                                    summed over the two lane workers         */
     double  repack_ms;           /* of which repacking                        */
     double  makespan_ms;         /* first submission -> last completion       */
     int64_t device_bytes;        /* current batch allocation                  */

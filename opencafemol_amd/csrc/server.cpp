@@ -26,6 +26,18 @@ double percentile(std::vector<double> v, double q) {
     return v[std::min(v.size() - 1, rank == 0 ? 0 : rank - 1)];
 }
 
+// Whether a job can step resident: at most kResMaxBeads beads, at most
+// kResMaxGroups nonbonded forces and, when periodic, a box of at most kResWide
+// nm.  The engine checks the rest (type rows) when it packs the lane; a batch
+// with a member that fails there steps the standard path, with the same bits.
+bool resident_sized(const HostSystem& s) {
+    if (s.n > kResMaxBeads) return false;
+    int pair_forces = 0;
+    for (const HostForce& f : s.forces) pair_forces += is_pair(f.kind) ? 1 : 0;
+    if (pair_forces > kResMaxGroups) return false;
+    return !s.periodic || std::max({s.box[0], s.box[1], s.box[2]}) <= kResWide;
+}
+
 }  // namespace
 
 Server::Server(const ocm_server_options& opt) : opt_(opt), t0_(std::chrono::steady_clock::now()) {
@@ -45,8 +57,14 @@ Server::Server(const ocm_server_options& opt) : opt_(opt), t0_(std::chrono::stea
         hip_ok(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         opt_.hbm_budget_bytes = static_cast<int64_t>(0.8 * static_cast<double>(free_b));
     }
-    hip_ok(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "hipStreamCreate");
-    worker_ = std::thread([this] { loop(); });
+    for (int l = 0; l < kLanes; ++l) {
+        lanes_[l].id = l;
+        hip_ok(hipStreamCreateWithFlags(&lanes_[l].copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    for (int l = 0; l < kLanes; ++l) {
+        Lane* L = &lanes_[l];
+        L->worker = std::thread([this, L] { loop(*L); });
+    }
     framer_ = std::thread([this] { frame_loop(); });
 }
 
@@ -59,9 +77,11 @@ Server::~Server() {
     slot_cv_.notify_all();
     frame_cv_.notify_all();
     done_cv_.notify_all();
-    if (worker_.joinable()) worker_.join();
+    for (Lane& L : lanes_)
+        if (L.worker.joinable()) L.worker.join();
     if (framer_.joinable()) framer_.join();
-    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
+    for (Lane& L : lanes_)
+        if (L.copy_stream) (void)hipStreamDestroy(L.copy_stream);
 }
 
 double Server::now_ms() const {
@@ -90,6 +110,8 @@ std::size_t Server::estimate_bytes(const HostSystem& s) const {
         const double list_bytes = opt_.engine.prune_interval > 1 ? 8.0 : 4.0;   // outer (+ inner) list
         bytes += s.n * (list_bytes * per_bead + 48.0) + 8.0 * f.exclusions.size();
     }
+    // a resident-sized system's own list holds every member pair (4 bytes each)
+    if (s.n <= kResMaxBeads) bytes += 2.0 * s.n * s.n;
     return static_cast<std::size_t>(bytes) + (1u << 20);
 }
 
@@ -102,6 +124,8 @@ long long Server::submit(HostSystem sys, std::vector<double> x, std::vector<doub
         if (!std::isfinite(c)) throw Error(OCM_ERR_INVALID, "position is not finite");
     auto j = std::make_shared<ServerJob>();
     j->est_bytes = estimate_bytes(sys);
+    // resident-sized jobs get the lane of their own (when resident stepping is on)
+    j->lane = opt_.engine.resident != 0 && resident_sized(sys) ? 0 : 1;
     j->sys = std::move(sys);
     j->x = std::move(x);
     j->v_set = !v.empty();
@@ -120,27 +144,39 @@ long long Server::submit(HostSystem sys, std::vector<double> x, std::vector<doub
         j->t_submit = now_ms();
         if (first_submit_ < 0) first_submit_ = j->t_submit;
         jobs_[id] = j;
-        queue_.push_back(j);
+        lanes_[j->lane].queue.push_back(j);
         ++submitted_;
     }
     work_cv_.notify_all();
     return id;
 }
 
-// FIFO admission: the head of the queue enters when a slot is free and its
-// estimate fits beside the active jobs; nothing overtakes it.
-bool Server::admit_locked() {
+int Server::active_jobs_locked() const {
+    int n = 0;
+    for (const Lane& L : lanes_) n += static_cast<int>(L.active.size());
+    return n;
+}
+
+// FIFO admission within a lane: the head of its queue enters when a job slot
+// is free and its estimate fits beside the active jobs of both lanes (or
+// nothing is active); nothing overtakes it.
+bool Server::admissible_locked(const Lane& L) const {
+    if (L.queue.empty() || active_jobs_locked() >= opt_.max_active_jobs) return false;
+    if (active_jobs_locked() == 0) return true;
+    return static_cast<long long>(active_bytes_ + L.queue.front()->est_bytes) <= opt_.hbm_budget_bytes;
+}
+
+bool Server::admit_locked(Lane& L) {
     bool any = false;
-    while (!queue_.empty() && static_cast<int>(active_.size()) < opt_.max_active_jobs) {
-        auto& j = queue_.front();
-        if (!active_.empty() && static_cast<long long>(active_bytes_ + j->est_bytes) > opt_.hbm_budget_bytes) break;
+    while (admissible_locked(L)) {
+        auto j = L.queue.front();
         active_bytes_ += j->est_bytes;
         if (j->t_start < 0) {   // a job handed back after a failed repack keeps its first admission
             j->t_start = now_ms();
             waits_.push_back(j->t_start - j->t_submit);
         }
-        active_.push_back(j);
-        queue_.pop_front();
+        L.active.push_back(j);
+        L.queue.pop_front();
         any = true;
     }
     return any;
@@ -160,84 +196,88 @@ void Server::finish_job(const std::shared_ptr<ServerJob>& j, int status, const s
         } else {
             ++failed_;
         }
-        auto it = std::find(active_.begin(), active_.end(), j);
-        if (it != active_.end()) {
+        Lane& L = lanes_[j->lane];
+        auto it = std::find(L.active.begin(), L.active.end(), j);
+        if (it != L.active.end()) {
             active_bytes_ -= j->est_bytes;
-            active_.erase(it);
-            membership_changed_ = true;
+            L.active.erase(it);
+            L.membership_changed = true;
         }
     }
     done_cv_.notify_all();
+    work_cv_.notify_all();   // freed memory may admit the other lane's head
 }
 
-// Waits until every frame snapshot has been unpacked.
-void Server::drain_slots() {
+// Waits until every frame snapshot of the lane has been unpacked.
+void Server::drain_slots(Lane& L) {
     std::unique_lock<std::mutex> lk(mu_);
     slot_cv_.wait(lk, [&] {
-        for (const FrameSlot& s : slots_)
+        for (const FrameSlot& s : L.slots)
             if (s.busy) return false;
         return true;
     });
 }
 
-void Server::release_slots() {
-    drain_slots();
-    for (FrameSlot& s : slots_) {
+void Server::release_slots(Lane& L) {
+    drain_slots(L);
+    for (FrameSlot& s : L.slots) {
         if (s.dev) (void)hipFree(s.dev);
         if (s.host) (void)hipHostFree(s.host);
         if (s.copied) (void)hipEventDestroy(s.copied);
         if (s.staged) (void)hipEventDestroy(s.staged);
     }
-    slots_.clear();
+    L.slots.clear();
 }
 
-void Server::repack() {
+void Server::repack(Lane& L) {
     const double t = now_ms();
     // carry the state of jobs that stay over from the old batch
-    if (engine_) {
-        for (std::size_t s = 0; s < packed_.size(); ++s) {
-            auto& j = packed_[s];
+    if (L.engine) {
+        for (std::size_t s = 0; s < L.packed.size(); ++s) {
+            auto& j = L.packed[s];
             if (j->finished) continue;
             ocm_state st;
             std::memset(&st, 0, sizeof(st));
             st.positions = j->x.data();
             st.velocities = j->v.data();
-            engine_->get_state(static_cast<int>(s), OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
+            L.engine->get_state(static_cast<int>(s), OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
             j->v_set = true;
         }
-        drain_slots();   // snapshots in flight index the old batch's beads
-        engine_.reset();
-        engine_bytes_ = 0;
+        drain_slots(L);   // snapshots in flight index the old batch's beads
+        L.engine.reset();
+        std::lock_guard<std::mutex> lk(mu_);
+        L.engine_bytes = 0;
     }
-    packed_.clear();
+    L.packed.clear();
     std::vector<std::shared_ptr<ServerJob>> members;
     {
         std::lock_guard<std::mutex> lk(mu_);
-        members = active_;
+        members = L.active;
     }
     if (members.empty()) return;
     std::vector<HostSystem> systems;
     systems.reserve(members.size());
     for (auto& j : members) systems.push_back(j->sys);
     try {
-        engine_ = std::make_unique<Engine>(std::move(systems), opt_.engine);
+        L.engine = std::make_unique<Engine>(std::move(systems), opt_.engine);
         for (std::size_t s = 0; s < members.size(); ++s) {
             auto& j = members[s];
-            engine_->set_positions(static_cast<int>(s), j->x.data());
+            L.engine->set_positions(static_cast<int>(s), j->x.data());
             if (j->v_set) {
-                engine_->set_velocities(static_cast<int>(s), j->v.data());
+                L.engine->set_velocities(static_cast<int>(s), j->v.data());
             } else {
-                engine_->set_velocities_to_temperature(static_cast<int>(s), j->sys.integ.temperature, j->sys.integ.seed);
+                L.engine->set_velocities_to_temperature(static_cast<int>(s), j->sys.integ.temperature,
+                                                        j->sys.integ.seed);
             }
-            engine_->set_steps_done(static_cast<int>(s), j->done);
+            L.engine->set_steps_done(static_cast<int>(s), j->done);
         }
-        const long long nb = engine_->n_beads_total();
-        if (slots_.empty() || slots_[0].cap < nb) {
+        const long long nb = L.engine->n_beads_total();
+        if (L.slots.empty() || L.slots[0].cap < nb) {
             // frame slots outlive repacks; they grow (by half again) only when a batch outgrows them
-            release_slots();
+            release_slots(L);
             const long long cap = std::max(nb, nb + nb / 2);
-            slots_.resize(kSlots);
-            for (FrameSlot& sl : slots_) {
+            L.slots.resize(kSlots);
+            for (FrameSlot& sl : L.slots) {
                 sl.cap = cap;
                 hip_ok(hipMalloc(&sl.dev, 3 * cap * sizeof(double)), "hipMalloc");
                 hip_ok(hipHostMalloc(&sl.host, 3 * cap * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
@@ -245,34 +285,34 @@ void Server::repack() {
                 hip_ok(hipEventCreateWithFlags(&sl.staged, hipEventDisableTiming), "event");
             }
         }
-        for (FrameSlot& sl : slots_) sl.nb = nb;
+        for (FrameSlot& sl : L.slots) sl.nb = nb;
     } catch (const Error& e) {
-        engine_.reset();
-        release_slots();
+        L.engine.reset();
+        release_slots(L);
         if (e.code == OCM_ERR_OOM && members.size() > 1) {
             // the estimate was short: hand the newest job back to the queue
             std::lock_guard<std::mutex> lk(mu_);
-            auto j = active_.back();
-            active_.pop_back();
+            auto j = L.active.back();
+            L.active.pop_back();
             active_bytes_ -= j->est_bytes;
-            queue_.push_front(j);
+            L.queue.push_front(j);
             opt_.hbm_budget_bytes = static_cast<int64_t>(active_bytes_);
-            membership_changed_ = true;
+            L.membership_changed = true;
             return;
         }
         for (auto& j : members) finish_job(j, e.code, e.what());
         return;
     }
-    packed_ = std::move(members);
-    engine_bytes_ = engine_->device_bytes();
+    L.packed = std::move(members);
     std::lock_guard<std::mutex> lk(mu_);
+    L.engine_bytes = L.engine->device_bytes();
     ++repacks_;
     repack_ms_ += now_ms() - t;
 }
 
 // Snapshots every bead position on the engine stream (ordered before the next
-// stretch), then copies the snapshot to pinned memory on the copy stream.
-void Server::stage_frames(const std::vector<int>& systems) {
+// stretch), then copies the snapshot to pinned memory on the lane's copy stream.
+void Server::stage_frames(Lane& L, const std::vector<int>& systems) {
     int slot = -1;
     {
         std::unique_lock<std::mutex> lk(mu_);
@@ -280,29 +320,29 @@ void Server::stage_frames(const std::vector<int>& systems) {
             if (stop_) return true;
             if (static_cast<int>(frames_.size()) >= opt_.max_queued_frames) return false;
             for (int i = 0; i < kSlots; ++i)
-                if (!slots_[i].busy) { slot = i; return true; }
+                if (!L.slots[i].busy) { slot = i; return true; }
             return false;
         });
         if (slot < 0) return;   // shutting down
-        slots_[slot].busy = true;
+        L.slots[slot].busy = true;
     }
-    FrameSlot& sl = slots_[slot];
+    FrameSlot& sl = L.slots[slot];
     sl.jobs.clear();
     sl.ranges.clear();
     for (int s : systems) {
-        sl.jobs.emplace_back(packed_[s]->id, packed_[s]->done);
-        sl.ranges.emplace_back(engine_->bead_begin(s), engine_->n_particles(s));
+        sl.jobs.emplace_back(L.packed[s]->id, L.packed[s]->done);
+        sl.ranges.emplace_back(L.engine->bead_begin(s), L.engine->n_particles(s));
     }
     const std::size_t bytes = 3 * sl.nb * sizeof(double);
-    engine_->enqueue_position_copy(sl.dev);
-    hip_ok(hipEventRecord(sl.copied, engine_->stream()), "event");
-    hip_ok(hipStreamWaitEvent(copy_stream_, sl.copied, 0), "wait");
-    hip_ok(hipMemcpyAsync(sl.host, sl.dev, bytes, hipMemcpyDeviceToHost, copy_stream_), "frame copy");
-    hip_ok(hipEventRecord(sl.staged, copy_stream_), "event");
+    L.engine->enqueue_position_copy(sl.dev);
+    hip_ok(hipEventRecord(sl.copied, L.engine->stream()), "event");
+    hip_ok(hipStreamWaitEvent(L.copy_stream, sl.copied, 0), "wait");
+    hip_ok(hipMemcpyAsync(sl.host, sl.dev, bytes, hipMemcpyDeviceToHost, L.copy_stream), "frame copy");
+    hip_ok(hipEventRecord(sl.staged, L.copy_stream), "event");
     {
         std::lock_guard<std::mutex> lk(mu_);
-        for (int s : systems) ++packed_[s]->frames_pending;
-        staged_.push_back(slot);
+        for (int s : systems) ++L.packed[s]->frames_pending;
+        staged_.emplace_back(L.id, slot);
         frames_made_ += static_cast<long long>(systems.size());
     }
     frame_cv_.notify_all();
@@ -311,15 +351,16 @@ void Server::stage_frames(const std::vector<int>& systems) {
 void Server::frame_loop() {
     (void)hipSetDevice(opt_.engine.device);
     while (true) {
-        int slot;
+        int lane, slot;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            frame_cv_.wait(lk, [&] { return !staged_.empty() || (stop_ && worker_done_); });
+            frame_cv_.wait(lk, [&] { return !staged_.empty() || (stop_ && workers_done_ == kLanes); });
             if (staged_.empty()) return;
-            slot = staged_.front();
+            lane = staged_.front().first;
+            slot = staged_.front().second;
             staged_.pop_front();
         }
-        FrameSlot& sl = slots_[slot];
+        FrameSlot& sl = lanes_[lane].slots[slot];
         (void)hipEventSynchronize(sl.staged);
         std::vector<ServerFrame> out(sl.jobs.size());
         const double* hx = sl.host;
@@ -350,67 +391,78 @@ void Server::frame_loop() {
     }
 }
 
-void Server::loop() {
+// Ends every job of the lane's batch with an error and drops the batch.
+void Server::fail_lane(Lane& L, int code, const std::string& msg) {
+    std::vector<std::shared_ptr<ServerJob>> members;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        members = L.active;
+    }
+    for (auto& j : members)
+        if (!j->finished) finish_job(j, code, msg);
+    release_slots(L);
+    L.engine.reset();
+    L.packed.clear();
+}
+
+void Server::loop(Lane& L) {
     (void)hipSetDevice(opt_.engine.device);
     while (true) {
         bool need_repack;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            work_cv_.wait(lk, [&] { return stop_ || !queue_.empty() || !active_.empty() || membership_changed_; });
+            // a queued head that cannot be admitted yet (the other lane holds
+            // the memory) does not wake the worker: finish_job notifies when
+            // memory frees
+            work_cv_.wait(lk, [&] {
+                return stop_ || admissible_locked(L) || !L.active.empty() || L.membership_changed;
+            });
             if (stop_) break;
-            if (admit_locked()) membership_changed_ = true;
-            need_repack = membership_changed_;
-            membership_changed_ = false;
+            if (admit_locked(L)) L.membership_changed = true;
+            need_repack = L.membership_changed;
+            L.membership_changed = false;
         }
         const double t = now_ms();
         if (need_repack) {
             try {
-                repack();
+                repack(L);
             } catch (const std::exception& e) {
-                release_slots();
-                engine_.reset();
-                std::vector<std::shared_ptr<ServerJob>> members;
-                {
-                    std::lock_guard<std::mutex> lk(mu_);
-                    members = active_;
-                }
-                for (auto& j : members) finish_job(j, OCM_ERR_HIP, e.what());
-                packed_.clear();
+                fail_lane(L, OCM_ERR_HIP, e.what());
                 continue;
             }
         }
-        if (!engine_) {
+        if (!L.engine) {
             std::lock_guard<std::mutex> lk(mu_);
             busy_ms_ += now_ms() - t;
             continue;
         }
         // stretch: up to the nearest frame or completion of any job
         long long n = opt_.quantum_steps;
-        for (auto& j : packed_) {
+        for (auto& j : L.packed) {
             long long next = j->nsteps;
             if (j->frame_interval > 0) next = std::min(next, (j->done / j->frame_interval + 1) * j->frame_interval);
             n = std::min(n, next - j->done);
         }
         std::vector<int> frame_sys, end_sys;
         try {
-            engine_->step(n);
-            for (std::size_t s = 0; s < packed_.size(); ++s) {
-                auto& j = packed_[s];
+            L.engine->step(n);
+            for (std::size_t s = 0; s < L.packed.size(); ++s) {
+                auto& j = L.packed[s];
                 j->done += n;
-                const int st = engine_->system_status(static_cast<int>(s));
+                const int st = L.engine->system_status(static_cast<int>(s));
                 if (st != OCM_OK) { end_sys.push_back(static_cast<int>(s)); continue; }
                 if (j->frame_interval > 0 && j->done % j->frame_interval == 0) frame_sys.push_back(static_cast<int>(s));
                 if (j->done >= j->nsteps) end_sys.push_back(static_cast<int>(s));
             }
-            if (!frame_sys.empty()) stage_frames(frame_sys);
+            if (!frame_sys.empty()) stage_frames(L, frame_sys);
             for (int s : end_sys) {
-                auto& j = packed_[s];
+                auto& j = L.packed[s];
                 ocm_state st;
                 std::memset(&st, 0, sizeof(st));
                 st.positions = j->x.data();
                 st.velocities = j->v.data();
-                engine_->get_state(s, OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
-                const int status = engine_->system_status(s);
+                L.engine->get_state(s, OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
+                const int status = L.engine->system_status(s);
                 double dt_ps = j->sys.integ.dt;
                 {
                     std::lock_guard<std::mutex> lk(mu_);
@@ -420,22 +472,18 @@ void Server::loop() {
             }
         } catch (const std::exception& e) {
             const int code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : OCM_ERR_HIP;
-            for (auto& j : packed_)
-                if (!j->finished) finish_job(j, code, e.what());
-            release_slots();
-            engine_.reset();
-            packed_.clear();
+            fail_lane(L, code, e.what());
         }
         std::lock_guard<std::mutex> lk(mu_);
         busy_ms_ += now_ms() - t;
     }
     // shutdown: let the frame thread finish the snapshots in flight, then free
-    release_slots();
-    engine_.reset();
-    packed_.clear();
+    release_slots(L);
+    L.engine.reset();
+    L.packed.clear();
     {
         std::lock_guard<std::mutex> lk(mu_);
-        worker_done_ = true;
+        ++workers_done_;
     }
     frame_cv_.notify_all();
 }
@@ -475,8 +523,8 @@ ocm_server_stats Server::stats() {
     st.submitted = submitted_;
     st.completed = completed_;
     st.failed = failed_;
-    st.active = static_cast<int64_t>(active_.size());
-    st.queued = static_cast<int64_t>(queue_.size());
+    st.active = active_jobs_locked();
+    st.queued = static_cast<int64_t>(lanes_[0].queue.size() + lanes_[1].queue.size());
     st.repacks = repacks_;
     st.frames = frames_made_;
     st.latency_p50_ms = percentile(latencies_, 0.50);
@@ -487,7 +535,7 @@ ocm_server_stats Server::stats() {
     st.busy_ms = busy_ms_;
     st.repack_ms = repack_ms_;
     st.makespan_ms = (first_submit_ >= 0 && last_end_ >= 0) ? last_end_ - first_submit_ : 0.0;
-    st.device_bytes = static_cast<int64_t>(engine_bytes_);
+    st.device_bytes = static_cast<int64_t>(lanes_[0].engine_bytes + lanes_[1].engine_bytes);
     st.budget_bytes = opt_.hbm_budget_bytes;
     return st;
 }

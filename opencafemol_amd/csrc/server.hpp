@@ -1,13 +1,18 @@
 // server.hpp -- continuous-batching job server over the packed-batch engine.
 //
-// One worker thread owns the device.  Jobs wait in a FIFO queue until the
-// batch has a free slot and their estimated device memory fits the budget;
-// then the batch is repacked with every active job and all of them advance
-// together, one launch per step stage.  The worker steps the batch in
-// stretches that end at the nearest frame or completion of any job, so a new
-// arrival waits at most one stretch (capped at `quantum_steps`).  Frames are
-// snapshotted on the device, copied to pinned host memory on a second stream
-// while the next stretch runs, and unpacked by a frame thread.
+// Jobs run in two lanes, each a batch with its own worker thread, engine and
+// streams, so the two batches run side by side on the device: jobs small
+// enough for resident stepping (up to kResMaxBeads beads; one workgroup each,
+// whole chunks on chip) in one, the others in the other.  A small job then
+// does not advance at the pace of a large one.  In each lane jobs wait in a
+// FIFO queue until there is a free slot and their estimated device memory
+// fits the budget (shared by the lanes); then the lane's batch is repacked
+// with its active jobs and all of them advance together.  A worker steps its
+// batch in stretches that end at the nearest frame or completion of any of
+// its jobs, so a new arrival waits at most one stretch (capped at
+// `quantum_steps`).  Frames are snapshotted on the device, copied to pinned
+// host memory on a copy stream while the next stretch runs, and unpacked by a
+// frame thread.
 //
 // A job's trajectory does not depend on what else is in the batch or where
 // the stretches end: forces are sums in fixed point, the noise is keyed by the
@@ -31,6 +36,7 @@ namespace ocm {
 
 struct ServerJob {
     long long id = 0;
+    int lane = 0;                      // 0: resident-sized, 1: the rest
     HostSystem sys;
     std::vector<double> x, v;          // state at `done` steps (host copy)
     bool v_set = false;                // false: Maxwell-Boltzmann at first admission
@@ -75,42 +81,54 @@ private:
         std::vector<std::pair<int, int>> ranges;              // (bead begin, count)
     };
 
-    void loop();
+    // One batch and its worker.  Fields marked (mu_) are shared with the API
+    // threads and the other lane under the server mutex; the rest belong to
+    // the lane's worker (the frame thread reads `slots` entries it was handed).
+    struct Lane {
+        int id = 0;
+        std::deque<std::shared_ptr<ServerJob>> queue;      // (mu_) waiting, FIFO
+        std::vector<std::shared_ptr<ServerJob>> active;    // (mu_) admitted, in engine order once packed
+        bool membership_changed = false;                    // (mu_)
+        std::size_t engine_bytes = 0;                       // (mu_)
+        std::unique_ptr<Engine> engine;
+        std::vector<std::shared_ptr<ServerJob>> packed;     // engine system s -> job
+        hipStream_t copy_stream = nullptr;
+        std::vector<FrameSlot> slots;
+        std::thread worker;
+    };
+    static constexpr int kLanes = 2;
+
+    void loop(Lane& L);
     void frame_loop();
-    bool admit_locked();
-    void repack();
-    void stage_frames(const std::vector<int>& systems);
-    void release_slots();
-    void drain_slots();
+    bool admissible_locked(const Lane& L) const;
+    bool admit_locked(Lane& L);
+    void repack(Lane& L);
+    void stage_frames(Lane& L, const std::vector<int>& systems);
+    void release_slots(Lane& L);
+    void drain_slots(Lane& L);
     void finish_job(const std::shared_ptr<ServerJob>& j, int status, const std::string& msg);
+    void fail_lane(Lane& L, int code, const std::string& msg);
     double now_ms() const;
     std::size_t estimate_bytes(const HostSystem& s) const;
+    int active_jobs_locked() const;
 
     ocm_server_options opt_;
     std::chrono::steady_clock::time_point t0_;
     std::mutex mu_;
     std::condition_variable work_cv_, done_cv_, frame_cv_, slot_cv_;
-    std::deque<std::shared_ptr<ServerJob>> queue_;
-    std::vector<std::shared_ptr<ServerJob>> active_;          // admitted, in engine order once packed
+    Lane lanes_[kLanes];
     std::map<long long, std::shared_ptr<ServerJob>> jobs_;     // all not yet collected
     std::deque<ServerFrame> frames_;
     std::vector<double> latencies_, waits_;
-    bool stop_ = false, worker_done_ = false, membership_changed_ = false;
+    bool stop_ = false;
+    int workers_done_ = 0;
     long long next_id_ = 1;
-    std::size_t active_bytes_ = 0;
+    std::size_t active_bytes_ = 0;                             // over both lanes
     long long submitted_ = 0, completed_ = 0, failed_ = 0, repacks_ = 0, frames_made_ = 0;
     double simulated_ns_ = 0, busy_ms_ = 0, repack_ms_ = 0, first_submit_ = -1, last_end_ = -1;
-    std::size_t engine_bytes_ = 0;
+    std::deque<std::pair<int, int>> staged_;                   // (lane, slot) awaiting the frame thread (mu_)
 
-    // worker-thread state
-    std::unique_ptr<Engine> engine_;
-    std::vector<std::shared_ptr<ServerJob>> packed_;   // engine system s -> job
-    hipStream_t copy_stream_ = nullptr;
-    std::vector<FrameSlot> slots_;
-    std::deque<int> staged_;                           // slot indices awaiting the frame thread (under mu_)
-    int team_ = 0;
-
-    std::thread worker_, framer_;
+    std::thread framer_;
 };
 
 }  // namespace ocm

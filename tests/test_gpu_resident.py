@@ -145,3 +145,17 @@ def test_state_queries_after_resident_steps(oracle):
     assert after["step"] == ref["step"] == 1000
     assert np.array_equal(after["positions"], ref["positions"])
     assert np.array_equal(after["velocities"], ref["velocities"])
+
+
+def test_replicas_256_resident_equals_standard():
+    # the benchmark batch at full size (256 Go replicas of 50-500 beads,
+    # 280-400 K ladder): every replica's state after 600 steps, resident and
+    # standard, bit for bit
+    reps = configs.go_replicas(256, seed=0)
+    rng = np.random.default_rng(11)
+    reps = [(s, it, x + rng.normal(scale=0.02, size=x.shape)) for s, it, x in reps]
+    std, t0 = _run_batch(reps, 600, 0)
+    res, t1 = _run_batch(reps, 600, -1)
+    assert t0 == 0 and t1 == 1024
+    for k, (a, b) in enumerate(zip(std, res)):
+        _same(a, b, k)
