@@ -383,14 +383,17 @@ def hps_open_cluster(n_chains=150, chain_len=140, extent=(12.0, 12.0), seed=0):
 
 
 def served_job_mix(n_jobs=24, seed=0, steps=(10000, 50000), frame_interval=10000, go_fraction=0.6):
-    """Job stream: Go chains of 50-500 beads (fraction go_fraction) and HPS
-    slabs of 2,000-20,000 beads, each running a whole number of frame
-    intervals drawn uniformly from [steps[0], steps[1]]."""
+    """Job stream: Go chains of 50-500 beads and HPS slabs of 2,000-20,000
+    beads, each running a whole number of frame intervals drawn uniformly
+    from [steps[0], steps[1]].  round(go_fraction * n_jobs) of the jobs are Go
+    chains, in a random order, so a short stream holds the stated mix."""
     rng = np.random.default_rng(seed)
     fi = frame_interval
+    n_go = int(round(go_fraction * n_jobs))
+    is_go = rng.permutation(np.arange(n_jobs) < n_go)
     jobs = []
     for k in range(n_jobs):
-        if rng.random() < go_fraction:
+        if is_go[k]:
             s, it, x = go_chain(int(rng.integers(50, 501)), seed=seed * 1000 + k)
             kind = "go"
         else:

@@ -151,6 +151,29 @@ long long Server::submit(HostSystem sys, std::vector<double> x, std::vector<doub
     return id;
 }
 
+void Server::gate_enter(bool exclusive) {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (exclusive) {
+        ++gate_waiting_;
+        gate_cv_.wait(lk, [&] { return gate_users_ == 0 && !gate_held_; });
+        --gate_waiting_;
+        gate_held_ = true;
+    } else {
+        // a waiting repack goes first
+        gate_cv_.wait(lk, [&] { return !gate_held_ && gate_waiting_ == 0; });
+        ++gate_users_;
+    }
+}
+
+void Server::gate_exit(bool exclusive) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (exclusive) gate_held_ = false;
+        else --gate_users_;
+    }
+    gate_cv_.notify_all();
+}
+
 int Server::active_jobs_locked() const {
     int n = 0;
     for (const Lane& L : lanes_) n += static_cast<int>(L.active.size());
@@ -424,12 +447,15 @@ void Server::loop(Lane& L) {
         }
         const double t = now_ms();
         if (need_repack) {
+            gate_enter(true);
             try {
                 repack(L);
             } catch (const std::exception& e) {
                 fail_lane(L, OCM_ERR_HIP, e.what());
+                gate_exit(true);
                 continue;
             }
+            gate_exit(true);
         }
         if (!L.engine) {
             std::lock_guard<std::mutex> lk(mu_);
@@ -444,6 +470,10 @@ void Server::loop(Lane& L) {
             n = std::min(n, next - j->done);
         }
         std::vector<int> frame_sys, end_sys;
+        bool failed = false;
+        int fail_code = OCM_OK;
+        std::string fail_msg;
+        gate_enter(false);
         try {
             L.engine->step(n);
             for (std::size_t s = 0; s < L.packed.size(); ++s) {
@@ -471,16 +501,25 @@ void Server::loop(Lane& L) {
                 finish_job(j, status, status == OCM_OK ? "" : "NaN or fixed-point force overflow detected");
             }
         } catch (const std::exception& e) {
-            const int code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : OCM_ERR_HIP;
-            fail_lane(L, code, e.what());
+            failed = true;
+            fail_code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : OCM_ERR_HIP;
+            fail_msg = e.what();
+        }
+        gate_exit(false);
+        if (failed) {
+            gate_enter(true);
+            fail_lane(L, fail_code, fail_msg);
+            gate_exit(true);
         }
         std::lock_guard<std::mutex> lk(mu_);
         busy_ms_ += now_ms() - t;
     }
     // shutdown: let the frame thread finish the snapshots in flight, then free
+    gate_enter(true);
     release_slots(L);
     L.engine.reset();
     L.packed.clear();
+    gate_exit(true);
     {
         std::lock_guard<std::mutex> lk(mu_);
         ++workers_done_;

@@ -111,6 +111,13 @@ private:
     double now_ms() const;
     std::size_t estimate_bytes(const HostSystem& s) const;
     int active_jobs_locked() const;
+    // Device maintenance gate.  hipFree synchronizes the whole device, so a
+    // repack freeing an engine while the other lane keeps submitting would
+    // wait for one of its stretches per free.  A lane steps inside the gate
+    // (shared); a repack or teardown holds it alone, so it waits for the
+    // other lane's stretch once and the other lane pauses meanwhile.
+    void gate_enter(bool exclusive);
+    void gate_exit(bool exclusive);
 
     ocm_server_options opt_;
     std::chrono::steady_clock::time_point t0_;
@@ -127,6 +134,9 @@ private:
     long long submitted_ = 0, completed_ = 0, failed_ = 0, repacks_ = 0, frames_made_ = 0;
     double simulated_ns_ = 0, busy_ms_ = 0, repack_ms_ = 0, first_submit_ = -1, last_end_ = -1;
     std::deque<std::pair<int, int>> staged_;                   // (lane, slot) awaiting the frame thread (mu_)
+    int gate_users_ = 0, gate_waiting_ = 0;                    // (mu_)
+    bool gate_held_ = false;                                   // (mu_)
+    std::condition_variable gate_cv_;
 
     std::thread framer_;
 };
