@@ -949,19 +949,28 @@ template <typename R> __device__ inline V3T<R> cross3(V3T<R> a, V3T<R> b) {
 template <typename R> __device__ inline R dot3(V3T<R> a, V3T<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 template <typename R> __device__ inline R r_sqrt(R v);
-template <> __device__ inline float r_sqrt<float>(float v) { return sqrtf(v); }
+template <> __device__ inline float r_sqrt<float>(float v) { return __builtin_amdgcn_sqrtf(v); }
 template <> __device__ inline double r_sqrt<double>(double v) { return sqrt(v); }
 template <typename R> __device__ inline R r_exp(R v);
-template <> __device__ inline float r_exp<float>(float v) { return expf(v); }
+template <> __device__ inline float r_exp<float>(float v) { return __expf(v); }
 template <> __device__ inline double r_exp<double>(double v) { return exp(v); }
 template <typename R> __device__ inline R r_atan2(R a, R b);
 template <> __device__ inline float r_atan2<float>(float a, float b) { return atan2f(a, b); }
 template <> __device__ inline double r_atan2<double>(double a, double b) { return atan2(a, b); }
+// The force pass (R = float) takes the hardware's square root, reciprocal,
+// exponential and sine/cosine (V_SQRT_F32, V_RCP_F32: 1 ulp; V_EXP_F32 on
+// x log2(e); V_SIN/COS_F32 on x / 2pi, for the angles of dihedral terms) in
+// place of the correctly rounded library sequences: a few instructions
+// instead of 10-120, in a chain that sets the resident step's latency.  The
+// energy pass (R = double) keeps the library functions and true division.
+template <typename R> __device__ inline R r_div(R a, R b);
+template <> __device__ inline float r_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <> __device__ inline double r_div<double>(double a, double b) { return a / b; }
 template <typename R> __device__ inline R r_floor(R v);
 template <> __device__ inline float r_floor<float>(float v) { return floorf(v); }
 template <> __device__ inline double r_floor<double>(double v) { return floor(v); }
 template <typename R> __device__ inline void r_sincos(R v, R* s, R* c);
-template <> __device__ inline void r_sincos<float>(float v, float* s, float* c) { sincosf(v, s, c); }
+template <> __device__ inline void r_sincos<float>(float v, float* s, float* c) { __sincosf(v, s, c); }
 template <> __device__ inline void r_sincos<double>(double v, double* s, double* c) { sincos(v, s, c); }
 
 struct ForceOut {
@@ -970,18 +979,21 @@ struct ForceOut {
     long long* fz;
 };
 
+// A force contribution in fixed point, and its addition to bead b.  The last
+// bead of a term takes the negated sum of the others' fixed-point words, so
+// each term adds exactly zero net force (and converts one bead fewer).
+struct FixV {
+    long long x, y, z;
+};
 template <typename R>
-__device__ inline void add_force(const ForceOut& o, int b, R gx, R gy, R gz, int& bad) {
-    // force on bead b = g (already the negative gradient)
-    if constexpr (sizeof(R) == 4) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fx[b]), static_cast<unsigned long long>(to_fix(gx, bad)));
-        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fy[b]), static_cast<unsigned long long>(to_fix(gy, bad)));
-        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fz[b]), static_cast<unsigned long long>(to_fix(gz, bad)));
-    } else {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fx[b]), static_cast<unsigned long long>(to_fix_d(gx, bad)));
-        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fy[b]), static_cast<unsigned long long>(to_fix_d(gy, bad)));
-        atomicAdd(reinterpret_cast<unsigned long long*>(&o.fz[b]), static_cast<unsigned long long>(to_fix_d(gz, bad)));
-    }
+__device__ inline FixV fix_v(R gx, R gy, R gz, int& bad) {
+    if constexpr (sizeof(R) == 4) return {to_fix(gx, bad), to_fix(gy, bad), to_fix(gz, bad)};
+    else return {to_fix_d(gx, bad), to_fix_d(gy, bad), to_fix_d(gz, bad)};
+}
+__device__ inline void add_fix(const ForceOut& o, int b, long long x, long long y, long long z) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&o.fx[b]), static_cast<unsigned long long>(x));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&o.fy[b]), static_cast<unsigned long long>(y));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&o.fz[b]), static_cast<unsigned long long>(z));
 }
 
 // One bonded term.  R = float for the force pass, double for the energy pass.
@@ -1007,7 +1019,7 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
         // FP64 over (r + r0) keeps the relative error of r - r0 itself ~1e-7.
         auto stretch = [&](double r0d) -> R {
             if (sizeof(R) == sizeof(double)) return static_cast<R>(sqrt(r2d) - r0d);
-            return static_cast<R>(fma(-r0d, r0d, r2d)) / (r + static_cast<R>(r0d));
+            return r_div<R>(static_cast<R>(fma(-r0d, r0d, r2d)), r + static_cast<R>(r0d));
         };
         R dedr;
         if (t.kind == 1) {
@@ -1018,31 +1030,32 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
         } else if (t.kind == 2) {
             const R k = static_cast<R>(t.p[0]), s = static_cast<R>(t.p[2]);
             const R dr = stretch(t.p[1]);
-            const R is2 = R(1) / (s * s);
+            const R is2 = r_div<R>(R(1), s * s);
             e = k * r_exp<R>(-R(0.5) * dr * dr * is2);
             dedr = -e * dr * is2;
-        } else if (t.kind == 3 || static_cast<R>(t.p[2]) - r / static_cast<R>(t.p[1]) < R(0)) {
+        } else if (t.kind == 3 || static_cast<R>(t.p[2]) - r_div<R>(r, static_cast<R>(t.p[1])) < R(0)) {
             // Go 12-10 (the capped kind beyond its cap, r / r0 > capping ratio)
             const R k = static_cast<R>(t.p[0]), r0 = static_cast<R>(t.p[1]);
-            const R q = r0 / r;
+            const R q = r_div<R>(r0, r);
             const R q2 = q * q, q4 = q2 * q2, q10 = q4 * q4 * q2, q12 = q10 * q2;
             e = k * (R(5) * q12 - R(6) * q10);
-            dedr = R(60) * k * (q10 - q12) / r;
+            dedr = r_div<R>(R(60) * k * (q10 - q12), r);
         } else {
             // capped Go contact inside its cap: the line through the 12-10
             // value and slope at r = capping_ratio * r0
             const R k = static_cast<R>(t.p[0]), r0 = static_cast<R>(t.p[1]);
-            const R ic = R(1) / static_cast<R>(t.p[2]);
+            const R ic = r_div<R>(R(1), static_cast<R>(t.p[2]));
             const R ic2 = ic * ic, ic10 = ic2 * ic2 * ic2 * ic2 * ic2, ic12 = ic10 * ic2;
             const R slope = R(-60) * (ic12 * ic - ic10 * ic);
             const R intercept = R(65) * ic12 - R(66) * ic10;
-            e = k * (slope * (r / r0) + intercept);
-            dedr = k * slope / r0;
+            e = k * (slope * r_div<R>(r, r0) + intercept);
+            dedr = r_div<R>(k * slope, r0);
         }
         if (!ENERGY) {
-            const R s = dedr / r;
-            add_force<R>(out, t.idx[0], d.x * s, d.y * s, d.z * s, bad);
-            add_force<R>(out, t.idx[1], -d.x * s, -d.y * s, -d.z * s, bad);
+            const R s = r_div<R>(dedr, r);
+            const FixV fi = fix_v<R>(d.x * s, d.y * s, d.z * s, bad);
+            add_fix(out, t.idx[0], fi.x, fi.y, fi.z);
+            add_fix(out, t.idx[1], -fi.x, -fi.y, -fi.z);
         }
         break;
     }
@@ -1063,14 +1076,14 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
             const R tmin = static_cast<R>(T.tmin), tmax = static_cast<R>(T.tmax), h = static_cast<R>(T.h);
             R u = -static_cast<R>(T.emin), du = 0;
             if (th >= tmin && th <= tmax) {
-                int lo = static_cast<int>(r_floor<R>((th - tmin) / h));
+                int lo = static_cast<int>(r_floor<R>(r_div<R>(th - tmin, h)));
                 lo = lo < 0 ? 0 : (lo > 8 ? 8 : lo);
                 const R xl = tmin + lo * h, xu = tmin + (lo + 1) * h;
-                const R A = (xu - th) / h, B = (th - xl) / h;
+                const R A = r_div<R>(xu - th, h), B = r_div<R>(th - xl, h);
                 const R ylo = static_cast<R>(T.y[lo]), yhi = static_cast<R>(T.y[lo + 1]);
                 const R dlo = static_cast<R>(T.y2[lo]), dhi = static_cast<R>(T.y2[lo + 1]);
-                u += A * ylo + B * yhi + ((A * A * A - A) * dlo + (B * B * B - B) * dhi) * h * h / R(6);
-                du += (yhi - ylo) / h + ((R(1) - R(3) * A * A) * dlo + (R(3) * B * B - R(1)) * dhi) * h / R(6);
+                u += A * ylo + B * yhi + r_div<R>(((A * A * A - A) * dlo + (B * B * B - B) * dhi) * h * h, R(6));
+                du += r_div<R>(yhi - ylo, h) + r_div<R>(((R(1) - R(3) * A * A) * dlo + (R(3) * B * B - R(1)) * dhi) * h, R(6));
             }
             if (tmin - th >= R(0)) { u += R(30) * (tmin - th) + static_cast<R>(T.y[0]); du -= R(30); }
             if (th - tmax >= R(0)) { u += R(30) * (th - tmax) + static_cast<R>(T.y[9]); du += R(30); }
@@ -1078,13 +1091,15 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
             dedt = k * du;
         }
         if (!ENERGY && pn > R(0)) {
-            const R ia = R(1) / (dot3(a, a) * pn), ic = R(1) / (dot3(c, c) * pn);
+            const R ia = r_div<R>(R(1), dot3(a, a) * pn), ic = r_div<R>(R(1), dot3(c, c) * pn);
             const V3T<R> ap = cross3(a, p), pc = cross3(p, c);
             const V3T<R> gi = {ap.x * ia, ap.y * ia, ap.z * ia};
             const V3T<R> gk = {pc.x * ic, pc.y * ic, pc.z * ic};
-            add_force<R>(out, t.idx[0], -dedt * gi.x, -dedt * gi.y, -dedt * gi.z, bad);
-            add_force<R>(out, t.idx[2], -dedt * gk.x, -dedt * gk.y, -dedt * gk.z, bad);
-            add_force<R>(out, t.idx[1], dedt * (gi.x + gk.x), dedt * (gi.y + gk.y), dedt * (gi.z + gk.z), bad);
+            const FixV fi = fix_v<R>(-dedt * gi.x, -dedt * gi.y, -dedt * gi.z, bad);
+            const FixV fk = fix_v<R>(-dedt * gk.x, -dedt * gk.y, -dedt * gk.z, bad);
+            add_fix(out, t.idx[0], fi.x, fi.y, fi.z);
+            add_fix(out, t.idx[2], fk.x, fk.y, fk.z);
+            add_fix(out, t.idx[1], -(fi.x + fk.x), -(fi.y + fk.y), -(fi.z + fk.z));
         }
         break;
     }
@@ -1100,8 +1115,8 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
             const R k = static_cast<R>(t.p[0]), p0 = static_cast<R>(t.p[1]), s = static_cast<R>(t.p[2]);
             const R pi = static_cast<R>(3.1415926535897932385);
             const R dt = phi - p0;
-            const R dtp = dt - r_floor<R>((dt + pi) / (R(2) * pi)) * (R(2) * pi);
-            const R is2 = R(1) / (s * s);
+            const R dtp = dt - r_floor<R>(r_div<R>(dt + pi, R(2) * pi)) * (R(2) * pi);
+            const R is2 = r_div<R>(R(1), s * s);
             e = k * r_exp<R>(-R(0.5) * dtp * dtp * is2);
             dedp = -e * dtp * is2;
         } else if (t.kind == 9) {
@@ -1128,19 +1143,22 @@ __device__ inline void bonded_term(const BTerm& t, const DevSys& S, const double
         if (!ENERGY) {
             const R n1s = dot3(n1, n1), n2s = dot3(n2, n2);
             if (n1s > R(0) && n2s > R(0) && b2n > R(0)) {
-                const R fi = -b2n / n1s, fl = b2n / n2s;
+                const R fi = r_div<R>(-b2n, n1s), fl = r_div<R>(b2n, n2s);
                 const V3T<R> gi = {n1.x * fi, n1.y * fi, n1.z * fi};
                 const V3T<R> gl = {n2.x * fl, n2.y * fl, n2.z * fl};
-                const R ib2 = R(1) / (b2n * b2n);
+                const R ib2 = r_div<R>(R(1), b2n * b2n);
                 const R f1 = dot3(b1, b2) * ib2, f3 = dot3(b3, b2) * ib2;
-                // middle beads: -(1 + f1) g_i + f3 g_l and f1 g_i - (1 + f3) g_l
-                const R a1 = -(R(1) + f1), a3 = -(R(1) + f3);
+                // middle beads: -(1 + f1) g_i + f3 g_l, and for bead k the
+                // remainder f1 g_i - (1 + f3) g_l (the four sum to zero)
+                const R a1 = -(R(1) + f1);
                 const V3T<R> gj = {gi.x * a1 + gl.x * f3, gi.y * a1 + gl.y * f3, gi.z * a1 + gl.z * f3};
-                const V3T<R> gk = {gi.x * f1 + gl.x * a3, gi.y * f1 + gl.y * a3, gi.z * f1 + gl.z * a3};
-                add_force<R>(out, t.idx[0], -dedp * gi.x, -dedp * gi.y, -dedp * gi.z, bad);
-                add_force<R>(out, t.idx[1], -dedp * gj.x, -dedp * gj.y, -dedp * gj.z, bad);
-                add_force<R>(out, t.idx[2], -dedp * gk.x, -dedp * gk.y, -dedp * gk.z, bad);
-                add_force<R>(out, t.idx[3], -dedp * gl.x, -dedp * gl.y, -dedp * gl.z, bad);
+                const FixV wi = fix_v<R>(-dedp * gi.x, -dedp * gi.y, -dedp * gi.z, bad);
+                const FixV wj = fix_v<R>(-dedp * gj.x, -dedp * gj.y, -dedp * gj.z, bad);
+                const FixV wl = fix_v<R>(-dedp * gl.x, -dedp * gl.y, -dedp * gl.z, bad);
+                add_fix(out, t.idx[0], wi.x, wi.y, wi.z);
+                add_fix(out, t.idx[1], wj.x, wj.y, wj.z);
+                add_fix(out, t.idx[2], -(wi.x + wj.x + wl.x), -(wi.y + wj.y + wl.y), -(wi.z + wj.z + wl.z));
+                add_fix(out, t.idx[3], wl.x, wl.y, wl.z);
             }
         }
         break;
