@@ -184,6 +184,10 @@ typedef struct ocm_options {
                                   bits as the standard path.  -1 (default):
                                   when the batch qualifies; 0: never;
                                   1: the same as -1                         */
+    int32_t entries_per_lane;  /* with team_size set: list entries each lane
+                                  of the pair kernel takes per round, 4
+                                  (0 = 4), or 2 with a team of 8, 16 or 32;
+                                  ignored when the shape is autotuned       */
 } ocm_options;
 
 /* getState selection bits */
@@ -316,8 +320,8 @@ int ocm_get_list_counts(ocm_context* ctx, int64_t* counts);
 int ocm_get_launch_shape(ocm_context* ctx, int32_t* team, int32_t* entries_per_lane,
                          int32_t* grid, int32_t* prune_team);
 /* Resident stepping: *threads receives the workgroup size each system steps
- * in (256, 512 or 1024), or 0 when the batch steps through the standard
- * path.  A diagnostic with no counterpart in the reference.               */
+ * in (1024), or 0 when the batch steps through the standard path.  A
+ * diagnostic with no counterpart in the reference.                        */
 int ocm_get_resident(ocm_context* ctx, int32_t* threads);
 
 /* ------------------------------------------------------------------ batch */

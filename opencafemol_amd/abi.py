@@ -89,7 +89,7 @@ class IntegratorDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("rebuild_interval", C.c_int32), ("skin", C.c_double),
                 ("team_size", C.c_int32), ("use_graph", C.c_int32), ("chunk_steps", C.c_int32),
-                ("prune_interval", C.c_int32), ("resident", C.c_int32)]
+                ("prune_interval", C.c_int32), ("resident", C.c_int32), ("entries_per_lane", C.c_int32)]
 
 
 class State(C.Structure):

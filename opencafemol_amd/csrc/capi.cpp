@@ -74,6 +74,7 @@ void ocm_options_default(ocm_options* o) {
     o->chunk_steps = 200;
     o->prune_interval = 5;
     o->resident = -1;
+    o->entries_per_lane = 0;
 }
 
 int ocm_batch_create(const ocm_system_desc* systems, const ocm_integrator_desc* integrators, int32_t n,

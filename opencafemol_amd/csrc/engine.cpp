@@ -225,6 +225,11 @@ Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(s
     if (opt_.rebuild_interval < 1) opt_.rebuild_interval = 1;
     if (opt_.chunk_steps < 1) opt_.chunk_steps = 200;
     if (!(opt_.skin >= 0.0)) invalid("skin must be >= 0");
+    if (opt_.team_size != 0) {
+        const int u = opt_.entries_per_lane, t = opt_.team_size;
+        if (u != 0 && u != 4 && !(u == 2 && (t == 8 || t == 16 || t == 32)))
+            invalid("entries_per_lane must be 4, or 2 with a team of 8, 16 or 32");
+    }
     interval_ = opt_.rebuild_interval;
     prune_ = opt_.prune_interval > 1 ? opt_.prune_interval : 0;
     int ndev = 0;
@@ -236,7 +241,15 @@ Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(s
     if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
         throw Error(OCM_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-    pack();
+    try {
+        pack();
+    } catch (...) {
+        // no destructor runs for a constructor that throws: free what the
+        // packing allocated (a server retrying after running out of memory
+        // would otherwise leak it)
+        release();
+        throw;
+    }
     // A stale list rolls the whole chunk back.  For large systems (a step of
     // milliseconds) chunks are capped at two list lifetimes, so a rollback
     // costs at most that; the host check per chunk stays negligible beside it.
@@ -244,15 +257,22 @@ Engine::Engine(std::vector<HostSystem> systems, const ocm_options& opt) : sys_(s
     if (d_.nm > kLargeMembers) chunk_ = std::min(chunk_, std::max(2 * opt_.rebuild_interval, 32));
 }
 
-Engine::~Engine() {
+Engine::~Engine() { release(); }
+
+void Engine::release() noexcept {
     // nothing may still run on the stream when its graphs and buffers go
     if (stream_) (void)hipStreamSynchronize(stream_);
     drop_graphs();
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+    event_pool_.clear();
     for (void* p : allocs_) (void)hipFree(p);
+    allocs_.clear();
     if (status_host_) (void)hipHostFree(status_host_);
+    status_host_ = nullptr;
     if (stamp_host_) (void)hipHostFree(stamp_host_);
+    stamp_host_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
 }
 
 // Callers guarantee the stream is idle (every chunk ends with a sync).
@@ -721,6 +741,7 @@ void Engine::pack() {
     if (team != 4 && team != 8 && team != 16 && team != 32 && team != 64) team = 16;
     build_pair_blocks(team);
     team_tuned_ = opt_.team_size != 0;
+    if (team_tuned_ && team == opt_.team_size && opt_.entries_per_lane != 0) d_.pair_u = opt_.entries_per_lane;
     if (opt_.resident != 0) pack_resident(terms, groups, types_f);
     excl_bead_.clear(); excl_start_h_.clear(); excl_count_h_.clear(); excl_h_.clear();
     // resident batches step outside the pair kernel: no launch shape to tune
@@ -1014,14 +1035,16 @@ void Engine::enqueue_forces(const ForceBuf& f, bool rebuild, uint32_t groups) {
 // its own age reaches the prune interval; other steps read the inner list.
 // Ages carry over from chunk to chunk, so the list lifetime does not depend
 // on how steps are grouped into calls.
-Engine::ChunkPlan Engine::plan_chunk(int k) const {
+Engine::ChunkPlan Engine::plan_chunk(int k) const { return plan_chunk_at(k, age_, page_, fresh_); }
+
+Engine::ChunkPlan Engine::plan_chunk_at(int k, int age_in, int page_in, int fresh_in) const {
     ChunkPlan p;
     p.k = k;
     p.interval = interval_;
     p.prune = prune_ > 1 ? prune_ : 0;
-    p.fresh = fresh_;
-    p.age0 = fresh_ ? 0 : age_;
-    p.page0 = fresh_ || !p.prune ? 0 : page_;
+    p.fresh = fresh_in;
+    p.age0 = fresh_in ? 0 : age_in;
+    p.page0 = fresh_in || !p.prune ? 0 : page_in;
     p.rebuild.assign(k, 0);
     p.pruned.assign(k, 0);
     int age = p.age0, page = p.page0;
@@ -1265,6 +1288,25 @@ hipGraphExec_t Engine::chunk_graph(const ChunkPlan& p, bool may_capture) {
     return it->second.exec;
 }
 
+bool Engine::needs_prepare(long long n) const {
+    if (n <= 0) return false;
+    if (forces_dirty_) return true;
+    if (d_.res_n > 0 || !opt_.use_graph || timing_) return false;
+    if (timing_mode_ == 2) return true;   // the stamp buffers may need sizing
+    int age = age_, page = page_, fresh = fresh_;
+    long long remaining = n;
+    while (remaining > 0) {
+        const int k = static_cast<int>(std::min<long long>(remaining, chunk_));
+        const ChunkPlan p = plan_chunk_at(k, age, page, fresh);
+        if (graphs_.find(p.key(0)) == graphs_.end()) return true;
+        age = p.age1;
+        page = p.page1;
+        fresh = 0;
+        remaining -= k;
+    }
+    return false;
+}
+
 void Engine::prepare(long long n) {
     if (n < 0) invalid("negative step count");
     if (n == 0) return;
@@ -1292,7 +1334,7 @@ int Engine::run_chunk(int k) {
     for (int attempt = 0; attempt < 16; ++attempt) {
         if (timing_mode_ == 2) ensure_stamps();
         const ChunkPlan p = plan_chunk(k);
-        hipGraphExec_t ge = chunk_graph(p, true);
+        hipGraphExec_t ge = chunk_graph(p, capture_in_step_);
         if (ge) check_hip(hipGraphLaunch(ge, stream_), "hipGraphLaunch");
         else enqueue_chunk(p);
         rebuilds_ += p.rebuilds;

@@ -76,6 +76,13 @@ public:
     // Captures the step graphs that step(n) will launch from the current
     // state (assuming no rollback), so that call spends no time on capture.
     void prepare(long long n);
+    // Whether prepare(n) has work to do: the first force evaluation, or a
+    // chunk of the coming n steps with no captured graph yet.  Host-side only.
+    bool needs_prepare(long long n) const;
+    // false: step runs a chunk that has no captured graph eagerly instead of
+    // capturing one.  For callers whose other threads make HIP calls while
+    // this engine steps: they capture through prepare, at a time they choose.
+    void set_capture_in_step(bool on) { capture_in_step_ = on; }
     // Capped steepest descent until the RMS force component of every system
     // is <= tol or max_iter iterations ran.  Returns iterations done.
     int minimize(double tol, int max_iter);
@@ -130,6 +137,7 @@ private:
         std::array<int, 7> key(int stamps) const { return {k, interval, prune, age0, page0, fresh, stamps}; }
     };
     ChunkPlan plan_chunk(int k) const;
+    ChunkPlan plan_chunk_at(int k, int age, int page, int fresh) const;
     void enqueue_chunk(const ChunkPlan& p);
     hipGraphExec_t chunk_graph(const ChunkPlan& p, bool may_capture);
     int run_chunk(int k);
@@ -141,6 +149,7 @@ private:
     void grow_rows();
     void autotune_team();
     void drop_graphs();
+    void release() noexcept;   // frees every device and pinned buffer, the graphs and the stream
     template <typename F> void timed(int slot, F&& f);
     void collect_timing();
 
@@ -163,6 +172,7 @@ private:
     std::vector<int> err_carry_;     // flags of evaluations outside a chunk, reported with the next one
     std::vector<int64_t> sys_step_origin_;
     bool forces_dirty_ = true;
+    bool capture_in_step_ = true;
     // resident steps leave the standard path's lists behind the positions:
     // the next evaluation that reads them rebuilds first
     bool lists_stale_ = false;

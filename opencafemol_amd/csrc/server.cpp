@@ -283,6 +283,7 @@ void Server::repack(Lane& L) {
     for (auto& j : members) systems.push_back(j->sys);
     try {
         L.engine = std::make_unique<Engine>(std::move(systems), opt_.engine);
+        L.engine->set_capture_in_step(false);
         for (std::size_t s = 0; s < members.size(); ++s) {
             auto& j = members[s];
             L.engine->set_positions(static_cast<int>(s), j->x.data());
@@ -357,11 +358,23 @@ void Server::stage_frames(Lane& L, const std::vector<int>& systems) {
         sl.ranges.emplace_back(L.engine->bead_begin(s), L.engine->n_particles(s));
     }
     const std::size_t bytes = 3 * sl.nb * sizeof(double);
-    L.engine->enqueue_position_copy(sl.dev);
-    hip_ok(hipEventRecord(sl.copied, L.engine->stream()), "event");
-    hip_ok(hipStreamWaitEvent(L.copy_stream, sl.copied, 0), "wait");
-    hip_ok(hipMemcpyAsync(sl.host, sl.dev, bytes, hipMemcpyDeviceToHost, L.copy_stream), "frame copy");
-    hip_ok(hipEventRecord(sl.staged, L.copy_stream), "event");
+    gate_enter(false);
+    try {
+        L.engine->enqueue_position_copy(sl.dev);
+        hip_ok(hipEventRecord(sl.copied, L.engine->stream()), "event");
+        hip_ok(hipStreamWaitEvent(L.copy_stream, sl.copied, 0), "wait");
+        hip_ok(hipMemcpyAsync(sl.host, sl.dev, bytes, hipMemcpyDeviceToHost, L.copy_stream), "frame copy");
+        hip_ok(hipEventRecord(sl.staged, L.copy_stream), "event");
+    } catch (...) {
+        gate_exit(false);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            sl.busy = false;
+        }
+        slot_cv_.notify_all();
+        throw;
+    }
+    gate_exit(false);
     {
         std::lock_guard<std::mutex> lk(mu_);
         for (int s : systems) ++L.packed[s]->frames_pending;
@@ -384,7 +397,9 @@ void Server::frame_loop() {
             staged_.pop_front();
         }
         FrameSlot& sl = lanes_[lane].slots[slot];
+        gate_enter(false);
         (void)hipEventSynchronize(sl.staged);
+        gate_exit(false);
         std::vector<ServerFrame> out(sl.jobs.size());
         const double* hx = sl.host;
         const double* hy = sl.host + sl.nb;
@@ -430,6 +445,20 @@ void Server::fail_lane(Lane& L, int code, const std::string& msg) {
 
 void Server::loop(Lane& L) {
     (void)hipSetDevice(opt_.engine.device);
+    // Gate discipline: no thread holds the gate while it waits for another
+    // thread (a free frame slot, the frame thread, the caller), so every
+    // holder leaves in finite time.  HIP work runs in the shared gate; graph
+    // capture, repacks and teardown in the exclusive one, after this lane's
+    // snapshots have been unpacked (the frame thread needs the shared gate).
+    auto fail = [&](int code, const std::string& msg) {
+        drain_slots(L);
+        gate_enter(true);
+        fail_lane(L, code, msg);
+        gate_exit(true);
+    };
+    auto code_of = [](const std::exception& e) {
+        return dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : OCM_ERR_HIP;
+    };
     while (true) {
         bool need_repack;
         {
@@ -447,15 +476,21 @@ void Server::loop(Lane& L) {
         }
         const double t = now_ms();
         if (need_repack) {
+            drain_slots(L);   // snapshots in flight index the old batch's beads
+            bool ok = true;
+            std::string msg;
             gate_enter(true);
             try {
                 repack(L);
             } catch (const std::exception& e) {
-                fail_lane(L, OCM_ERR_HIP, e.what());
-                gate_exit(true);
-                continue;
+                ok = false;
+                msg = e.what();
             }
             gate_exit(true);
+            if (!ok) {
+                fail(OCM_ERR_HIP, msg);
+                continue;
+            }
         }
         if (!L.engine) {
             std::lock_guard<std::mutex> lk(mu_);
@@ -473,48 +508,80 @@ void Server::loop(Lane& L) {
         bool failed = false;
         int fail_code = OCM_OK;
         std::string fail_msg;
-        gate_enter(false);
-        try {
-            L.engine->step(n);
-            for (std::size_t s = 0; s < L.packed.size(); ++s) {
-                auto& j = L.packed[s];
-                j->done += n;
-                const int st = L.engine->system_status(static_cast<int>(s));
-                if (st != OCM_OK) { end_sys.push_back(static_cast<int>(s)); continue; }
-                if (j->frame_interval > 0 && j->done % j->frame_interval == 0) frame_sys.push_back(static_cast<int>(s));
-                if (j->done >= j->nsteps) end_sys.push_back(static_cast<int>(s));
-            }
-            if (!frame_sys.empty()) stage_frames(L, frame_sys);
-            for (int s : end_sys) {
-                auto& j = L.packed[s];
-                ocm_state st;
-                std::memset(&st, 0, sizeof(st));
-                st.positions = j->x.data();
-                st.velocities = j->v.data();
-                L.engine->get_state(s, OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
-                const int status = L.engine->system_status(s);
-                double dt_ps = j->sys.integ.dt;
-                {
-                    std::lock_guard<std::mutex> lk(mu_);
-                    simulated_ns_ += j->done * dt_ps * 1e-3;
-                }
-                finish_job(j, status, status == OCM_OK ? "" : "NaN or fixed-point force overflow detected");
-            }
-        } catch (const std::exception& e) {
-            failed = true;
-            fail_code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : OCM_ERR_HIP;
-            fail_msg = e.what();
-        }
-        gate_exit(false);
-        if (failed) {
+        // graphs the stretch needs are captured with the other threads held
+        // off (their HIP calls would invalidate a capture in progress); the
+        // engine never captures inside step
+        if (L.engine->needs_prepare(n)) {
             gate_enter(true);
-            fail_lane(L, fail_code, fail_msg);
+            try {
+                L.engine->prepare(n);
+            } catch (const std::exception& e) {
+                failed = true;
+                fail_code = code_of(e);
+                fail_msg = e.what();
+            }
             gate_exit(true);
         }
+        if (!failed) {
+            gate_enter(false);
+            try {
+                L.engine->step(n);
+                for (std::size_t s = 0; s < L.packed.size(); ++s) {
+                    auto& j = L.packed[s];
+                    j->done += n;
+                    const int st = L.engine->system_status(static_cast<int>(s));
+                    if (st != OCM_OK) { end_sys.push_back(static_cast<int>(s)); continue; }
+                    if (j->frame_interval > 0 && j->done % j->frame_interval == 0)
+                        frame_sys.push_back(static_cast<int>(s));
+                    if (j->done >= j->nsteps) end_sys.push_back(static_cast<int>(s));
+                }
+            } catch (const std::exception& e) {
+                failed = true;
+                fail_code = code_of(e);
+                fail_msg = e.what();
+            }
+            gate_exit(false);
+        }
+        if (!failed && !frame_sys.empty()) {
+            try {
+                stage_frames(L, frame_sys);   // waits for a slot outside the gate
+            } catch (const std::exception& e) {
+                failed = true;
+                fail_code = code_of(e);
+                fail_msg = e.what();
+            }
+        }
+        if (!failed && !end_sys.empty()) {
+            gate_enter(false);
+            try {
+                for (int s : end_sys) {
+                    auto& j = L.packed[s];
+                    ocm_state st;
+                    std::memset(&st, 0, sizeof(st));
+                    st.positions = j->x.data();
+                    st.velocities = j->v.data();
+                    L.engine->get_state(s, OCM_STATE_POSITIONS | OCM_STATE_VELOCITIES, 0u, &st);
+                    const int status = L.engine->system_status(s);
+                    double dt_ps = j->sys.integ.dt;
+                    {
+                        std::lock_guard<std::mutex> lk(mu_);
+                        simulated_ns_ += j->done * dt_ps * 1e-3;
+                    }
+                    finish_job(j, status, status == OCM_OK ? "" : "NaN or fixed-point force overflow detected");
+                }
+            } catch (const std::exception& e) {
+                failed = true;
+                fail_code = code_of(e);
+                fail_msg = e.what();
+            }
+            gate_exit(false);
+        }
+        if (failed) fail(fail_code, fail_msg);
         std::lock_guard<std::mutex> lk(mu_);
         busy_ms_ += now_ms() - t;
     }
     // shutdown: let the frame thread finish the snapshots in flight, then free
+    drain_slots(L);
     gate_enter(true);
     release_slots(L);
     L.engine.reset();

@@ -111,11 +111,13 @@ private:
     double now_ms() const;
     std::size_t estimate_bytes(const HostSystem& s) const;
     int active_jobs_locked() const;
-    // Device maintenance gate.  hipFree synchronizes the whole device, so a
-    // repack freeing an engine while the other lane keeps submitting would
-    // wait for one of its stretches per free.  A lane steps inside the gate
-    // (shared); a repack or teardown holds it alone, so it waits for the
-    // other lane's stretch once and the other lane pauses meanwhile.
+    // Device gate.  HIP work of the lanes and the frame thread runs inside it
+    // (shared); graph capture, repacks and teardown hold it alone.  A HIP call
+    // from another thread during a capture can invalidate the capture, and
+    // hipFree synchronizes the whole device, so a repack freeing an engine
+    // while the other lane kept submitting would wait for one of its
+    // stretches per free.  Held alone, the gate waits for the other threads'
+    // current HIP work once and keeps them out until it is released.
     void gate_enter(bool exclusive);
     void gate_exit(bool exclusive);
 

@@ -71,6 +71,21 @@ def _create(system):
     return code, lib.ocm_last_error().decode()
 
 
+def test_entries_per_lane_needs_a_matching_team():
+    # 2 or 1 entries per lane exist for teams of 8, 16 and 32 only: the
+    # options are checked before any device is touched
+    lib = abi.load_library()
+    s = SystemSpec(masses=np.ones(3))
+    sd, keep = s.descriptor()
+    it = IntegratorSpec(300.0, 0.5, 0.01).descriptor()
+    o = abi.Options()
+    lib.ocm_options_default(C.byref(o))
+    o.team_size, o.entries_per_lane = 64, 2
+    h = C.c_void_p()
+    assert lib.ocm_context_create(C.byref(sd), C.byref(it), C.byref(o), C.byref(h)) == abi.OCM_ERR_INVALID
+    assert "entries_per_lane" in lib.ocm_last_error().decode()
+
+
 def test_index_out_of_range_is_rejected():
     # ReadTOMLForceFieldGenerator.cpp:48-55 throws on an index beyond the system
     s = SystemSpec(masses=np.ones(3))

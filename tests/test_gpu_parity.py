@@ -243,14 +243,21 @@ def test_rebuild_interval_does_not_change_bits(oracle):
 
 
 def test_team_sizes_give_identical_forces(oracle):
+    # every launch shape the autotuner may pick: lanes per row, and list
+    # entries per lane per round
     s, it, x = _small_hps(5)
     ref = None
-    for team in (4, 8, 16, 32, 64):
-        with _ctx(s, it, x, team_size=team) as c:
+    shapes = [(t, 4) for t in (4, 8, 16, 32, 64)] + [(t, 2) for t in (8, 16, 32)]
+    for team, u in shapes:
+        with _ctx(s, it, x, team_size=team, entries_per_lane=u) as c:
             F = c.get_state(abi.STATE_FORCES)["forces"]
+            shape = c.launch_shape()
+        assert (shape["team"], shape["entries_per_lane"]) == (team, u)
         if ref is None:
             ref = F
-        assert np.array_equal(F, ref)
+            Fo, _ = oracle.evaluate(s, x)
+            assert_forces_close(F, Fo)
+        assert np.array_equal(F, ref), (team, u)
 
 
 @pytest.mark.parametrize("resident", [0, -1])

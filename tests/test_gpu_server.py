@@ -167,3 +167,27 @@ def test_server_numeric_failure_stays_in_its_job():
     assert stats["failed"] == 1 and stats["completed"] == 3
     good = [0, 2, 3]
     _assert_same([mixed[k] for k in good], [ids[k] for k in good], frames, results)
+
+
+def test_server_shutdown_with_both_lanes_running():
+    # resident-sized Go jobs (one lane) and a 1,500-bead slab (the other) are
+    # mid-run, with frames left uncollected until the workers wait on the
+    # frame queue: destroying the server stops both lanes and returns
+    import time
+    reps = configs.go_replicas(3, n_min=60, n_max=200, seed=2)
+    s, it, x = configs.hps_slab_of_size(1500, seed=1)
+    with Context(s, it) as c:
+        c.set_positions(x)
+        c.minimize(tolerance=50.0, max_iterations=400)
+        x = c.get_state(abi.STATE_POSITIONS)["positions"]
+    with Server(max_queued_frames=2, quantum_steps=200) as srv:
+        ids = [srv.submit(sr, itr, xr, None, nsteps=10**6, frame_interval=100) for sr, itr, xr in reps]
+        ids.append(srv.submit(s, it, x, None, nsteps=10**6, frame_interval=100))
+        t0 = time.perf_counter()
+        while srv.stats()["frames"] < 2 and time.perf_counter() - t0 < 60:
+            time.sleep(0.05)
+        st = srv.stats()
+        ended = {j: r["status"] for j in ids for r in [srv.wait(j, timeout_ms=0)] if r is not None}
+        assert st["frames"] >= 2 and st["active"] == 4 and not ended, (st, ended, abi.load_library().ocm_last_error())
+        t0 = time.perf_counter()
+    assert time.perf_counter() - t0 < 60
