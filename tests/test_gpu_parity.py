@@ -305,3 +305,32 @@ def test_large_list_build_pairs():
             assert np.isin(inner, g).all(), (f.name, int((~np.isin(inner, g)).sum()))
             assert np.isin(g, outer).all(), (f.name, int((~np.isin(g, outer)).sum()))
             assert len(inner) > 100_000
+
+
+@pytest.mark.parametrize("resident", [0, -1])
+@pytest.mark.parametrize("case", ["go", "hps"])
+def test_zero_friction_conserves_energy(case, resident):
+    # with no friction BAOAB is velocity Verlet: over 3,000 steps of 2 fs the
+    # total energy (FP64 energy pass + kinetic) stays put, which ties the
+    # force pass to the energy pass along a trajectory on both step paths
+    from opencafemol_amd.model import IntegratorSpec
+    if case == "go":
+        s, _, x = configs.go_chain(62, seed=3)   # the native structure: thermal motion only
+    else:
+        s, it0, x = _small_hps(3)
+        with Context(s, it0) as c:
+            c.set_positions(x)
+            c.minimize(tolerance=20.0, max_iterations=500)
+            x = c.get_state(abi.STATE_POSITIONS)["positions"]
+    it = IntegratorSpec(300.0, 0.0, 0.002, 5)
+    with Context(s, it, resident=resident) as c:
+        c.set_positions(x)
+        c.set_velocities_to_temperature(300.0, 5)
+        assert c.resident_threads() == (1024 if resident else 0)
+        st = c.get_state(abi.STATE_ENERGY)
+        e0, k0 = st["potential"] + st["kinetic"], st["kinetic"]
+        for _ in range(6):
+            assert c.step(500) == abi.OCM_OK
+            st = c.get_state(abi.STATE_ENERGY)
+            e = st["potential"] + st["kinetic"]
+            assert abs(e - e0) < 2e-3 * (abs(e0) + k0), (case, resident, st["step"], e, e0, k0)
