@@ -112,6 +112,7 @@ constexpr int kResMaxTypeRows = 64;     // staged type rows per system (6 bits)
 // wrapped into boxes of at most kResWide nm, open systems' relative to bead 0
 // (a build that finds one beyond kResWide tests that build's pairs in FP64).
 constexpr double kResWide = 1000.0;
+constexpr int kResMaxSeg = 9;   // bonded kinds 1-9: one segment each
 struct ResSys {
     int sys;          // DevSys index
     int nb;           // beads
@@ -120,6 +121,10 @@ struct ResSys {
     int g0, ng;       // resident pair groups [g0, g0 + ng)
     int row0, nrow;   // staged type rows [row0, row0 + nrow) of the resident row array
     int ex0, exn;     // the groups' exclusion tables: words [ex0, ex0 + exn) of the resident exclusion array
+    // the terms in runs of one kind: run q is terms [seg_t[q], seg_t[q] + seg_n[q])
+    // of the system's own (kind-ordered) list
+    int nseg;
+    int seg_t[kResMaxSeg], seg_n[kResMaxSeg];
 };
 // A group's exclusions in member order (members follow bead order), for the
 // in-kernel build, which tests member pairs (i, j > i): per member a mask

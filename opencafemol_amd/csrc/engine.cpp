@@ -797,10 +797,19 @@ void Engine::pack_resident(const std::vector<BTerm>& terms, const std::vector<De
         R.b0 = bead_begin_[s];
         R.t0 = static_cast<int>(rterms.size());
         R.nt = static_cast<int>(sys_terms[s].size());
+        R.nseg = 0;
         for (int t : sys_terms[s]) {   // kind order is kept within the system
             BTerm T = terms[t];
             for (int k = 0; k < 4; ++k)
                 if (T.idx[k] >= 0) T.idx[k] -= R.b0;
+            const int local = static_cast<int>(rterms.size()) - R.t0;
+            if (R.nseg == 0 || rterms.back().kind != T.kind) {
+                if (R.nseg == kResMaxSeg) throw Error(OCM_ERR_INVALID, "resident terms are not in kind order");
+                R.seg_t[R.nseg] = local;
+                R.seg_n[R.nseg] = 0;
+                ++R.nseg;
+            }
+            ++R.seg_n[R.nseg - 1];
             rterms.push_back(T);
         }
         R.g0 = static_cast<int>(grp.size());

@@ -2307,22 +2307,33 @@ __device__ __forceinline__ void res_forces(const ResArgs& a, const ResSys& R, co
     static_assert(sizeof(BTerm) == 48, "a term is three 16-byte loads");
     const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<BTerm*>(a.terms + R.t0), 0, R.nt * static_cast<int>(sizeof(BTerm)), 0x00020000);
-    for (int t = threadIdx.x; t < R.nt; t += T) {
-        BTerm tm;
-        if (t < nl) {
-            tm = L.terms[t];
-        } else {
-            uint32_t w[12];
+    // Each run of one bonded kind, and each pair group below, starts its
+    // share of the work on a wavefront boundary (the work sequence is padded
+    // to multiples of 64), so no wavefront runs two kinds' code one after the
+    // other.  The run table is read through the global record (uniform
+    // scalar loads) rather than indexed in registers.
+    const ResSys* Rg = a.tab + blockIdx.x;
+    int off = 0;
+    for (int q = 0; q < R.nseg; ++q) {
+        const int s0 = Rg->seg_t[q], n = Rg->seg_n[q];
+        for (int v = res_first<T>(off); v < n; v += T) {
+            const int t = s0 + v;
+            BTerm tm;
+            if (t < nl) {
+                tm = L.terms[t];
+            } else {
+                uint32_t w[12];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const auto v = __builtin_amdgcn_raw_buffer_load_b128(trs, t * 48 + 16 * k, 0, 0);
-                w[4 * k] = v[0]; w[4 * k + 1] = v[1]; w[4 * k + 2] = v[2]; w[4 * k + 3] = v[3];
+                for (int k = 0; k < 3; ++k) {
+                    const auto u = __builtin_amdgcn_raw_buffer_load_b128(trs, t * 48 + 16 * k, 0, 0);
+                    w[4 * k] = u[0]; w[4 * k + 1] = u[1]; w[4 * k + 2] = u[2]; w[4 * k + 3] = u[3];
+                }
+                __builtin_memcpy(&tm, w, sizeof(tm));
             }
-            __builtin_memcpy(&tm, w, sizeof(tm));
+            bonded_term<false, float>(tm, S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
         }
-        bonded_term<false, float>(tm, S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
+        off += (n + 63) & ~63;
     }
-    int off = R.nt;
     for (int g = 0; g < R.ng; ++g) {
         const ResGroupK& K = L.gk[g];
         switch (K.spec) {
@@ -2333,7 +2344,7 @@ __device__ __forceinline__ void res_forces(const ResArgs& a, const ResSys& R, co
         case kSpecSolo: res_pairs<kSpecSolo, T>(a, K, off, L, bad); break;
         default: res_pairs<kSpecGeneric, T>(a, K, off, L, bad); break;
         }
-        off += K.cnt;
+        off += (K.cnt + 63) & ~63;
     }
     if (noise) {
         for (int b = res_first<T>(off); b < R.nb; b += T) {
