@@ -2181,61 +2181,103 @@ __device__ void res_build(const ResArgs& a, const ResSys& R, const DevSys& S, co
         const int* far = exm + 2 * nm + 1;
         // consecutive threads take different row pairs: the entries one wave
         // appends together (consecutive list slots, read together by a wave of
-        // the force pass) then hit different force words
+        // the force pass) then hit different force words.  A wavefront's kept
+        // candidates take their slots with one counter atomic.
+        const int lane = tid & 63;
+        auto append = [&](bool keep, uint32_t w) {
+            const unsigned long long m = __ballot(keep);   // over the active lanes
+            if (m == 0ull) return;                         // the same for every active lane
+            const int leader = __ffsll(static_cast<long long>(m)) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&K.cnt, __popcll(m));
+            base = __shfl(base, leader, 64);
+            if (keep) {
+                const int k = base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                         static_cast<uint32_t>(m >> 32),
+                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+                if (k < lds_room) L.list[lds_base + k] = w;
+                else glist[k] = w;
+            }
+        };
+        auto excluded = [&](int i, int j, uint32_t near_mask, int f0, int f1) {
+            const int d = j - i;
+            if (d <= kResNearExcl) return ((near_mask >> (d - 1)) & 1u) != 0u;
+            bool hit = false;
+            for (int k = f0; k < f1; ++k) hit |= far[k] == j;
+            return hit;
+        };
         for (int u = tid; u < nrp * nseg; u += T) {
             const int r = u % nrp, sg = u / nrp;
             const int r2 = nm - 1 - r;
-            const int len = r2 == r ? nm - 1 - r : nm - 1;   // the middle row of an odd count stands alone
+            const int split = nm - 1 - r;   // q < split: row r, candidate r + 1 + q; else row r2, q + 1
+            const int len = r2 == r ? split : nm - 1;   // the middle row of an odd count stands alone
             const int q0 = sg * seglen, q1 = min(q0 + seglen, len);
-            int i = -1, bi = 0, ti = 0, f0 = 0, f1 = 0;
-            uint32_t allow = 0u, near_mask = 0u;
-            float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int q = q0; q < q1; ++q) {
-                const int ii = q < nm - 1 - r ? r : r2;
-                const int j = q < nm - 1 - r ? r + 1 + q : q + 1;   // row r2's candidates are r2+1 .. nm-1
-                if (ii != i) {
-                    i = ii;
-                    const int mi = L.mem[i];
-                    bi = mi & 1023;
-                    ti = (mi >> 10) & 63;
-                    allow = G.allow[mi >> 16];
-                    near_mask = static_cast<uint32_t>(exm[i]);
-                    f0 = exm[nm + i];
-                    f1 = exm[nm + i + 1];
-                    pi = L.p32[bi];
-                }
-                const int mj = L.mem[j];
-                const int bj = mj & 1023;
-                bool keep;
-                if (!wide) {
-                    const float4 pj = L.p32[bj];
-                    float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                    if (per) {
-                        dx = fmaf(-Lx, rintf(dx * iLx), dx);
-                        dy = fmaf(-Ly, rintf(dy * iLy), dy);
-                        dz = fmaf(-Lz, rintf(dz * iLz), dz);
+            if (q0 >= q1) continue;
+            // both rows' values: a segment crosses from row r to row r2 at most once
+            const int ma = L.mem[r], mb = L.mem[r2];
+            const int bia = ma & 1023, bib = mb & 1023;
+            const int tia = (ma >> 10) & 63, tib = (mb >> 10) & 63;
+            const uint32_t alla = G.allow[ma >> 16], allb = G.allow[mb >> 16];
+            const uint32_t nma = static_cast<uint32_t>(exm[r]), nmb = static_cast<uint32_t>(exm[r2]);
+            const int f0a = exm[nm + r], f1a = exm[nm + r + 1], f0b = exm[nm + r2], f1b = exm[nm + r2 + 1];
+            if (!wide) {
+                const float4 pa = L.p32[bia], pb = L.p32[bib];
+                // four candidates a round, their loads issued before any test
+                for (int q = q0; q < q1; q += 4) {
+                    int mj[4];
+                    float4 pj[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int qq = min(q + k, q1 - 1);   // past the end: the last again, not kept
+                        mj[k] = L.mem[qq < split ? r + 1 + qq : qq + 1];
                     }
-                    keep = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < K.rl2w;
-                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) pj[k] = L.p32[mj[k] & 1023];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int qq = q + k;
+                        const bool first = qq < split;
+                        const float4 pi = first ? pa : pb;
+                        float dx = pj[k].x - pi.x, dy = pj[k].y - pi.y, dz = pj[k].z - pi.z;
+                        if (per) {
+                            dx = fmaf(-Lx, rintf(dx * iLx), dx);
+                            dy = fmaf(-Ly, rintf(dy * iLy), dy);
+                            dz = fmaf(-Lz, rintf(dz * iLz), dz);
+                        }
+                        const uint32_t allow = first ? alla : allb;
+                        bool keep = qq < q1 && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < K.rl2w &&
+                                    ((allow >> (mj[k] >> 16)) & 1u);
+                        const int i = first ? r : r2, j = first ? r + 1 + qq : qq + 1;
+                        if (keep) keep = !excluded(i, j, first ? nma : nmb, first ? f0a : f0b, first ? f1a : f1b);
+                        const uint32_t w = static_cast<uint32_t>(first ? bia : bib) |
+                                           (static_cast<uint32_t>(mj[k] & 1023) << 10) |
+                                           (static_cast<uint32_t>(first ? tia : tib) << 20) |
+                                           (static_cast<uint32_t>((mj[k] >> 10) & 63) << 26);
+                        append(keep, w);
+                    }
+                }
+            } else {
+                // a coordinate beyond kResWide: the FP64 test, one candidate at a time
+                for (int q = q0; q < q1; ++q) {
+                    const bool first = q < split;
+                    const int i = first ? r : r2, j = first ? r + 1 + q : q + 1;
+                    const int bi = first ? bia : bib;
+                    const int mj = L.mem[j];
+                    const int bj = mj & 1023;
                     double dx = L.x[bj] - L.x[bi], dy = L.y[bj] - L.y[bi], dz = L.z[bj] - L.z[bi];
                     if (per) {
                         dx = min_image(dx, G.box[0], G.inv_box[0]);
                         dy = min_image(dy, G.box[1], G.inv_box[1]);
                         dz = min_image(dz, G.box[2], G.inv_box[2]);
                     }
-                    keep = dist2(dx, dy, dz) < K.rl2;
+                    const uint32_t allow = first ? alla : allb;
+                    bool keep = dist2(dx, dy, dz) < K.rl2 && ((allow >> (mj >> 16)) & 1u);
+                    if (keep) keep = !excluded(i, j, first ? nma : nmb, first ? f0a : f0b, first ? f1a : f1b);
+                    const uint32_t w = static_cast<uint32_t>(bi) | (static_cast<uint32_t>(bj) << 10) |
+                                       (static_cast<uint32_t>(first ? tia : tib) << 20) |
+                                       (static_cast<uint32_t>((mj >> 10) & 63) << 26);
+                    append(keep, w);
                 }
-                if (!keep || !((allow >> (mj >> 16)) & 1u)) continue;
-                const int d = j - i;
-                bool hit = false;
-                if (d <= kResNearExcl) hit = (near_mask >> (d - 1)) & 1u;
-                else for (int k = f0; k < f1; ++k) hit |= far[k] == j;
-                if (hit) continue;
-                const uint32_t w = static_cast<uint32_t>(bi) | (static_cast<uint32_t>(bj) << 10) |
-                                   (static_cast<uint32_t>(ti) << 20) | (static_cast<uint32_t>((mj >> 10) & 63) << 26);
-                const int k = atomicAdd(&K.cnt, 1);
-                if (k < lds_room) L.list[lds_base + k] = w;
-                else glist[k] = w;
             }
         }
         __syncthreads();   // the count is final; the next group reuses mem
