@@ -4,7 +4,9 @@
 
 The default (`all`) prints one JSON line whose `value` is the ns/day of the
 HPS slab (100 chains x 140 beads = 14,000 beads, periodic 15 x 15 x 150 nm,
-dt 10 fs, 300 K), with two more keys for the metric's other parts:
+dt 10 fs, 300 K), with more keys for the metric's other parts:
+  - `go_chain_62`: ns/day of the 62-bead Go chain alone (20,000 timed steps;
+    GPU only);
   - `replicas_256`: aggregate ns/day of 256 Go replicas (50-500 beads,
     280-400 K ladder) packed in one batch, against the same replicas stepped
     one at a time through single-system contexts;
@@ -608,6 +610,12 @@ def main():
         raise SystemExit(f"unknown workload {args.workload}")
 
     out = single_run(args, "hps")
+    # the 62-bead Go chain alone (GPU rate only; its CPU baseline is the go workload's own line)
+    go_args = argparse.Namespace(**dict(vars(args), steps=20000, warmup=2000, no_cpu=True))
+    go = single_run(go_args, "go")
+    out["go_chain_62"] = {"ns_day": go["value"], "us_per_step": go["ms_per_step"] * 1000.0,
+                          "steps": go["steps"], "resident_threads": go["kernels"].get("resident_threads", 0),
+                          "config": go["config"]}
     out["replicas_256"] = replicas_run(args, args.replica_steps, 500,
                                        0 if args.no_serial else args.replica_serial_steps)
     out["served_stream"] = serve_run(args, args.serve_jobs, steps_range, args.serve_serial_frames,
