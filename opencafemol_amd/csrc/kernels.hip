@@ -2299,6 +2299,9 @@ __device__ __forceinline__ int res_first(int off) {
 
 template <int SPEC, int T>
 __device__ __forceinline__ void res_pairs(const ResArgs& a, const ResGroupK& K, int off, const ResLds& L, int& bad) {
+    // a thread with no entry of this group reads none of its constants
+    const int e0 = res_first<T>(off);
+    if (e0 >= K.cnt) return;
     const uint32_t* glist = a.list + K.l0;
     const uint32_t* llist = L.list + K.lds_off;
     const int cnt = K.cnt, lcnt = K.lds_cnt;
@@ -2330,7 +2333,7 @@ __device__ __forceinline__ void res_pairs(const ResArgs& a, const ResGroupK& K, 
     // a generic load, and the entry body is inlined once)
     const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint32_t*>(glist), 0, cnt * static_cast<int>(sizeof(uint32_t)), 0x00020000);
-    for (int e = res_first<T>(off); e < cnt; e += T) {
+    for (int e = e0; e < cnt; e += T) {
         uint32_t w;
         if (e < lcnt) w = llist[e];
         else w = static_cast<uint32_t>(__builtin_amdgcn_raw_buffer_load_b32(grs, e * 4, 0, 0));
@@ -2352,29 +2355,36 @@ __device__ __forceinline__ void res_forces(const ResArgs& a, const ResSys& R, co
     // Each run of one bonded kind, and each pair group below, starts its
     // share of the work on a wavefront boundary (the work sequence is padded
     // to multiples of 64), so no wavefront runs two kinds' code one after the
-    // other.  The run table is read through the global record (uniform
-    // scalar loads) rather than indexed in registers.
-    const ResSys* Rg = a.tab + blockIdx.x;
+    // other.  A slot finds its run by an unrolled walk over the run table,
+    // which the system record holds in scalar registers (a loop indexing it
+    // would load it from memory once per run and step).
     int off = 0;
-    for (int q = 0; q < R.nseg; ++q) {
-        const int s0 = Rg->seg_t[q], n = Rg->seg_n[q];
-        for (int v = res_first<T>(off); v < n; v += T) {
-            const int t = s0 + v;
-            BTerm tm;
-            if (t < nl) {
-                tm = L.terms[t];
-            } else {
-                uint32_t w[12];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const auto u = __builtin_amdgcn_raw_buffer_load_b128(trs, t * 48 + 16 * k, 0, 0);
-                    w[4 * k] = u[0]; w[4 * k + 1] = u[1]; w[4 * k + 2] = u[2]; w[4 * k + 3] = u[3];
-                }
-                __builtin_memcpy(&tm, w, sizeof(tm));
+    for (int q = 0; q < kResMaxSeg; ++q)
+        if (q < R.nseg) off += (R.seg_n[q] + 63) & ~63;
+    for (int v = threadIdx.x; v < off; v += T) {
+        int t = -1, base = 0;
+#pragma unroll
+        for (int q = 0; q < kResMaxSeg; ++q) {
+            if (q < R.nseg) {
+                if (v >= base && v < base + R.seg_n[q]) t = R.seg_t[q] + (v - base);
+                base += (R.seg_n[q] + 63) & ~63;
             }
-            bonded_term<false, float>(tm, S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
         }
-        off += (n + 63) & ~63;
+        if (t < 0) continue;   // padding of a run
+        BTerm tm;
+        if (t < nl) {
+            tm = L.terms[t];
+        } else {
+            uint32_t w[12];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const auto u = __builtin_amdgcn_raw_buffer_load_b128(trs, t * 48 + 16 * k, 0, 0);
+                w[4 * k] = u[0]; w[4 * k + 1] = u[1]; w[4 * k + 2] = u[2]; w[4 * k + 3] = u[3];
+            }
+            __builtin_memcpy(&tm, w, sizeof(tm));
+        }
+        bonded_term<false, float>(tm, S, L.x, L.y, L.z, a.fla, a.fld, out, nullptr, R.sys, bad);
     }
     for (int g = 0; g < R.ng; ++g) {
         const ResGroupK& K = L.gk[g];
