@@ -334,3 +334,32 @@ def test_zero_friction_conserves_energy(case, resident):
             st = c.get_state(abi.STATE_ENERGY)
             e = st["potential"] + st["kinetic"]
             assert abs(e - e0) < 2e-3 * (abs(e0) + k0), (case, resident, st["step"], e, e0, k0)
+
+
+@pytest.mark.parametrize("resident", [0, -1])
+def test_linear_chain_trajectory_matches_oracle_over_2000_steps(oracle, resident):
+    # a harmonic chain is a linear system: the FP32-force and FP32-noise
+    # differences (~1e-6 relative) stay bounded instead of growing as in a
+    # chaotic one, so the GPU trajectory must follow the FP64 oracle's (the
+    # same Philox stream) for thousands of steps
+    from opencafemol_amd.model import ForceSpec, IntegratorSpec, SystemSpec
+    n = 24
+    s = SystemSpec(masses=np.full(n, 60.0))
+    b = np.stack([np.arange(n - 1), np.arange(1, n)], 1)
+    s.add(ForceSpec(abi.HARMONIC_BOND, "HarmonicBond", indices=b, params=np.tile([0.38, 2000.0], (n - 1, 1))))
+    it = IntegratorSpec(300.0, 2.0, 0.005, 31)
+    x = np.stack([0.38 * np.arange(n), 0.05 * np.sin(np.arange(n)), np.zeros(n)], 1)
+    v = np.random.default_rng(8).normal(scale=0.2, size=x.shape)
+    with _ctx(s, it, x, resident=resident) as c:
+        assert (c.resident_threads() > 0) == (resident != 0)
+        c.set_velocities(v)
+        got = []
+        for _ in range(4):
+            assert c.step(500) == abi.OCM_OK
+            got.append(c.get_state(abi.STATE_POSITIONS | abi.STATE_VELOCITIES))
+    xo, vo = x, v
+    for k in range(4):
+        xo, vo = oracle.baoab(s, it, xo, vo, 500 * k, 500)
+        assert got[k]["step"] == 500 * (k + 1)
+        np.testing.assert_allclose(got[k]["positions"], xo, rtol=0, atol=1e-4)
+        np.testing.assert_allclose(got[k]["velocities"], vo, rtol=0, atol=1e-2)
