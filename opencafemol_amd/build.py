@@ -46,7 +46,8 @@ def build_hip(verbose=False):
             if p.returncode != 0:
                 raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
         tmp_lib = os.path.join(tmp, "libocm.so")
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-pthread", "-o", tmp_lib, *objs])
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-pthread", "-o", tmp_lib, *objs,
+              "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"])
         shutil.move(tmp_lib, os.path.join(LIB_DIR, "libocm.so"))
     if verbose:
         print("built", os.path.join(LIB_DIR, "libocm.so"))

@@ -1,5 +1,6 @@
 // engine.cpp -- packing of systems into device arrays and the chunked step loop.
 #include "engine.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -1175,6 +1176,7 @@ void Engine::carry_errors() {
 }
 
 void Engine::ensure_forces() {
+    const TraceRange trace("ocm.forces");
     if (!forces_dirty_) return;
     for (int attempt = 0; attempt < 8; ++attempt) {
         check_hip(hipMemsetAsync(d_.flags, 0, sizeof(int), stream_), "memset");
@@ -1283,6 +1285,7 @@ hipGraphExec_t Engine::chunk_graph(const ChunkPlan& p, bool may_capture) {
             (void)hipGraphExecDestroy(old->second.exec);   // the stream is idle between chunks
             graphs_.erase(old);
         }
+        const TraceRange trace("ocm.capture");
         hipGraph_t g;
         check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
         enqueue_chunk(p);
@@ -1317,6 +1320,7 @@ bool Engine::needs_prepare(long long n) const {
 }
 
 void Engine::prepare(long long n) {
+    const TraceRange trace("ocm.prepare");
     if (n < 0) invalid("negative step count");
     if (n == 0) return;
     ensure_forces();
@@ -1340,6 +1344,7 @@ void Engine::prepare(long long n) {
 }
 
 int Engine::run_chunk(int k) {
+    const TraceRange trace("ocm.chunk");
     for (int attempt = 0; attempt < 16; ++attempt) {
         if (timing_mode_ == 2) ensure_stamps();
         const ChunkPlan p = plan_chunk(k);
@@ -1416,6 +1421,7 @@ int Engine::run_chunk(int k) {
 // nothing rolls back (each workgroup rebuilds its own list when a bead has
 // moved more than skin/2), so a chunk only reports numeric errors.
 int Engine::step_resident(long long n) {
+    const TraceRange trace("ocm.resident_steps");
     int status = OCM_OK;
     long long remaining = n;
     while (remaining > 0) {
@@ -1440,6 +1446,7 @@ int Engine::step_resident(long long n) {
 }
 
 int Engine::step(long long n) {
+    const TraceRange trace("ocm.step");
     if (n < 0) invalid("negative step count");
     if (n == 0) return OCM_OK;
     ensure_forces();
@@ -1489,6 +1496,7 @@ std::vector<double> Engine::potentials() {
 //   - stop when the RMS force component of every system is <= tol (the
 //     reference's criterion) or after max_iter iterations.
 int Engine::minimize(double tol, int max_iter) {
+    const TraceRange trace("ocm.minimize");
     if (!(tol >= 0.0)) invalid("tolerance must be >= 0");
     if (max_iter < 0) invalid("max_iterations must be >= 0");
     if (max_iter == 0) max_iter = 100000;
@@ -1666,6 +1674,7 @@ static void to_host3(const double* dx, const double* dy, const double* dz, int n
 }
 
 void Engine::get_state(int s, int what, uint32_t groups, ocm_state* out) {
+    const TraceRange trace("ocm.get_state");
     const HostSystem& H = sys_.at(s);
     const size_t off = bead_begin_[s];
     sync();

@@ -2,6 +2,7 @@
 // job stream).  The reference runs one simulation per process
 // (src/OpenAICG2Plus.cpp:12-93); this loop keeps many resident on one device.
 #include "server.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -253,6 +254,7 @@ void Server::release_slots(Lane& L) {
 }
 
 void Server::repack(Lane& L) {
+    const TraceRange trace("ocm.server.repack");
     const double t = now_ms();
     // carry the state of jobs that stay over from the old batch
     if (L.engine) {
@@ -337,6 +339,7 @@ void Server::repack(Lane& L) {
 // Snapshots every bead position on the engine stream (ordered before the next
 // stretch), then copies the snapshot to pinned memory on the lane's copy stream.
 void Server::stage_frames(Lane& L, const std::vector<int>& systems) {
+    const TraceRange trace("ocm.server.frames");
     int slot = -1;
     {
         std::unique_lock<std::mutex> lk(mu_);
@@ -525,6 +528,7 @@ void Server::loop(Lane& L) {
         if (!failed) {
             gate_enter(false);
             try {
+                const TraceRange trace(L.id == 0 ? "ocm.server.stretch.lane0" : "ocm.server.stretch.lane1");
                 L.engine->step(n);
                 for (std::size_t s = 0; s < L.packed.size(); ++s) {
                     auto& j = L.packed[s];

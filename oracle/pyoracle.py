@@ -11,7 +11,8 @@ import numpy as np
 from opencafemol_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# OCM_ORACLE_LIB selects another build of the same sources (the sanitizer build)
+LIB = os.environ.get("OCM_ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 _lib = None
 
 
