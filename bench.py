@@ -7,6 +7,9 @@ HPS slab (100 chains x 140 beads = 14,000 beads, periodic 15 x 15 x 150 nm,
 dt 10 fs, 300 K), with more keys for the metric's other parts:
   - `go_chain_62`: ns/day of the 62-bead Go chain alone (20,000 timed steps;
     GPU only);
+  - `condensate_1m`: ns/day of the 1,000,000-bead condensate (1,000 timed
+    steps, 0.3 nm skin, rebuilds every 20 steps at most; GPU only), with the
+    force pass's roofline figures;
   - `replicas_256`: aggregate ns/day of 256 Go replicas (50-500 beads,
     280-400 K ladder) packed in one batch, against the same replicas stepped
     one at a time through single-system contexts;
@@ -616,6 +619,13 @@ def main():
     out["go_chain_62"] = {"ns_day": go["value"], "us_per_step": go["ms_per_step"] * 1000.0,
                           "steps": go["steps"], "resident_threads": go["kernels"].get("resident_threads", 0),
                           "config": go["config"]}
+    # the 1,000,000-bead condensate (its stated list parameters; GPU only)
+    c3_args = argparse.Namespace(**dict(vars(args), steps=1000, warmup=200, no_cpu=True))
+    c3 = single_run(c3_args, "condensate")
+    out["condensate_1m"] = {"ns_day": c3["value"], "us_per_step": c3["ms_per_step"] * 1000.0,
+                            "steps": c3["steps"], "roofline": c3["roofline"],
+                            "pair_us": c3["kernels"].get("pair_us"), "pairs_per_s": c3["kernels"].get("pairs_per_s"),
+                            "config": c3["config"]}
     out["replicas_256"] = replicas_run(args, args.replica_steps, 500,
                                        0 if args.no_serial else args.replica_serial_steps)
     out["served_stream"] = serve_run(args, args.serve_jobs, steps_range, args.serve_serial_frames,
