@@ -1770,10 +1770,49 @@ __device__ __forceinline__ void prune_row(const DevPairGroup& G, int row, int la
     const __amdgpu_buffer_rsrc_t rs32 = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float4*>(rec32), 0, F32 ? nrec * 16 : 0, 0x00020000);
     const __amdgpu_buffer_rsrc_t rs = rec_rsrc(rec, F32 ? 0 : nrec);
+    if (F32) {
+        // software-pipelined: round r's tests run while round r+1's gathers
+        // and round r+2's column indices are in flight (the kernel has few
+        // wavefronts per SIMD to hide a gather behind)
+        int cs[4], cn[4];
+        float4 qj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cs[u] = next_col(crow, lane + u * TEAM, cnt);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cn[u] = next_col(crow, 4 * TEAM + lane + u * TEAM, cnt);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) qj[u] = load_rec32(rs32, cs[u]);
+        for (int base = 0; base < cnt; base += 4 * TEAM) {
+            float4 qn[4];
+            int cn2[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) qn[u] = load_rec32(rs32, cn[u]);   // -1 reads past the range: zeros
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cn2[u] = next_col(crow, base + 8 * TEAM + lane + u * TEAM, cnt);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float dx = qj[u].x - qi.x, dy = qj[u].y - qi.y, dz = qj[u].z - qi.z;
+                dx = fmaf(-fbx, rintf(dx * fibx), dx);
+                dy = fmaf(-fby, rintf(dy * fiby), dy);
+                dz = fmaf(-fbz, rintf(dz * fibz), dz);
+                const bool in = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rin2f;
+                const bool valid = cs[u] >= 0, keep = valid & in;
+                const unsigned long long m =
+                    __builtin_amdgcn_ballot_w64(valid) & __builtin_amdgcn_ballot_w64(in) & team_lanes;
+                const int pos = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                if (keep) wrow[n + pos] = cs[u];
+                n += __popcll(m);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { cs[u] = cn[u]; cn[u] = cn2[u]; qj[u] = qn[u]; }
+        }
+    }
     int cn[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) cn[u] = next_col(crow, lane + u * TEAM, cnt);
-    for (int base = 0; base < cnt; base += 4 * TEAM) {
+    for (int base = 0; base < (F32 ? 0 : cnt); base += 4 * TEAM) {
         const int cs[4] = {cn[0], cn[1], cn[2], cn[3]};
         double px[4], py[4], pz[4];
         float4 qj[4];
