@@ -159,3 +159,16 @@ def test_replicas_256_resident_equals_standard():
     assert t0 == 0 and t1 == 1024
     for k, (a, b) in enumerate(zip(std, res)):
         _same(a, b, k)
+
+
+def test_far_apart_chains_use_the_fp64_build():
+    # an open system whose beads lie more than 1,000 nm from bead 0 (one chain
+    # moved 1,500 nm away): the in-kernel build tests its pairs in FP64
+    # instead of the FP32 relative coordinates; the bits still match the
+    # standard path's
+    s, it, x = configs.hps_open_cluster(n_chains=6, chain_len=40, extent=(6.0, 6.0), seed=3)
+    assert s.n <= 1024
+    x = x.copy()
+    x[:40, 0] += 1500.0   # the first chain, far from the rest
+    x = _relaxed(s, it, x, iters=200)
+    _both([(s, it, x)], 300)
