@@ -191,3 +191,29 @@ def test_server_shutdown_with_both_lanes_running():
         assert st["frames"] >= 2 and st["active"] == 4 and not ended, (st, ended, abi.load_library().ocm_last_error())
         t0 = time.perf_counter()
     assert time.perf_counter() - t0 < 60
+
+
+def test_server_many_small_jobs_beside_slabs():
+    # 60 resident-sized Go chains arriving over time beside two 1,500-bead
+    # slabs: every job completes with all its frames, and a sample of them
+    # carries the bits of its run alone
+    rng = np.random.default_rng(12)
+    jobs = []
+    reps = configs.go_replicas(60, n_min=50, n_max=150, seed=12)
+    for k, (s, it, x) in enumerate(reps):
+        n = int(rng.integers(3, 15)) * 100
+        jobs.append((s, it, x, None, n, 100))
+    for seed in (1, 2):
+        s, it, x = configs.hps_slab_of_size(1500, seed=seed)
+        with Context(s, it) as c:
+            c.set_positions(x)
+            c.minimize(tolerance=50.0, max_iterations=400)
+            x = c.get_state(abi.STATE_POSITIONS)["positions"]
+        jobs.insert(0, (s, it, x, None, 600, 200))
+    ids, frames, results, stats = _serve(jobs, stagger=40, quantum_steps=300)
+    assert stats["completed"] == len(jobs) and stats["failed"] == 0
+    assert stats["frames"] == sum(n // fi for *_, n, fi in jobs)
+    for jid in ids:
+        assert results[jid]["status"] == abi.OCM_OK
+    sample = [0, 1, 2, 17, 33, 61]
+    _assert_same([jobs[k] for k in sample], [ids[k] for k in sample], frames, results)
