@@ -43,10 +43,13 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "ns/day (single system + 256-replica aggregate); p50/p99 job latency"
-# Counter summaries of the force pass on the slab (profiles/): HBM bytes per
-# inner-list entry (FETCH_SIZE / WRITE_SIZE, calibrated against a copy kernel)
-# and VALU wave-instructions per dispatch (SQ_INSTS_VALU).
-TRAFFIC_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_traffic.json")}
+# Counter summaries of the force pass (profiles/): HBM bytes per inner-list
+# entry on the slab and on the condensate (FETCH_SIZE / WRITE_SIZE, calibrated
+# against a copy kernel) and VALU wave-instructions per dispatch on the slab
+# (SQ_INSTS_VALU).
+TRAFFIC_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_traffic.json"),
+                 "hps_condensate_1000x1000": os.path.join(ROOT, "profiles",
+                                                          "k_pair_hps_condensate_1000000_traffic.json")}
 VALU_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_valu.json")}
 HBM_PEAK_GBS = 8000.0
 # VALU issue ceiling: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
