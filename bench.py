@@ -44,13 +44,14 @@ import numpy as np  # noqa: E402
 
 METRIC = "ns/day (single system + 256-replica aggregate); p50/p99 job latency"
 # Counter summaries of the force pass (profiles/): HBM bytes per inner-list
-# entry on the slab and on the condensate (FETCH_SIZE / WRITE_SIZE, calibrated
-# against a copy kernel) and VALU wave-instructions per dispatch on the slab
-# (SQ_INSTS_VALU).
+# entry (FETCH_SIZE / WRITE_SIZE, calibrated against a copy kernel) and VALU
+# wave-instructions per entry (SQ_INSTS_VALU), on the slab and on the
+# condensate.
 TRAFFIC_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_traffic.json"),
                  "hps_condensate_1000x1000": os.path.join(ROOT, "profiles",
                                                           "k_pair_hps_condensate_1000000_traffic.json")}
-VALU_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_valu.json")}
+VALU_FILES = {"hps_slab_100x140": os.path.join(ROOT, "profiles", "k_pair_hps_slab_14000_valu.json"),
+              "hps_condensate_1000x1000": os.path.join(ROOT, "profiles", "k_pair_hps_condensate_1000000_valu.json")}
 HBM_PEAK_GBS = 8000.0
 # VALU issue ceiling: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
 # SIMD every 4 cycles (16 lanes wide); x 64 lanes x 2 flops per FMA this is
@@ -627,6 +628,7 @@ def main():
     c3 = single_run(c3_args, "condensate")
     out["condensate_1m"] = {"ns_day": c3["value"], "us_per_step": c3["ms_per_step"] * 1000.0,
                             "steps": c3["steps"], "roofline": c3["roofline"],
+                            "valu_roofline": c3["valu_roofline"],
                             "pair_us": c3["kernels"].get("pair_us"), "pairs_per_s": c3["kernels"].get("pairs_per_s"),
                             "config": c3["config"]}
     out["replicas_256"] = replicas_run(args, args.replica_steps, 500,
